@@ -1,0 +1,135 @@
+/*
+ * oracle.h -- FP64 CPU oracle (TEST INFRASTRUCTURE ONLY).
+ *
+ * This library is a plain-C restatement of the reference's algorithms for the
+ * cardiac cell-model hot path.  It exists only to check the HIP product path:
+ * it is linked by tests/, by __graft_entry__.smoke() and by bench.py's
+ * cpu_baseline leg, and by nothing else.  The product (the package's C-ABI
+ * library) never calls into it.
+ *
+ * Pinning (what checks each part against something other than itself):
+ *   ord_*  ORd-CiPA v1 2017 init/drug block/RHS -> golden vectors produced by the
+ *          compiled reference model (oracle/_ref, tests/golden/ord_*.bin).
+ *   tt_*   TT06 endo/epi/M init/RHS/Rush-Larsen -> golden vectors + 1-beat RL
+ *          trajectories from the compiled reference model.
+ *   ord11_*  ORd 2011 (static Hill) RHS + RL -> golden vectors from the compiled
+ *          reference model (CEPPatch_CiPA_Sim copy).
+ *   bdf_*  CVODE-style BDF: the reference only ships CVODE as a binary archive, so
+ *          CVODE parity is UNPINNED.  The restatement is pinned against textbook
+ *          problems (closed-form linear systems, Robertson, order-of-accuracy) and
+ *          against fine-step integrations of the same cell model.
+ *   bm_*   biomarker state machine -> synthetic closed-form traces with known
+ *          APD/CaD/dVdt answers.
+ *   fem_*  Q1 element matrices -> closed forms; CG -> dense direct solve.
+ */
+#ifndef CEP_ORACLE_H
+#define CEP_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------- ORd-CiPA v1 2017 (CEP5Cell/cellmodels/ohara_rudy_cipa_v1_2017.cpp) */
+#define ORD_NS 52
+#define ORD_NC 206
+#define ORD_NA 200
+void ord_init_consts(double *C, double *S, double celltype);
+/* herg[0..13] = IC50/Hill pairs, herg[14..18] = Kmax, Ku, n, halfmax, Vhalf */
+void ord_apply_drug(double *C, double *S, double conc, const double *herg);
+void ord_rates(double t, const double *C, double *R, const double *S, double *A);
+
+/* ---------------- TT06 (CEP4Cell/cellmodels/ten_tusscher_model_2006_IK1Ko_*_units.cpp) */
+#define TT_NS 19
+#define TT_NC 53
+#define TT_NA 70
+/* celltype: 0 endo, 1 epi, 2 M */
+void tt_init_consts(double *C, double *S, int celltype);
+void tt_rates(double t, const double *C, double *R, const double *S, double *A, int celltype);
+void tt_rush_larsen(double dt, const double *C, const double *R, double *S, const double *A);
+
+/* ---------------- ORd 2011 static-Hill (CEPPatch_CiPA_Sim/patches/Ohara_Rudy_2011.cpp) */
+#define ORD11_NS 44
+#define ORD11_NC 170
+#define ORD11_NA 200
+
+/* ---------------- CVODE-style BDF */
+typedef void (*bdf_rhs_fn)(double t, const double *y, double *ydot, void *user);
+
+typedef struct {
+  double rtol, atol, hmax, hmin;
+  int maxord, max_steps;   /* max_steps per call of bdf_advance */
+} bdf_opts_t;
+
+typedef struct {
+  long nst, nfe, nsetups, nje, nni, ncfn, netf;
+} bdf_stats_t;
+
+typedef struct bdf_mem bdf_mem_t;
+bdf_mem_t *bdf_create(int n, bdf_rhs_fn f, void *user, const bdf_opts_t *o);
+void bdf_free(bdf_mem_t *m);
+int bdf_init(bdf_mem_t *m, double t0, const double *y0);
+/* advance to tout in normal mode: steps past tout and interpolates y(tout). returns 0 ok, <0 error */
+int bdf_advance(bdf_mem_t *m, double tout, double *yout);
+void bdf_get_stats(const bdf_mem_t *m, bdf_stats_t *s);
+double bdf_tn(const bdf_mem_t *m);
+double bdf_h(const bdf_mem_t *m);
+int bdf_q(const bdf_mem_t *m);
+
+/* ---------------- biomarkers (CEPCell_NEO/modules/cipa_drug_bench.cpp:210-381) */
+typedef struct {
+  double qnet, inal_auc, ical_auc, dvmdt_repol, dvmdt_max, vm_peak, vm_valley, vm_dia,
+         apd90, apd50, ca_peak, ca_valley, ca_dia, cad90, cad50;
+  int steepest_pace;
+} bm_result_t;
+
+typedef struct bm_state bm_state_t;
+/* nbeats = num_pace1, nwin = number of trailing beats scanned (reference: 250) */
+bm_state_t *bm_create(double bcl, double dt, double stim_start, int nbeats, int nwin, int print_freq);
+void bm_free(bm_state_t *b);
+void bm_begin(bm_state_t *b, double v0, double ca0);
+/* one output sample: iout already incremented, t = tcurr, dvdt = RATES[V],
+   qsum = INaL+ICaL+Ito+IKr+IKs+IK1, inal, ical */
+void bm_sample(bm_state_t *b, int iout, double t, double v, double ca, double dvdt,
+               double qsum, double inal, double ical);
+void bm_result(const bm_state_t *b, bm_result_t *out);
+
+/* ---------------- drivers */
+typedef struct {
+  double celltype, conc, bcl, stim_amp_scale, dt, rtol, atol;
+  double herg[19];
+  double gscale[7];   /* GNa, GNaL, PCa, GKr, GKs, GK1, Gto */
+  int nbeats, nwin, maxord;
+} ord_cell_cfg_t;
+
+/* Runs one ORd-CiPA cell through nbeats beats with the BDF, output every dt,
+   biomarkers over the trailing nwin beats.  Writes final states (52), the result,
+   optional last-beat Vm trace (vm_trace[nsamp], nsamp = bcl/dt) and stats. */
+int oracle_ord_run(const ord_cell_cfg_t *cfg, double *states_io, int use_states_in,
+                   bm_result_t *res, double *vm_trace, bdf_stats_t *st);
+
+typedef struct {
+  double bcl, dt, stim_start_unused;
+  int celltype, nbeats, nwin;
+  double conc;
+  double ic50[3], hill[3];      /* IKr, ICaL, INa */
+  double gscale[10];            /* GNa GK1 GKr GKs Gto GCaL GbNa GbCa GpCa GpK */
+} tt_cell_cfg_t;
+int oracle_tt_run(const tt_cell_cfg_t *cfg, double *states_io, int use_states_in,
+                  bm_result_t *res, double *vm_trace);
+
+/* threads: run ncell independent ORd cells round-robin over nthreads host threads */
+double oracle_ord_run_many(const ord_cell_cfg_t *cfgs, int ncell, int nthreads,
+                           bm_result_t *res, double *states_out);
+double oracle_tt_run_many(const tt_cell_cfg_t *cfgs, int ncell, int nthreads,
+                          bm_result_t *res, double *states_out);
+
+/* ---------------- tissue (fem.cpp / heart.cpp) */
+void fem_q1_element(double hx, double hy, double sxx, double sxy, double syy,
+                    double *Me /*16*/, double *Ke /*16*/);
+int cg_solve_csr(int n, const int *rowptr, const int *col, const double *val,
+                 const double *b, double *x, double rtol, int maxit, int jacobi);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,182 @@
+"""ctypes binding of the C oracle (oracle/liboracle.so) -- test infrastructure only.
+
+Used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+_LIB = None
+
+dp = C.POINTER(C.c_double)
+
+
+class BmResult(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "qnet", "inal_auc", "ical_auc", "dvmdt_repol", "dvmdt_max", "vm_peak", "vm_valley",
+        "vm_dia", "apd90", "apd50", "ca_peak", "ca_valley", "ca_dia", "cad90", "cad50")] + [
+        ("steepest_pace", C.c_int)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class BdfStats(C.Structure):
+    _fields_ = [(n, C.c_long) for n in ("nst", "nfe", "nsetups", "nje", "nni", "ncfn", "netf")]
+
+
+class OrdCellCfg(C.Structure):
+    _fields_ = [("celltype", C.c_double), ("conc", C.c_double), ("bcl", C.c_double),
+                ("stim_amp_scale", C.c_double), ("dt", C.c_double), ("rtol", C.c_double),
+                ("atol", C.c_double), ("herg", C.c_double * 19), ("gscale", C.c_double * 7),
+                ("nbeats", C.c_int), ("nwin", C.c_int), ("maxord", C.c_int)]
+
+
+class TtCellCfg(C.Structure):
+    _fields_ = [("bcl", C.c_double), ("dt", C.c_double), ("stim_start_unused", C.c_double),
+                ("celltype", C.c_int), ("nbeats", C.c_int), ("nwin", C.c_int),
+                ("conc", C.c_double), ("ic50", C.c_double * 3), ("hill", C.c_double * 3),
+                ("gscale", C.c_double * 10)]
+
+
+BDF_RHS = C.CFUNCTYPE(None, C.c_double, dp, dp, C.c_void_p)
+
+
+class BdfOpts(C.Structure):
+    _fields_ = [("rtol", C.c_double), ("atol", C.c_double), ("hmax", C.c_double),
+                ("hmin", C.c_double), ("maxord", C.c_int), ("max_steps", C.c_int)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(ORACLE_DIR, "liboracle.so")
+        if not os.path.exists(path):
+            subprocess.run(["make", "-C", ORACLE_DIR, "liboracle.so"], check=True,
+                           stdout=subprocess.DEVNULL)
+        L = C.CDLL(path)
+        L.ord_init_consts.argtypes = [dp, dp, C.c_double]
+        L.ord_apply_drug.argtypes = [dp, dp, C.c_double, dp]
+        L.ord_rates.argtypes = [C.c_double, dp, dp, dp, dp]
+        L.tt_init_consts.argtypes = [dp, dp, C.c_int]
+        L.tt_rates.argtypes = [C.c_double, dp, dp, dp, dp, C.c_int]
+        L.tt_rush_larsen.argtypes = [C.c_double, dp, dp, dp, dp]
+        L.oracle_ord_run.argtypes = [C.POINTER(OrdCellCfg), dp, C.c_int, C.POINTER(BmResult), dp,
+                                     C.POINTER(BdfStats)]
+        L.oracle_tt_run.argtypes = [C.POINTER(TtCellCfg), dp, C.c_int, C.POINTER(BmResult), dp]
+        L.oracle_ord_run_many.argtypes = [C.POINTER(OrdCellCfg), C.c_int, C.c_int,
+                                          C.POINTER(BmResult), dp]
+        L.oracle_ord_run_many.restype = C.c_double
+        L.oracle_tt_run_many.argtypes = [C.POINTER(TtCellCfg), C.c_int, C.c_int,
+                                         C.POINTER(BmResult), dp]
+        L.oracle_tt_run_many.restype = C.c_double
+        L.bdf_create.argtypes = [C.c_int, BDF_RHS, C.c_void_p, C.POINTER(BdfOpts)]
+        L.bdf_create.restype = C.c_void_p
+        L.bdf_init.argtypes = [C.c_void_p, C.c_double, dp]
+        L.bdf_advance.argtypes = [C.c_void_p, C.c_double, dp]
+        L.bdf_free.argtypes = [C.c_void_p]
+        L.bdf_get_stats.argtypes = [C.c_void_p, C.POINTER(BdfStats)]
+        L.bdf_q.argtypes = [C.c_void_p]
+        L.bdf_h.argtypes = [C.c_void_p]
+        L.bdf_h.restype = C.c_double
+        L.bm_create.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int]
+        L.bm_create.restype = C.c_void_p
+        L.bm_begin.argtypes = [C.c_void_p, C.c_double, C.c_double]
+        L.bm_sample.argtypes = [C.c_void_p, C.c_int] + [C.c_double] * 7
+        L.bm_result.argtypes = [C.c_void_p, C.POINTER(BmResult)]
+        L.bm_free.argtypes = [C.c_void_p]
+        L.fem_q1_element.argtypes = [C.c_double] * 5 + [dp, dp]
+        L.cg_solve_csr.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), dp, dp, dp,
+                                   C.c_double, C.c_int, C.c_int]
+        _LIB = L
+    return _LIB
+
+
+def ptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(dp)
+
+
+def ord_init(celltype, conc=0.0, herg=None):
+    Cc = np.zeros(206); S = np.zeros(52)
+    lib().ord_init_consts(ptr(Cc), ptr(S), float(celltype))
+    if herg is not None:
+        h = np.ascontiguousarray(herg, dtype=np.float64)
+        lib().ord_apply_drug(ptr(Cc), ptr(S), float(conc), ptr(h))
+    else:
+        S[43] = conc
+    return Cc, S
+
+
+def ord_rates(t, Cc, S):
+    R = np.zeros(52); A = np.zeros(200)
+    S = np.ascontiguousarray(S, dtype=np.float64)
+    lib().ord_rates(float(t), ptr(Cc), ptr(R), ptr(S), ptr(A))
+    return R, A
+
+
+def tt_init(celltype):
+    Cc = np.zeros(53); S = np.zeros(19)
+    lib().tt_init_consts(ptr(Cc), ptr(S), int(celltype))
+    return Cc, S
+
+
+def tt_rates(t, Cc, S, celltype):
+    R = np.zeros(19); A = np.zeros(70)
+    S = np.ascontiguousarray(S, dtype=np.float64)
+    lib().tt_rates(float(t), ptr(Cc), ptr(R), ptr(S), ptr(A), int(celltype))
+    return R, A
+
+
+def make_ord_cfg(celltype=0, conc=0.0, herg=None, gscale=None, bcl=1000.0, nbeats=1, nwin=None,
+                 dt=0.1, rtol=1e-6, atol=1e-8, stim_amp_scale=1.0, maxord=5):
+    c = OrdCellCfg()
+    c.celltype = celltype; c.conc = conc; c.bcl = bcl; c.stim_amp_scale = stim_amp_scale
+    c.dt = dt; c.rtol = rtol; c.atol = atol; c.nbeats = nbeats
+    c.nwin = nbeats if nwin is None else nwin; c.maxord = maxord
+    h = np.zeros(19) if herg is None else herg
+    for k in range(19):
+        c.herg[k] = float(h[k])
+    g = np.ones(7) if gscale is None else gscale
+    for k in range(7):
+        c.gscale[k] = float(g[k])
+    return c
+
+
+def ord_run(cfg: OrdCellCfg, states=None, trace=False):
+    S = np.zeros(52) if states is None else np.array(states, dtype=np.float64)
+    res = BmResult(); st = BdfStats()
+    nsamp = int(cfg.bcl / cfg.dt)
+    vm = np.zeros(nsamp) if trace else None
+    rc = lib().oracle_ord_run(C.byref(cfg), ptr(S), 0 if states is None else 1, C.byref(res),
+                              ptr(vm) if trace else None, C.byref(st))
+    return rc, S, res.as_dict(), vm, {n: getattr(st, n) for n, _ in st._fields_}
+
+
+def make_tt_cfg(celltype=0, conc=0.0, ic50=(0, 0, 0), hill=(0, 0, 0), gscale=None, bcl=1000.0,
+                nbeats=1, nwin=None, dt=0.02):
+    c = TtCellCfg()
+    c.bcl = bcl; c.dt = dt; c.celltype = int(celltype); c.nbeats = nbeats
+    c.nwin = nbeats if nwin is None else nwin; c.conc = conc
+    for k in range(3):
+        c.ic50[k] = float(ic50[k]); c.hill[k] = float(hill[k])
+    g = np.ones(10) if gscale is None else gscale
+    for k in range(10):
+        c.gscale[k] = float(g[k])
+    return c
+
+
+def tt_run(cfg: TtCellCfg, states=None, trace=False):
+    S = np.zeros(19) if states is None else np.array(states, dtype=np.float64)
+    res = BmResult()
+    nsamp = int(round(cfg.bcl / cfg.dt))
+    vm = np.zeros(nsamp) if trace else None
+    lib().oracle_tt_run(C.byref(cfg), ptr(S), 0 if states is None else 1, C.byref(res),
+                        ptr(vm) if trace else None)
+    return S, res.as_dict(), vm
