@@ -1,0 +1,228 @@
+"""MI355X-native engine for the cardiac cell-model integration of the CiPA drug-population path.
+
+The compute path is the C-ABI library ``libcepgpu.so`` (HIP kernels for gfx950, declared in
+``include/cep_cellsim.h``).  This module is a thin ctypes binding of that ABI plus a
+host-side mirror of the reference's CellML cell-model interface (``CellBatch``).  There is
+no CPU fallback: if the library is missing the import of the binding raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import workload  # noqa: F401  (re-exported)
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libcepgpu.so")
+
+MODEL_ORD_CIPA = 1
+MODEL_TT06 = 2
+PREC_FP64 = 0
+PREC_MIXED = 1
+
+BM_FIELDS = ("qnet", "inal_auc", "ical_auc", "dvmdt_repol", "dvmdt_max", "vm_peak", "vm_valley",
+             "vm_dia", "apd90", "apd50", "ca_peak", "ca_valley", "ca_dia", "cad90", "cad50",
+             "steepest_pace")
+NBM = len(BM_FIELDS)
+STATUS_NAMES = {0: "ok", 1: "conv_fail", 2: "err_fail", 3: "nonfinite", 4: "max_steps"}
+
+EXPORTED = (
+    "cep_batch_create", "cep_batch_destroy", "cep_last_error", "cep_batch_info", "cep_set_cells",
+    "cep_set_run_config", "cep_init_consts", "cep_get_constants", "cep_compute_rates",
+    "cep_run_beats", "cep_run_beats_async", "cep_sync", "cep_get_states", "cep_set_states",
+    "cep_get_biomarkers", "cep_get_status", "cep_get_stats", "cep_enable_vm_trace",
+    "cep_get_vm_trace", "cep_kernel_time",
+)
+
+_lib = None
+dp = C.POINTER(C.c_double)
+
+
+class CepError(RuntimeError):
+    pass
+
+
+def library(path: str | None = None):
+    """Load libcepgpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise CepError(f"{p} is missing: build it with __graft_entry__.build() "
+                       "(the HIP path has no CPU fallback)")
+    L = C.CDLL(p)
+    L.cep_batch_create.argtypes = [C.c_int, C.c_int, C.c_int]
+    L.cep_batch_create.restype = C.c_void_p
+    L.cep_batch_destroy.argtypes = [C.c_void_p]
+    L.cep_batch_destroy.restype = None
+    L.cep_last_error.argtypes = [C.c_void_p]
+    L.cep_last_error.restype = C.c_char_p
+    L.cep_batch_info.argtypes = [C.c_void_p] + [C.POINTER(C.c_int)] * 5
+    L.cep_set_cells.argtypes = [C.c_void_p, C.c_int, C.c_int, dp, dp, dp, C.c_int, dp, C.c_int, dp]
+    L.cep_set_run_config.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int,
+                                     C.c_int, C.c_int, C.c_double]
+    L.cep_init_consts.argtypes = [C.c_void_p]
+    L.cep_get_constants.argtypes = [C.c_void_p, C.c_int, dp]
+    L.cep_compute_rates.argtypes = [C.c_void_p, C.c_double, dp, dp, dp]
+    L.cep_run_beats.argtypes = [C.c_void_p, C.c_int]
+    L.cep_run_beats_async.argtypes = [C.c_void_p, C.c_int]
+    L.cep_sync.argtypes = [C.c_void_p]
+    L.cep_get_states.argtypes = [C.c_void_p, dp]
+    L.cep_set_states.argtypes = [C.c_void_p, dp]
+    L.cep_get_biomarkers.argtypes = [C.c_void_p, dp]
+    L.cep_get_status.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+    L.cep_get_stats.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
+    L.cep_enable_vm_trace.argtypes = [C.c_void_p, C.c_int]
+    L.cep_get_vm_trace.argtypes = [C.c_void_p, dp, C.POINTER(C.c_int)]
+    L.cep_kernel_time.argtypes = [C.c_void_p, dp, dp]
+    if path is None:
+        _lib = L
+    return L
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(dp)
+
+
+class CellBatch:
+    """Host mirror of the reference's CellML model object, for N cells at once.
+
+    ``init_consts`` ~ ``initConsts(celltype, conc, herg, drug)`` per cell, ``compute_rates``
+    ~ ``computeRates(TIME, CONSTANTS, RATES, STATES, ALGEBRAIC)``, ``run_beats`` replaces the
+    per-cell CVode / solveAnalytical loops of the drivers, ``states``/``constants`` expose the
+    STATES / CONSTANTS arrays in the reference layout.
+    """
+
+    def __init__(self, model: int, ncells: int, precision: int = PREC_FP64):
+        self.L = library()
+        h = self.L.cep_batch_create(model, ncells, precision)
+        if not h:
+            raise CepError(f"cep_batch_create({model}, {ncells}, {precision}) failed")
+        self.h = h
+        self.model = model
+        self.n = ncells
+        self.precision = precision
+        info = [C.c_int() for _ in range(5)]
+        self.L.cep_batch_info(h, *[C.byref(x) for x in info])
+        _, _, self.nstates, self.nconsts, self.nalg = (x.value for x in info)
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise CepError(self.L.cep_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.cep_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- configuration
+    def set_cells(self, celltype=None, conc=None, herg=None, gscale=None, bcl=None, first=0):
+        def arr(x, w=None):
+            if x is None:
+                return None, 0
+            a = np.ascontiguousarray(x, dtype=np.float64)
+            return a, (a.shape[1] if a.ndim == 2 else 0)
+        ct, _ = arr(celltype)
+        cc, _ = arr(conc)
+        hg, hs = arr(herg)
+        gs, gss = arr(gscale)
+        bc, _ = arr(bcl)
+        count = next(len(a) for a in (ct, cc, hg, gs, bc) if a is not None)
+        self._chk(self.L.cep_set_cells(self.h, first, count, _p(ct), _p(cc), _p(hg), hs, _p(gs), gss, _p(bc)))
+
+    def set_run_config(self, dt=None, rtol=1e-6, atol=1e-8, maxord=5, nbeats_total=1, nwin=250,
+                       stim_amp_scale=1.0):
+        if dt is None:
+            dt = 0.1 if self.model == MODEL_ORD_CIPA else 0.02
+        self._chk(self.L.cep_set_run_config(self.h, dt, rtol, atol, maxord, nbeats_total, nwin, stim_amp_scale))
+
+    def init_consts(self):
+        self._chk(self.L.cep_init_consts(self.h))
+
+    # ---- CellML arrays
+    def constants(self, cell: int) -> np.ndarray:
+        out = np.zeros(self.nconsts)
+        self._chk(self.L.cep_get_constants(self.h, cell, _p(out)))
+        return out
+
+    def compute_rates(self, t: float, states: np.ndarray, algebraic: bool = False):
+        s = np.ascontiguousarray(states, dtype=np.float64).reshape(self.n, self.nstates)
+        r = np.zeros_like(s)
+        a = np.zeros((self.n, self.nalg)) if algebraic else None
+        self._chk(self.L.cep_compute_rates(self.h, float(t), _p(s), _p(r), _p(a)))
+        return (r, a) if algebraic else r
+
+    def states(self) -> np.ndarray:
+        out = np.zeros((self.n, self.nstates))
+        self._chk(self.L.cep_get_states(self.h, _p(out)))
+        return out
+
+    def set_states(self, s):
+        a = np.ascontiguousarray(s, dtype=np.float64).reshape(self.n, self.nstates)
+        self._chk(self.L.cep_set_states(self.h, _p(a)))
+
+    # ---- integration
+    def run_beats(self, nbeats: int):
+        self._chk(self.L.cep_run_beats(self.h, nbeats))
+
+    def run_beats_async(self, nbeats: int):
+        self._chk(self.L.cep_run_beats_async(self.h, nbeats))
+
+    def sync(self):
+        self._chk(self.L.cep_sync(self.h))
+
+    def enable_vm_trace(self, on=True):
+        self._chk(self.L.cep_enable_vm_trace(self.h, 1 if on else 0))
+
+    def vm_trace(self):
+        ns = C.c_int()
+        self._chk(self.L.cep_get_vm_trace(self.h, None, C.byref(ns)))
+        out = np.zeros((self.n, ns.value))
+        self._chk(self.L.cep_get_vm_trace(self.h, _p(out), C.byref(ns)))
+        return out
+
+    def biomarkers(self) -> dict:
+        out = np.zeros((self.n, NBM))
+        self._chk(self.L.cep_get_biomarkers(self.h, _p(out)))
+        return {k: out[:, i].copy() for i, k in enumerate(BM_FIELDS)}
+
+    def biomarker_array(self) -> np.ndarray:
+        out = np.zeros((self.n, NBM))
+        self._chk(self.L.cep_get_biomarkers(self.h, _p(out)))
+        return out
+
+    def status(self) -> np.ndarray:
+        out = np.zeros(self.n, dtype=np.int32)
+        self._chk(self.L.cep_get_status(self.h, out.ctypes.data_as(C.POINTER(C.c_int))))
+        return out
+
+    def stats(self) -> np.ndarray:
+        out = np.zeros((self.n, 8), dtype=np.int64)
+        self._chk(self.L.cep_get_stats(self.h, out.ctypes.data_as(C.POINTER(C.c_longlong))))
+        return out
+
+    def kernel_time(self):
+        avg, tot = C.c_double(), C.c_double()
+        n = self.L.cep_kernel_time(self.h, C.byref(avg), C.byref(tot))
+        return n, avg.value, tot.value
+
+
+def qinward(inal_auc_drug, ical_auc_drug, inal_auc_ctl, ical_auc_ctl):
+    """qInward = (INaL_AUC_drug/INaL_AUC_control + ICaL_AUC_drug/ICaL_AUC_control) / 2
+    (CEPCell_NEO/modules/cipa_drug_bench.cpp:372-377)."""
+    return 0.5 * (np.asarray(inal_auc_drug) / inal_auc_ctl + np.asarray(ical_auc_drug) / ical_auc_ctl)

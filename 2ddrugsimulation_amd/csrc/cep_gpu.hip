@@ -1,0 +1,843 @@
+// cep_gpu.hip -- kernels and C ABI of the cardiac cell-population engine (gfx950).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/cep_cellsim.h"
+#include "bdf_ord.hpp"
+#include "tt06_model.hpp"
+
+namespace cep {
+namespace ord {
+__constant__ MarkovTable g_markov;
+}
+
+using ordbdf::Buffers;
+using ordbdf::Machine;
+
+// ============================================================================ kernels
+
+struct PrmAcc {
+  const double *p;
+  int pitch;
+  CEP_D double operator()(int k) const { return p[(size_t)k * pitch]; }
+};
+
+// computeRates for a batch: AoS states in, AoS rates / algebraic slots out
+constexpr int ORD_NALG = 200;
+template <typename R>
+__global__ void __launch_bounds__(64) ord_rates_kernel(int n, int pitch, double t, const double *__restrict__ yin,
+                                 double *__restrict__ fout, double *__restrict__ alg, const double *__restrict__ prm)
+{
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  double y[ord::NS], f[ord::NS];
+#pragma unroll
+  for (int i = 0; i < ord::NS; ++i) y[i] = yin[(size_t)c * ord::NS + i];
+  ord::Currents cur;
+  const PrmAcc P{prm + c, pitch};
+  ord::rhs<R>(t, y, f, P, &cur);
+#pragma unroll
+  for (int i = 0; i < ord::NS; ++i) fout[(size_t)c * ord::NS + i] = f[i];
+  if (alg) {
+    double *a = alg + (size_t)c * ORD_NALG;
+    a[0] = cur.Istim; a[58] = cur.INa; a[60] = cur.INaL; a[66] = cur.Ito; a[83] = cur.ICaL;
+    a[84] = cur.ICaNa; a[87] = cur.ICaK; a[90] = cur.IKr; a[96] = cur.IKs; a[98] = cur.IK1;
+    a[130] = cur.INaCa_i; a[160] = cur.INaCa_ss; a[179] = cur.INaK; a[181] = cur.IKb;
+    a[185] = cur.INab; a[189] = cur.ICab; a[190] = cur.IpCa; a[193] = cur.Jrel; a[198] = cur.Jup;
+    a[42] = cur.CaMKa;
+  }
+}
+
+constexpr int TT_NALG = 70;
+template <typename R>
+__global__ void __launch_bounds__(64) tt_rates_kernel(int n, int pitch, double t, const double *__restrict__ yin,
+                                double *__restrict__ fout, double *__restrict__ alg, const double *__restrict__ prm)
+{
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  double y[tt::NS], f[tt::NS];
+#pragma unroll
+  for (int i = 0; i < tt::NS; ++i) y[i] = yin[(size_t)c * tt::NS + i];
+  tt::Currents cur;
+  const PrmAcc P{prm + c, pitch};
+  tt::rhs<R>(t, y, f, P, &cur);
+#pragma unroll
+  for (int i = 0; i < tt::NS; ++i) fout[(size_t)c * tt::NS + i] = f[i];
+  if (alg) {
+    double *a = alg + (size_t)c * TT_NALG;
+    a[47] = cur.IK1; a[54] = cur.Ito; a[48] = cur.IKr; a[49] = cur.IKs; a[52] = cur.ICaL;
+    a[55] = cur.INaK; a[50] = cur.INa; a[51] = cur.IbNa; a[56] = cur.INaCa; a[53] = cur.IbCa;
+    a[58] = cur.IpK; a[57] = cur.IpCa; a[12] = cur.Istim;
+  }
+}
+
+struct RunCfg {
+  double dt;
+  int nbeats_total, nwin, print_freq;
+};
+
+CEP_D void bm_config(Biomarker &bm, double bcl, double stim_start, const RunCfg &rc, int cap)
+{
+  bm.bcl = bcl;
+  bm.dt = rc.dt;
+  bm.stim_start = stim_start;
+  bm.beat_steps = (int)(bcl / rc.dt);
+  const int nwin = rc.nwin > rc.nbeats_total ? rc.nbeats_total : rc.nwin;
+  bm.win_start = rc.nbeats_total - nwin;
+  bm.print_freq = rc.print_freq;
+  bm.cap = cap;
+}
+
+// ---- ORd-CiPA: persistent BDF + biomarkers, one cell per lane
+template <typename R>
+__global__ void __launch_bounds__(64) ord_bdf_kernel(Buffers B, RunCfg rc, int nbeats)
+{
+  using namespace ordbdf;
+  const int c = blockIdx.x * 64 + threadIdx.x;
+  if (c >= B.ncell) return;
+  Machine<R> m(B, c);
+  m.load();
+  m.nstloc = 0;
+  if (m.status != ST_OK) return;
+  Biomarker bm;
+  const double bcl = m.P(ord::OP_BCL);
+  bm_config(bm, bcl, m.P(ord::OP_STIM_START), rc, B.cap);
+  double *ct = B.ct + c, *cv = B.cv + c;
+  double y[N], f[N], treq = 0.0;
+  bool need_rhs;
+  if (m.phase == PH_FRESH) {
+    // CVodeInit at t0 = 0 with the initial states
+#pragma unroll
+    for (int i = 0; i < N; ++i) { y[i] = m.Y(i); m.ZN(0, i) = y[i]; }
+    m.tn = 0.0; m.tout = rc.dt; m.q = 1; m.qprime = 1; m.qwait = 2; m.etamax = ETAMX1;
+    m.h = 0.0; m.hprime = 0.0; m.eta = 1.0; m.nst = m.nstlp = m.nstlj = 0; m.crate = 1.0;
+    m.gamrat = 1.0; m.gammap = 0.0; m.saved_tq5 = 0.0; m.jcur = 0; m.iout = 0;
+    m.nfe = m.nje = m.nsetups = m.nni = m.ncfn = m.netf = 0;
+    for (int i = 0; i < 7; ++i) m.tau[i] = 0.0;
+    m.tq1 = m.tq2 = m.tq3 = m.tq4 = m.tq5 = 0.0;
+    bm.begin(y[0], y[9]);
+    treq = 0.0;
+    m.phase = PH_INIT_F;
+    need_rhs = true;
+  } else {
+    bm.load(B.bmd, B.bmi, B.pitch, c);
+    m.phase = PH_READY;
+    need_rhs = false;
+  }
+  m.iout_end = m.iout + nbeats * bm.beat_steps;
+  const int trace_start = m.iout_end - bm.beat_steps;
+  const PrmAcc P{B.prm + c, B.pitch};
+  for (;;) {
+    if (need_rhs) {
+      ord::Currents cur;
+      ord::rhs<R>(treq, y, f, P, &cur);
+      bool fin = true;
+#pragma unroll
+      for (int i = 0; i < N; ++i) fin = fin && isfinite(f[i]);
+      if (!fin && m.phase != PH_JAC) { m.status = ST_NONFINITE; m.phase = PH_DONE; break; }
+      if (m.phase == PH_OUTPUT) {
+        // CVode returned y(tout); computeRates at tout feeds the biomarkers
+        m.iout++;
+        bm.sample(m.iout, m.tout, y[0], y[9], f[0],
+                  cur.INaL + cur.ICaL + cur.Ito + cur.IKr + cur.IKs + cur.IK1, cur.INaL, cur.ICaL,
+                  ct, cv, B.pitch);
+        if (B.vtrace && m.iout > trace_start && m.iout - trace_start <= B.tcap)
+          B.vtrace[(size_t)(m.iout - trace_start - 1) * B.pitch + c] = y[0];
+        m.tout += rc.dt;
+        m.nstloc = 0;
+        m.phase = PH_READY;
+        if (m.iout >= m.iout_end) {
+#pragma unroll
+          for (int i = 0; i < N; ++i) m.Y(i) = y[i];
+          break;
+        }
+      }
+    }
+    need_rhs = true;
+    if (!m.consume(f, y, treq)) break;
+  }
+  m.store();
+  bm.store(B.bmd, B.bmi, B.pitch, c);
+}
+
+// ---- TT06: fixed-dt Rush-Larsen + biomarkers, one cell per lane
+struct TtBuffers {
+  int ncell, pitch, cap, tcap;
+  double *y, *prm, *bmd, *ct, *cv, *tcur, *vtrace;
+  int *bmi, *iout, *status;
+};
+
+template <typename R>
+__global__ void __launch_bounds__(256) tt_rl_kernel(TtBuffers B, RunCfg rc, int nbeats)
+{
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= B.ncell) return;
+  const PrmAcc P{B.prm + c, B.pitch};
+  double y[tt::NS];
+#pragma unroll
+  for (int i = 0; i < tt::NS; ++i) y[i] = B.y[(size_t)i * B.pitch + c];
+  Biomarker bm;
+  bm_config(bm, P(tt::TP_BCL), P(tt::TP_STIM_START), rc, B.cap);
+  int iout = B.iout[c];
+  double tcurr = B.tcur[c];
+  if (iout == 0) bm.begin(y[0], y[3]); else bm.load(B.bmd, B.bmi, B.pitch, c);
+  double *ct = B.ct + c, *cv = B.cv + c;
+  const int steps = nbeats * bm.beat_steps;
+  const int trace_start = iout + steps - bm.beat_steps;
+  for (int s = 0; s < steps; ++s) {
+    tt::Currents cur;
+    double dvdt;
+    tt::rush_larsen<R>(tcurr, rc.dt, y, P, dvdt, &cur);
+    tcurr += rc.dt;
+    iout++;
+    bm.sample(iout, tcurr, y[0], y[3], dvdt, cur.ICaL + cur.Ito + cur.IKr + cur.IKs + cur.IK1, 0.0,
+              cur.ICaL, ct, cv, B.pitch);
+    if (B.vtrace && iout > trace_start && iout - trace_start <= B.tcap)
+      B.vtrace[(size_t)(iout - trace_start - 1) * B.pitch + c] = y[0];
+  }
+  bool fin = true;
+#pragma unroll
+  for (int i = 0; i < tt::NS; ++i) { B.y[(size_t)i * B.pitch + c] = y[i]; fin = fin && isfinite(y[i]); }
+  if (!fin) B.status[c] = ST_NONFINITE;
+  B.iout[c] = iout;
+  B.tcur[c] = tcurr;
+  bm.store(B.bmd, B.bmi, B.pitch, c);
+}
+
+// ============================================================================ host side
+
+namespace host {
+
+// ---- ORd-CiPA CellML constants and initial states (___initConsts, :545-804)
+static const double kOrdInitStates[49] = {
+  -88.00190465, 0.0125840447, 8.49e-05, 7.268004498, 7.268089977, 144.6555918, 144.6555651,
+  1.619574538, 1.571234014, 8.6e-05, 0.007344121102, 0.6981071913, 0.6980895801, 0.6979908432,
+  0.4549485525, 0.6979245865, 0.0001882617273, 0.5008548855, 0.2693065357, 0.001001097687,
+  0.9995541745, 0.5865061736, 0.0005100862934, 0.9995541823, 0.6393399482, 2.34e-9,
+  0.9999999909, 0.9102412777, 0.9999999909, 0.9998046777, 0.9999738312, 0.9999999909,
+  0.9999999909, 0.002749414044, 0.999637, 6.83208e-05, 1.80145e-08, 8.26619e-05, 0.00015551,
+  5.67623e-05, 0, 0, 0, 0, 0.2707758025, 0.0001928503426, 0.9967597594, 2.5e-7, 3.12e-7 };
+
+// Markov IKr (A, B, q) per transition, order k1 k2 k3 k4 k11 k21 k31 k41 k51 k52 k53 k61 k62 k63
+static const double kMarkovABq[14][3] = {
+  { 0.0264, 4.631E-05, 4.843 }, { 4.986E-06, -0.004226, 4.23 }, { 0.001214, 0.008516, 4.962 },
+  { 1.854E-05, -0.04641, 3.769 }, { 0.0007868, 1.535E-08, 4.942 }, { 5.455E-06, -0.1688, 4.156 },
+  { 0.005509, 7.771E-09, 4.22 }, { 0.001416, -0.02877, 1.459 }, { 0.4492, 0.008595, 5 },
+  { 0.3181, 3.613E-08, 4.663 }, { 0.149, 0.004668, 2.412 }, { 0.01241, 0.1725, 5.568 },
+  { 0.3226, -0.0006575, 5 }, { 0.008978, -0.02215, 5.682 } };
+
+static double hill_factor(double D, double ic50, double h)
+{
+  return (ic50 > 10E-14 && h > 10E-14) ? pow(1. + pow(D / ic50, h), -1) : 1.;
+}
+
+struct OrdCellIn {
+  double celltype, conc, bcl;
+  double herg[19];
+  double gs[7];
+};
+
+// per-cell parameters (celltype-derived constants :749-803, Hill block :826-842,
+// population scales applied after initConsts)
+static void ord_params(const OrdCellIn &in, double stim_scale, double *p)
+{
+  using namespace ord;
+  const bool epi = in.celltype == 1.0, mid = in.celltype == 2.0;
+  const double D = in.conc;
+  const double *hg = in.herg;
+  const double pca_base = epi ? 0.0001007 * 1.2 : mid ? 0.0001007 * 2.5 : 0.0001007;
+  const double pcap = 1.1 * pca_base;
+  p[OP_CELLTYPE] = in.celltype;
+  p[OP_GNA] = 75.0 * hill_factor(D, hg[6], hg[7]) * in.gs[0];
+  p[OP_GNAL] = (epi ? 0.019957499999999975 * 0.6 : 0.019957499999999975) * hill_factor(D, hg[8], hg[9]) * in.gs[1];
+  p[OP_PCA] = pca_base * hill_factor(D, hg[0], hg[1]) * in.gs[2];
+  p[OP_PCAP] = pcap;
+  p[OP_PCANA] = 0.00125 * pca_base;
+  p[OP_PCAK] = 0.0003574 * pca_base;
+  p[OP_PCANAP] = 0.00125 * pcap;
+  p[OP_PCAKP] = 0.0003574 * pcap;
+  p[OP_GKR] = (epi ? 0.04658545454545456 * 1.3 : mid ? 0.04658545454545456 * 0.8 : 0.04658545454545456) * in.gs[3];
+  p[OP_GKS] = (epi ? 0.006358000000000001 * 1.4 : 0.006358000000000001) * hill_factor(D, hg[4], hg[5]) * in.gs[4];
+  p[OP_GK1] = (epi ? 0.3239783999999998 * 1.2 : mid ? 0.3239783999999998 * 1.3 : 0.3239783999999998) * hill_factor(D, hg[2], hg[3]) * in.gs[5];
+  p[OP_GTO] = ((epi || mid) ? 0.02 * 4.0 : 0.02) * hill_factor(D, hg[10], hg[11]) * in.gs[6];
+  p[OP_GKB] = epi ? 0.003 * 0.6 : 0.003;
+  p[OP_CMDNMAX] = epi ? 0.05 * 1.3 : 0.05;
+  p[OP_UPSCALE] = epi ? 1.3 : 1.0;
+  p[OP_GNCX] = epi ? 0.0008 * 1.1 : mid ? 0.0008 * 1.4 : 0.0008;
+  p[OP_PNAK] = epi ? 30 * 0.9 : mid ? 30 * 0.7 : 30;
+  const double Kmax = hg[14], Ku = hg[15], n = hg[16], halfmax = hg[17];
+  p[OP_KU] = Ku;
+  p[OP_BIND] = (Kmax * Ku * exp(n * log(D))) / (exp(n * log(D)) + halfmax);
+  p[OP_KT] = 0.0;
+  p[OP_VHALF] = hg[18];
+  p[OP_STIM_START] = 3;
+  p[OP_STIM_END] = 100000000000000000.0;
+  p[OP_STIM_AMP] = -80 * stim_scale;
+  p[OP_BCL] = in.bcl;
+  p[OP_STIM_DUR] = 0.5;
+  p[OP_CONC] = D;
+  p[OP_JREL_SCALE] = mid ? 1.7 : 1.0;
+  p[OP_EPI] = epi ? 1.0 : 0.0;
+}
+
+// Full CellML CONSTANTS[206] of a cell, for hosts that read the reference layout
+static void ord_constants(const OrdCellIn &in, double stim_scale, double *C)
+{
+  memset(C, 0, sizeof(double) * 206);
+  double p[ord::NP];
+  ord_params(in, stim_scale, p);
+  const double tab[][2] = {
+    { 1, 140 }, { 2, 1.8 }, { 3, 5.4 }, { 4, 8314 }, { 5, 310 }, { 6, 96485 }, { 7, 1 }, { 8, 2 }, { 9, 1 },
+    { 10, 0.01 }, { 11, 0.0011 }, { 17, 0.15 }, { 18, 0.05 }, { 19, 0.00068 }, { 20, 0.05 }, { 21, 0.0015 },
+    { 22, 0.05 }, { 23, 0.00238 }, { 24, 0.07 }, { 25, 0.0005 }, { 26, 0.047 }, { 27, 0.00087 }, { 28, 1.124 },
+    { 29, 0.0087 }, { 30, 10 }, { 31, 0.8 }, { 32, 1 }, { 33, 0.01833 }, { 34, 39.57 }, { 35, 9.871 },
+    { 36, 11.64 }, { 37, 34.77 }, { 38, 6.765 }, { 39, 8.552 }, { 40, 77.42 }, { 41, 5.955 }, { 42, 82.9 },
+    { 43, 6.086 }, { 44, 0.99 }, { 47, 200 }, { 48, 0.019957499999999975 }, { 49, 0.02 }, { 50, 0.002 },
+    { 51, 1000 }, { 52, 0.0001007 }, { 53, 0.04658545454545456 }, { 102, 37 }, { 103, 0.006358000000000001 },
+    { 104, 817.3 }, { 105, 0.3239783999999998 }, { 106, 15 }, { 107, 5 }, { 108, 88.12 }, { 109, 12.5 },
+    { 110, 6e4 }, { 111, 6e4 }, { 112, 5e3 }, { 113, 1.5e6 }, { 114, 5e3 }, { 115, 0.5224 }, { 116, 0.167 },
+    { 117, 150e-6 }, { 118, 0.0008 }, { 119, 949.5 }, { 120, 182.4 }, { 121, 687.2 }, { 122, 39.4 },
+    { 123, 1899 }, { 124, 79300 }, { 125, 639 }, { 126, 40 }, { 127, 9.073 }, { 128, 27.78 }, { 129, -0.155 },
+    { 130, 0.5 }, { 131, 0.3582 }, { 132, 0.05 }, { 133, 9.8 }, { 134, 1.698e-7 }, { 135, 1e-7 }, { 136, 4.2 },
+    { 137, 1.698e-7 }, { 138, 224 }, { 139, 292 }, { 140, 30 }, { 141, 0.003 }, { 142, 3.75e-10 },
+    { 143, 2.5e-8 }, { 144, 0.0005 }, { 145, 0.0005 }, { 146, 4.75 }, { 147, 1.0 }, { 148, 1.0 } };
+  for (const auto &e : tab) C[(int)e[0]] = e[1];
+  for (int k = 0; k < 14; ++k) for (int j = 0; j < 3; ++j) C[54 + 3 * k + j] = 0;
+  {
+    // CellML order of the Markov parameter triplets at CONSTANTS[54..95]
+    const int order[14] = { 0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13 };
+    for (int k = 0; k < 14; ++k) for (int j = 0; j < 3; ++j) C[54 + 3 * k + j] = kMarkovABq[order[k]][j];
+  }
+  using namespace ord;
+  C[0] = in.celltype;
+  C[12] = p[OP_STIM_START]; C[13] = p[OP_STIM_END]; C[14] = p[OP_STIM_AMP]; C[15] = p[OP_BCL]; C[16] = p[OP_STIM_DUR];
+  C[45] = p[OP_GNA];
+  C[96] = in.herg[14]; C[97] = in.herg[15]; C[98] = in.herg[16]; C[99] = in.herg[17]; C[100] = 0; C[101] = in.herg[18];
+  C[149] = frt; C[150] = p[OP_CMDNMAX]; C[151] = Ahs; C[152] = thLp; C[153] = p[OP_GNAL]; C[154] = p[OP_GTO];
+  C[155] = 0.6; C[156] = p[OP_PCA]; C[157] = 75; C[158] = 0; C[159] = p[OP_GKR]; C[160] = p[OP_GKS];
+  C[161] = p[OP_GK1]; C[162] = vcell; C[163] = p[OP_GKB]; C[164] = 0; C[165] = 0; C[166] = a_rel; C[167] = btp;
+  C[168] = p[OP_UPSCALE]; C[169] = ffrt; C[170] = 1 - 0.6; C[171] = p[OP_PCAP]; C[172] = p[OP_PCANA];
+  C[173] = p[OP_PCAK]; C[174] = 2 * frt; C[175] = frt; C[176] = frt; C[177] = Ageo; C[178] = frt;
+  C[179] = 2 * frt; C[180] = a_relp; C[181] = p[OP_PCANAP]; C[182] = p[OP_PCAKP]; C[183] = Acap;
+  C[184] = vmyo; C[185] = vnsr; C[186] = vjsr; C[187] = vss; C[188] = h10; C[189] = h11; C[190] = h12;
+  C[191] = k1ncx; C[192] = kcaoff; C[193] = kcaoff; C[194] = p[OP_GNCX]; C[195] = h10; C[196] = h11;
+  C[197] = h12; C[198] = k1ncx; C[199] = kcaoff; C[200] = kcaoff; C[201] = b1nak; C[202] = a2nak;
+  C[203] = a4nak; C[204] = p[OP_PNAK]; C[205] = 0;
+}
+
+// ---- TT06 constants / initial states (initConsts :196-270 and the epi/M siblings)
+static const double kTtInit[3][19] = {
+  { -86.709, 138.4, 10.355, 0.00013, 0.00448, 0.476, 0.0087, 0.00155, 0.7573, 0.7225, 0.00036,
+    3.164e-5, 0.8009, 0.9778, 0.9953, 0.3212, 2.235e-8, 3.715, 0.9068 },
+  { -85.23, 136.89, 8.604, 0.000126, 0.00621, 0.4712, 0.0095, 0.00172, 0.7444, 0.7045, 0.00036,
+    3.373e-5, 0.7888, 0.9755, 0.9953, 0.999998, 2.42e-8, 3.64, 0.9073 },
+  { -85.423, 138.52, 10.132, 0.000153, 0.0165, 0.473, 0.0174, 0.00165, 0.749, 0.6788, 0.00042,
+    3.288e-5, 0.7026, 0.9526, 0.9942, 0.999998, 2.347e-8, 4.272, 0.8978 } };
+
+struct TtCellIn {
+  double celltype, conc, bcl;
+  double ic50[3], hill[3];
+  double gs[10];
+};
+
+static void tt_params(const TtCellIn &in, double stim_scale, double *p)
+{
+  using namespace tt;
+  const int ct = (int)in.celltype;
+  const double base[10] = { 14.838, 5.405, 0.153, ct == 2 ? 0.098 : 0.392, ct == 0 ? 0.073 : 0.294,
+                            0.0000398, 0.00029, 0.000592, 0.1238, 0.0146 };
+  const int slot[10] = { TP_GNA, TP_GK1, TP_GKR, TP_GKS, TP_GTO, TP_GCAL, TP_GBNA, TP_GBCA, TP_GPCA, TP_GPK };
+  p[TP_CELLTYPE] = in.celltype;
+  for (int k = 0; k < 10; ++k) p[slot[k]] = base[k] * in.gs[k];
+  // Hill block on IKr, ICaL, INa with the CiPA initConsts rule
+  p[TP_GKR] *= hill_factor(in.conc, in.ic50[0], in.hill[0]);
+  p[TP_GCAL] *= hill_factor(in.conc, in.ic50[1], in.hill[1]);
+  p[TP_GNA] *= hill_factor(in.conc, in.ic50[2], in.hill[2]);
+  p[TP_PNAK] = 2.724;
+  p[TP_STIM_START] = 10;
+  p[TP_BCL] = in.bcl;
+  p[TP_STIM_DUR] = 1;
+  p[TP_STIM_AMP] = 52 * stim_scale;
+}
+
+static void tt_constants(const TtCellIn &in, double stim_scale, double *C)
+{
+  static const double c0[53] = {
+    8314.472, 310, 96485.3415, 0.185, 0.016404, 10, 1000, 1, 52, 0.03, 5.4, 140, 2, 5.405, 0.153,
+    0.392, 14.838, 0.00029, 0.0000398, 0.000592, 0.073, 2.724, 1, 40, 1000, 0.1, 2.5, 0.35, 1.38,
+    87.5, 0.1238, 0.0005, 0.0146, 0.15, 0.045, 0.06, 0.005, 1.5, 2.5, 1, 0.102, 0.0038, 0.00025,
+    0.00036, 0.006375, 0.2, 0.001, 10, 0.3, 0.4, 0.00025, 0.001094, 0.00005468 };
+  memcpy(C, c0, sizeof(c0));
+  double p[tt::NP];
+  tt_params(in, stim_scale, p);
+  using namespace tt;
+  C[16] = p[TP_GNA]; C[13] = p[TP_GK1]; C[14] = p[TP_GKR]; C[15] = p[TP_GKS]; C[20] = p[TP_GTO];
+  C[18] = p[TP_GCAL]; C[17] = p[TP_GBNA]; C[19] = p[TP_GBCA]; C[30] = p[TP_GPCA]; C[32] = p[TP_GPK];
+  C[6] = p[TP_BCL]; C[8] = p[TP_STIM_AMP];
+}
+
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+};
+
+}  // namespace host
+}  // namespace cep
+
+using namespace cep;
+
+struct cep_batch {
+  int model, n, prec, pitch;
+  int ns, nc, nalg, np;
+  std::string err;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double ktime_total = 0.0;
+  int klaunches = 0;
+  bool pending = false;
+  // run config
+  double dt = 0.1, rtol = 1e-6, atol = 1e-8, stim_scale = 1.0;
+  int maxord = 5, nbeats_total = 1, nwin = 1;
+  bool vtrace_on = false;
+  int tcap = 0, cap = 0;
+  // host copies of inputs
+  std::vector<host::OrdCellIn> ord_in;
+  std::vector<host::TtCellIn> tt_in;
+  // device memory
+  double *d_y = nullptr, *d_prm = nullptr, *d_bmd = nullptr, *d_ct = nullptr, *d_cv = nullptr;
+  double *d_zn = nullptr, *d_ewt = nullptr, *d_acor = nullptr, *d_f0 = nullptr, *d_jd = nullptr;
+  double *d_fac = nullptr, *d_sd = nullptr, *d_tcur = nullptr, *d_vtrace = nullptr;
+  int *d_si = nullptr, *d_bmi = nullptr, *d_status = nullptr, *d_iout = nullptr;
+  long long *d_stats = nullptr;
+  std::vector<void *> allocs;
+};
+
+static int fail(cep_batch *b, int code, const std::string &msg)
+{
+  if (b) b->err = msg;
+  return code;
+}
+
+#define HIPCHK(b, call)                                                                  \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess) return fail(b, CEP_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+static int dalloc(cep_batch *b, T **p, size_t count)
+{
+  void *q = nullptr;
+  hipError_t e = hipMalloc(&q, count * sizeof(T) + 64);
+  if (e != hipSuccess) return fail(b, CEP_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+  e = hipMemset(q, 0, count * sizeof(T) + 64);
+  if (e != hipSuccess) return fail(b, CEP_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
+  b->allocs.push_back(q);
+  *p = (T *)q;
+  return 0;
+}
+
+static int ensure_markov_table()
+{
+  static bool done = false;
+  if (done) return 0;
+  ord::MarkovTable t;
+  for (int k = 0; k < 14; ++k) {
+    const double A = host::kMarkovABq[k][0], Bc = host::kMarkovABq[k][1], q = host::kMarkovABq[k][2];
+    t.AQ[k] = A * exp(((37.0 - 20.0) * log(q)) / 10.0);
+    t.B[k] = Bc;
+  }
+  if (hipMemcpyToSymbol(HIP_SYMBOL(ord::g_markov), &t, sizeof(t)) != hipSuccess) return CEP_E_HIP;
+  done = true;
+  return 0;
+}
+
+static int alloc_trace_buffers(cep_batch *b);
+
+extern "C" {
+
+cep_batch *cep_batch_create(int model, int ncells, int precision)
+{
+  if ((model != CEP_MODEL_ORD_CIPA && model != CEP_MODEL_TT06) || ncells <= 0 ||
+      (precision != CEP_PREC_FP64 && precision != CEP_PREC_MIXED))
+    return nullptr;
+  cep_batch *b = new cep_batch();
+  b->model = model;
+  b->n = ncells;
+  b->prec = precision;
+  b->pitch = (ncells + 63) / 64 * 64;
+  if (model == CEP_MODEL_ORD_CIPA) {
+    b->ns = ord::NS; b->nc = ord::NC; b->nalg = ORD_NALG; b->np = ord::NP;
+    b->ord_in.resize(ncells);
+    for (auto &c : b->ord_in) { memset(&c, 0, sizeof(c)); c.bcl = 1000; for (double &g : c.gs) g = 1.0; c.herg[16] = 1.0; c.herg[17] = 1.0; }
+  } else {
+    b->ns = tt::NS; b->nc = tt::NC; b->nalg = TT_NALG; b->np = tt::NP;
+    b->dt = 0.02;
+    b->tt_in.resize(ncells);
+    for (auto &c : b->tt_in) { memset(&c, 0, sizeof(c)); c.bcl = 1000; for (double &g : c.gs) g = 1.0; }
+  }
+  const size_t P = b->pitch;
+  int rc = 0;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+    delete b;
+    return nullptr;
+  }
+  rc |= dalloc(b, &b->d_y, P * b->ns);
+  rc |= dalloc(b, &b->d_prm, P * b->np);
+  rc |= dalloc(b, &b->d_bmd, P * BM_TOTAL_D);
+  rc |= dalloc(b, &b->d_bmi, P * BM_I_N);
+  rc |= dalloc(b, &b->d_status, P);
+  if (model == CEP_MODEL_ORD_CIPA) {
+    rc |= dalloc(b, &b->d_zn, P * 6 * ordbdf::N);
+    rc |= dalloc(b, &b->d_ewt, P * ordbdf::N);
+    rc |= dalloc(b, &b->d_acor, P * ordbdf::N);
+    rc |= dalloc(b, &b->d_f0, P * ordbdf::N);
+    rc |= dalloc(b, &b->d_jd, P * ordbdf::NJC * ordbdf::N);
+    rc |= dalloc(b, &b->d_fac, P * ordbdf::NFAC);
+    rc |= dalloc(b, &b->d_sd, P * ordbdf::SD_N);
+    rc |= dalloc(b, &b->d_si, P * ordbdf::SI_N);
+    rc |= dalloc(b, &b->d_stats, P * 8);
+  } else {
+    rc |= dalloc(b, &b->d_tcur, P);
+    rc |= dalloc(b, &b->d_iout, P);
+  }
+  if (rc || ensure_markov_table()) {
+    cep_batch_destroy(b);
+    return nullptr;
+  }
+  return b;
+}
+
+void cep_batch_destroy(cep_batch *b)
+{
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (void *p : b->allocs) (void)hipFree(p);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+const char *cep_last_error(const cep_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+int cep_batch_info(const cep_batch *b, int *model, int *ncells, int *nstates, int *nconsts, int *nalg)
+{
+  if (!b) return CEP_E_ARG;
+  if (model) *model = b->model;
+  if (ncells) *ncells = b->n;
+  if (nstates) *nstates = b->ns;
+  if (nconsts) *nconsts = b->nc;
+  if (nalg) *nalg = b->nalg;
+  return 0;
+}
+
+int cep_set_cells(cep_batch *b, int first, int count, const double *celltype, const double *conc,
+                  const double *herg, int herg_stride, const double *gscale, int gscale_stride,
+                  const double *bcl)
+{
+  if (!b || first < 0 || count < 0 || first + count > b->n) return fail(b, CEP_E_ARG, "cell range");
+  for (int k = 0; k < count; ++k) {
+    const int i = first + k;
+    if (b->model == CEP_MODEL_ORD_CIPA) {
+      host::OrdCellIn &c = b->ord_in[i];
+      if (celltype) c.celltype = celltype[k];
+      if (conc) c.conc = conc[k];
+      if (bcl) c.bcl = bcl[k];
+      if (herg) for (int j = 0; j < 19; ++j) c.herg[j] = herg[(size_t)k * herg_stride + j];
+      if (gscale) for (int j = 0; j < 7; ++j) c.gs[j] = gscale[(size_t)k * gscale_stride + j];
+      if (!(c.bcl > 0) || c.celltype < 0 || c.celltype > 2) return fail(b, CEP_E_ARG, "bad celltype/bcl");
+    } else {
+      host::TtCellIn &c = b->tt_in[i];
+      if (celltype) c.celltype = celltype[k];
+      if (conc) c.conc = conc[k];
+      if (bcl) c.bcl = bcl[k];
+      if (herg) for (int j = 0; j < 3; ++j) { c.ic50[j] = herg[(size_t)k * herg_stride + j]; c.hill[j] = herg[(size_t)k * herg_stride + 3 + j]; }
+      if (gscale) for (int j = 0; j < 10; ++j) c.gs[j] = gscale[(size_t)k * gscale_stride + j];
+      if (!(c.bcl > 0) || c.celltype < 0 || c.celltype > 2) return fail(b, CEP_E_ARG, "bad celltype/bcl");
+    }
+  }
+  return 0;
+}
+
+int cep_set_run_config(cep_batch *b, double dt, double rtol, double atol, int maxord, int nbeats_total,
+                       int nwin, double stim_amp_scale)
+{
+  if (!b || !(dt > 0) || !(rtol > 0) || !(atol > 0) || maxord < 1 || maxord > 5 || nbeats_total < 1 || nwin < 1)
+    return fail(b, CEP_E_ARG, "run config");
+  b->dt = dt; b->rtol = rtol; b->atol = atol; b->maxord = maxord; b->nbeats_total = nbeats_total;
+  b->nwin = nwin; b->stim_scale = stim_amp_scale;
+  return 0;
+}
+
+static double max_bcl(const cep_batch *b)
+{
+  double m = 0;
+  if (b->model == CEP_MODEL_ORD_CIPA) for (auto &c : b->ord_in) m = c.bcl > m ? c.bcl : m;
+  else for (auto &c : b->tt_in) m = c.bcl > m ? c.bcl : m;
+  return m;
+}
+
+static int alloc_trace_buffers(cep_batch *b)
+{
+  // Ca trace of one beat sampled every 1 ms (dt_write), sized for the longest BCL
+  const int print_freq = (int)((1. / b->dt) * 1.0);
+  const int beat_steps = (int)(max_bcl(b) / b->dt);
+  const int cap = beat_steps / (print_freq > 0 ? print_freq : 1) + 4;
+  if (cap > b->cap) {
+    int rc = dalloc(b, &b->d_ct, (size_t)b->pitch * cap);
+    rc |= dalloc(b, &b->d_cv, (size_t)b->pitch * cap);
+    if (rc) return rc;
+    b->cap = cap;
+  }
+  if (b->vtrace_on && beat_steps > b->tcap) {
+    if (dalloc(b, &b->d_vtrace, (size_t)b->pitch * beat_steps)) return CEP_E_HIP;
+    b->tcap = beat_steps;
+  }
+  return 0;
+}
+
+static int reset_integrators(cep_batch *b)
+{
+  const size_t P = b->pitch;
+  HIPCHK(b, hipMemsetAsync(b->d_status, 0, P * sizeof(int), b->stream));
+  HIPCHK(b, hipMemsetAsync(b->d_bmd, 0, P * BM_TOTAL_D * sizeof(double), b->stream));
+  HIPCHK(b, hipMemsetAsync(b->d_bmi, 0, P * BM_I_N * sizeof(int), b->stream));
+  if (b->model == CEP_MODEL_ORD_CIPA) {
+    HIPCHK(b, hipMemsetAsync(b->d_si, 0, P * ordbdf::SI_N * sizeof(int), b->stream));   // phase = FRESH
+    HIPCHK(b, hipMemsetAsync(b->d_sd, 0, P * ordbdf::SD_N * sizeof(double), b->stream));
+    HIPCHK(b, hipMemsetAsync(b->d_stats, 0, P * 8 * sizeof(long long), b->stream));
+  } else {
+    HIPCHK(b, hipMemsetAsync(b->d_tcur, 0, P * sizeof(double), b->stream));
+    HIPCHK(b, hipMemsetAsync(b->d_iout, 0, P * sizeof(int), b->stream));
+  }
+  return 0;
+}
+
+int cep_init_consts(cep_batch *b)
+{
+  if (!b) return CEP_E_ARG;
+  const size_t P = b->pitch;
+  std::vector<double> prm(P * b->np, 0.0), y(P * b->ns, 0.0);
+  double p[64];
+  for (int i = 0; i < b->n; ++i) {
+    if (b->model == CEP_MODEL_ORD_CIPA) {
+      host::ord_params(b->ord_in[i], b->stim_scale, p);
+      for (int s = 0; s < 49; ++s) y[s * P + i] = host::kOrdInitStates[s];
+      y[43 * P + i] = b->ord_in[i].conc;
+    } else {
+      host::tt_params(b->tt_in[i], b->stim_scale, p);
+      const int ct = (int)b->tt_in[i].celltype;
+      for (int s = 0; s < tt::NS; ++s) y[s * P + i] = host::kTtInit[ct][s];
+    }
+    for (int k = 0; k < b->np; ++k) prm[k * P + i] = p[k];
+  }
+  if (int rc = alloc_trace_buffers(b)) return rc;
+  HIPCHK(b, hipMemcpyAsync(b->d_prm, prm.data(), prm.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  HIPCHK(b, hipMemcpyAsync(b->d_y, y.data(), y.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  if (int rc = reset_integrators(b)) return rc;
+  HIPCHK(b, hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int cep_get_constants(const cep_batch *b, int cell, double *out)
+{
+  if (!b || cell < 0 || cell >= b->n || !out) return CEP_E_ARG;
+  if (b->model == CEP_MODEL_ORD_CIPA) host::ord_constants(b->ord_in[cell], b->stim_scale, out);
+  else host::tt_constants(b->tt_in[cell], b->stim_scale, out);
+  return 0;
+}
+
+int cep_compute_rates(cep_batch *b, double t, const double *states, double *rates, double *algebraic)
+{
+  if (!b || !states || !rates) return fail(b, CEP_E_ARG, "null array");
+  const size_t nsz = (size_t)b->n * b->ns;
+  double *dy = nullptr, *df = nullptr, *da = nullptr;
+  HIPCHK(b, hipMalloc(&dy, nsz * sizeof(double)));
+  HIPCHK(b, hipMalloc(&df, nsz * sizeof(double)));
+  if (algebraic) {
+    HIPCHK(b, hipMalloc(&da, (size_t)b->n * b->nalg * sizeof(double)));
+    HIPCHK(b, hipMemsetAsync(da, 0, (size_t)b->n * b->nalg * sizeof(double), b->stream));
+  }
+  HIPCHK(b, hipMemcpyAsync(dy, states, nsz * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  // the rates kernels are compiled for 64-thread workgroups (__launch_bounds__(64))
+  constexpr int kRatesBlock = 64;
+  const int blocks = (b->n + kRatesBlock - 1) / kRatesBlock;
+  if (b->model == CEP_MODEL_ORD_CIPA) {
+    if (b->prec == CEP_PREC_FP64)
+      hipLaunchKernelGGL(ord_rates_kernel<double>, dim3(blocks), dim3(kRatesBlock), 0, b->stream, b->n, b->pitch, t, dy, df, da, b->d_prm);
+    else
+      hipLaunchKernelGGL(ord_rates_kernel<float>, dim3(blocks), dim3(kRatesBlock), 0, b->stream, b->n, b->pitch, t, dy, df, da, b->d_prm);
+  } else {
+    if (b->prec == CEP_PREC_FP64)
+      hipLaunchKernelGGL(tt_rates_kernel<double>, dim3(blocks), dim3(kRatesBlock), 0, b->stream, b->n, b->pitch, t, dy, df, da, b->d_prm);
+    else
+      hipLaunchKernelGGL(tt_rates_kernel<float>, dim3(blocks), dim3(kRatesBlock), 0, b->stream, b->n, b->pitch, t, dy, df, da, b->d_prm);
+  }
+  HIPCHK(b, hipGetLastError());
+  HIPCHK(b, hipMemcpyAsync(rates, df, nsz * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  if (algebraic) HIPCHK(b, hipMemcpyAsync(algebraic, da, (size_t)b->n * b->nalg * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(b, hipStreamSynchronize(b->stream));
+  (void)hipFree(dy);
+  (void)hipFree(df);
+  if (da) (void)hipFree(da);
+  return 0;
+}
+
+int cep_run_beats_async(cep_batch *b, int nbeats)
+{
+  if (!b || nbeats < 1) return fail(b, CEP_E_ARG, "nbeats");
+  if (b->cap == 0) return fail(b, CEP_E_STATE, "cep_init_consts not called");
+  if (int rc = alloc_trace_buffers(b)) return rc;
+  RunCfg rc{b->dt, b->nbeats_total, b->nwin, (int)((1. / b->dt) * 1.0)};
+  if (rc.print_freq < 1) rc.print_freq = 1;
+  HIPCHK(b, hipEventRecord(b->ev0, b->stream));
+  if (b->model == CEP_MODEL_ORD_CIPA) {
+    Buffers B;
+    B.ncell = b->n; B.pitch = b->pitch; B.y = b->d_y; B.zn = b->d_zn; B.ewt = b->d_ewt;
+    B.acor = b->d_acor; B.f0 = b->d_f0; B.jd = b->d_jd; B.fac = b->d_fac; B.sd = b->d_sd;
+    B.si = b->d_si; B.prm = b->d_prm; B.bmd = b->d_bmd; B.bmi = b->d_bmi; B.ct = b->d_ct;
+    B.cv = b->d_cv; B.vtrace = b->vtrace_on ? b->d_vtrace : nullptr; B.tcap = b->tcap;
+    B.stats = b->d_stats; B.status = b->d_status; B.dt = b->dt; B.rtol = b->rtol; B.atol = b->atol;
+    B.uround_fd = b->prec == CEP_PREC_FP64 ? ordbdf::UROUND : 1.1920928955078125e-07;
+    B.maxord = b->maxord; B.nwin_unused = 0; B.cap = b->cap;
+    const int blocks = (b->n + 63) / 64;
+    if (b->prec == CEP_PREC_FP64)
+      hipLaunchKernelGGL(ord_bdf_kernel<double>, dim3(blocks), dim3(64), 0, b->stream, B, rc, nbeats);
+    else
+      hipLaunchKernelGGL(ord_bdf_kernel<float>, dim3(blocks), dim3(64), 0, b->stream, B, rc, nbeats);
+  } else {
+    TtBuffers B;
+    B.ncell = b->n; B.pitch = b->pitch; B.cap = b->cap; B.tcap = b->tcap; B.y = b->d_y;
+    B.prm = b->d_prm; B.bmd = b->d_bmd; B.ct = b->d_ct; B.cv = b->d_cv; B.tcur = b->d_tcur;
+    B.vtrace = b->vtrace_on ? b->d_vtrace : nullptr; B.bmi = b->d_bmi; B.iout = b->d_iout;
+    B.status = b->d_status;
+    const int blocks = (b->n + 255) / 256;
+    if (b->prec == CEP_PREC_FP64)
+      hipLaunchKernelGGL(tt_rl_kernel<double>, dim3(blocks), dim3(256), 0, b->stream, B, rc, nbeats);
+    else
+      hipLaunchKernelGGL(tt_rl_kernel<float>, dim3(blocks), dim3(256), 0, b->stream, B, rc, nbeats);
+  }
+  HIPCHK(b, hipGetLastError());
+  HIPCHK(b, hipEventRecord(b->ev1, b->stream));
+  b->pending = true;
+  return 0;
+}
+
+int cep_sync(cep_batch *b)
+{
+  if (!b) return CEP_E_ARG;
+  HIPCHK(b, hipStreamSynchronize(b->stream));
+  if (b->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, b->ev0, b->ev1) == hipSuccess) { b->ktime_total += ms; b->klaunches++; }
+    b->pending = false;
+  }
+  return 0;
+}
+
+int cep_run_beats(cep_batch *b, int nbeats)
+{
+  int rc = cep_run_beats_async(b, nbeats);
+  if (rc) return rc;
+  return cep_sync(b);
+}
+
+static int get_soa(const cep_batch *b, const double *d, int width, double *out)
+{
+  std::vector<double> tmp((size_t)b->pitch * width);
+  if (hipMemcpyAsync(tmp.data(), d, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
+      hipStreamSynchronize(b->stream) != hipSuccess)
+    return CEP_E_HIP;
+  for (int i = 0; i < b->n; ++i)
+    for (int k = 0; k < width; ++k) out[(size_t)i * width + k] = tmp[(size_t)k * b->pitch + i];
+  return 0;
+}
+
+int cep_get_states(const cep_batch *b, double *out)
+{
+  if (!b || !out) return CEP_E_ARG;
+  return get_soa(b, b->d_y, b->ns, out);
+}
+
+int cep_set_states(cep_batch *b, const double *in)
+{
+  if (!b || !in) return CEP_E_ARG;
+  std::vector<double> tmp((size_t)b->pitch * b->ns, 0.0);
+  for (int i = 0; i < b->n; ++i)
+    for (int k = 0; k < b->ns; ++k) tmp[(size_t)k * b->pitch + i] = in[(size_t)i * b->ns + k];
+  HIPCHK(b, hipMemcpyAsync(b->d_y, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  if (int rc = reset_integrators(b)) return rc;
+  HIPCHK(b, hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int cep_get_biomarkers(const cep_batch *b, double *out)
+{
+  if (!b || !out) return CEP_E_ARG;
+  std::vector<double> d((size_t)b->pitch * BM_TOTAL_D);
+  std::vector<int> iv((size_t)b->pitch * BM_I_N);
+  if (hipMemcpyAsync(d.data(), b->d_bmd, d.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
+      hipMemcpyAsync(iv.data(), b->d_bmi, iv.size() * sizeof(int), hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
+      hipStreamSynchronize(b->stream) != hipSuccess)
+    return CEP_E_HIP;
+  for (int i = 0; i < b->n; ++i) {
+    for (int k = 0; k < BM_RES_N; ++k) out[(size_t)i * CEP_NBM + k] = d[(size_t)(BM_D_N + BM_RES_N + k) * b->pitch + i];
+    out[(size_t)i * CEP_NBM + CEP_BM_STEEPEST_PACE] = iv[(size_t)1 * b->pitch + i];
+  }
+  return 0;
+}
+
+int cep_get_status(const cep_batch *b, int *status)
+{
+  if (!b || !status) return CEP_E_ARG;
+  if (hipMemcpy(status, b->d_status, (size_t)b->n * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return CEP_E_HIP;
+  return 0;
+}
+
+int cep_get_stats(const cep_batch *b, long long *out)
+{
+  if (!b || !out) return CEP_E_ARG;
+  if (!b->d_stats) { memset(out, 0, sizeof(long long) * 8 * b->n); return 0; }
+  std::vector<long long> tmp((size_t)b->pitch * 8);
+  if (hipMemcpy(tmp.data(), b->d_stats, tmp.size() * sizeof(long long), hipMemcpyDeviceToHost) != hipSuccess) return CEP_E_HIP;
+  for (int i = 0; i < b->n; ++i)
+    for (int k = 0; k < 8; ++k) out[(size_t)i * 8 + k] = tmp[(size_t)k * b->pitch + i];
+  return 0;
+}
+
+int cep_enable_vm_trace(cep_batch *b, int enable)
+{
+  if (!b) return CEP_E_ARG;
+  b->vtrace_on = enable != 0;
+  return 0;
+}
+
+int cep_get_vm_trace(const cep_batch *b, double *out, int *nsamp)
+{
+  if (!b || !nsamp) return CEP_E_ARG;
+  if (!b->vtrace_on || !b->d_vtrace) return fail(const_cast<cep_batch *>(b), CEP_E_STATE, "Vm trace not enabled");
+  *nsamp = b->tcap;
+  if (!out) return 0;   // size query
+  return get_soa(b, b->d_vtrace, b->tcap, out);
+}
+
+int cep_kernel_time(cep_batch *b, double *avg_ms, double *total_ms)
+{
+  if (!b) return CEP_E_ARG;
+  const int n = b->klaunches;
+  if (avg_ms) *avg_ms = n ? b->ktime_total / n : 0.0;
+  if (total_ms) *total_ms = b->ktime_total;
+  b->ktime_total = 0.0;
+  b->klaunches = 0;
+  return n;
+}
+
+}  // extern "C"

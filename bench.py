@@ -1,0 +1,185 @@
+"""Headline benchmark: ORd-CiPA cell-beats/s on one MI355X (batched population config).
+
+Workload: 10,000 ORd-CiPA endo cells = 50 seeded drugs x 4 concentrations x 50 variants with
+7 lognormal(0, 0.2) conductance scales, BCL drawn from {500, 1000, 2000} ms, CVODE-style BDF
+(rtol 1e-6, atol 1e-8, max step = output dt = 0.1 ms), on-device biomarkers every output
+point.  A "step" is one beat of the whole population (every cell advances one beat of its
+own BCL); inputs and states are resident in HBM.  The timed solver is the mixed-precision
+one (FP32 gating/currents, FP64 state/Newton/LU); an FP64 run of a subsample checks APD90
+and qNet against it.
+
+Prints one JSON line.  --gpus N > 1 is not supported (single-GPU build).
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+cep = importlib.import_module("2ddrugsimulation_amd")
+
+# algorithmic work per unit for the BDF kernel (flops: + - * / each 1; transcendentals apart).
+# RHS counted by oracle/opcount.cpp on the ORd-CiPA model; setup/solve/step from the
+# structured Newton algebra (see the kernel header): factor ~2.7 kflop, solve+residual
+# ~1.45 kflop, Nordsieck predict/correct/error/order logic ~1.1 kflop per step.
+F_RHS, X_RHS = 1292, 172
+F_SETUP, F_NEWTON, F_STEP = 2700, 1450, 1100
+PEAK_FP32 = 157.3e12
+PEAK_FP64 = 78.6e12
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_population(ncell: int):
+    pop = cep.workload.ord_population(20251204, 50, 4, 50)
+    # host-side scheduling: cells grouped by BCL so each wavefront holds equal-length beats
+    order = np.argsort(pop["bcl"], kind="stable")
+    if ncell < len(order):
+        order = order[np.linspace(0, len(order) - 1, ncell).astype(int)]
+    return {k: v[order] for k, v in pop.items()}
+
+
+def make_batch(pop, prec, nbeats_total):
+    n = len(pop["bcl"])
+    b = cep.CellBatch(cep.MODEL_ORD_CIPA, n, prec)
+    b.set_cells(celltype=pop["celltype"], conc=pop["conc"], herg=pop["herg"], gscale=pop["gscale"],
+                bcl=pop["bcl"])
+    b.set_run_config(dt=0.1, rtol=1e-6, atol=1e-8, maxord=5, nbeats_total=nbeats_total, nwin=nbeats_total)
+    b.init_consts()
+    return b
+
+
+def flops_of(stats: np.ndarray) -> float:
+    nst, nfe, nje, nsetups, nni = (stats[:, k].astype(float) for k in range(5))
+    nout = stats[:, 7].astype(float)
+    return float(np.sum((nfe + nout) * F_RHS + nsetups * F_SETUP + nni * F_NEWTON + nst * F_STEP))
+
+
+def cpu_baseline(seconds: float):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    import ctypes as C
+    threads = min(16, os.cpu_count() or 1)
+    pop = make_population(10000)
+    # bounded sample: one beat of a spread of cells from the same population, from rest
+    rng = np.random.default_rng(7)
+    ncell = max(threads, int(seconds * threads / 0.2))
+    idx = rng.choice(len(pop["bcl"]), size=ncell, replace=False)
+    cfgs = (O.OrdCellCfg * ncell)()
+    for k, i in enumerate(idx):
+        cfgs[k] = O.make_ord_cfg(celltype=pop["celltype"][i], conc=pop["conc"][i], herg=pop["herg"][i],
+                                 gscale=pop["gscale"][i], bcl=pop["bcl"][i], nbeats=1, nwin=1)
+    res = (O.BmResult * ncell)()
+    wall = O.lib().oracle_ord_run_many(cfgs, ncell, threads, res, None)
+    return {"value": ncell / wall, "unit": "cell-beats/s", "cores": threads, "kind": "port",
+            "sample": f"{ncell} cells x 1 beat of the same population (FP64 BDF oracle, "
+                      f"{threads} threads, {wall:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--ncell", type=int, default=10000)
+    ap.add_argument("--precision", choices=("mixed", "fp64"), default="mixed")
+    ap.add_argument("--check-cells", type=int, default=64)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "single-GPU build: run with --gpus 1 (cells shard trivially across "
+                          "GPUs, see DESIGN.md)"}))
+        sys.exit(2)
+    prec = cep.PREC_MIXED if args.precision == "mixed" else cep.PREC_FP64
+    K, W = args.steps, args.warmup
+    pop = make_population(args.ncell)
+    n = len(pop["bcl"])
+    nb_total = 200
+    b = make_batch(pop, prec, nb_total)
+    log(f"bench: {n} cells, warmup {W} beat(s)")
+    if W:
+        b.run_beats(W)
+    b.kernel_time()                     # reset the event timer
+    stats0 = b.stats()
+    log(f"bench: timing {K} beat(s)")
+    t0 = time.perf_counter()
+    for _ in range(K):
+        b.run_beats(1)
+    t1 = time.perf_counter()
+    nl, kavg_ms, ktot_ms = b.kernel_time()
+    stats1 = b.stats()
+    st = b.status()
+    bm_mixed = b.biomarker_array()
+    elapsed = t1 - t0
+    value = n * K / elapsed
+    flops_launch = flops_of(stats1 - stats0) / max(nl, 1)
+    achieved = flops_launch / (kavg_ms * 1e-3) if kavg_ms > 0 else 0.0
+    peak = PEAK_FP32 if prec == cep.PREC_MIXED else PEAK_FP64
+    nrhs = float(np.sum((stats1 - stats0)[:, [1, 7]]))
+    b.close()
+
+    # accuracy of the timed solver vs the FP64 solver on a subsample (same beats)
+    chk = {}
+    if args.check_cells > 0:
+        sel = np.linspace(0, n - 1, args.check_cells).astype(int)
+        sub = {k: v[sel] for k, v in pop.items()}
+        log(f"bench: FP64 check on {len(sel)} cells")
+        b64 = make_batch(sub, cep.PREC_FP64, nb_total)
+        t2 = time.perf_counter()
+        b64.run_beats(W + K)
+        t3 = time.perf_counter()
+        bm64 = b64.biomarker_array()
+        b64.close()
+        bi = {k: i for i, k in enumerate(cep.BM_FIELDS)}
+        chk = {
+            "apd90_max_abs_err_ms": float(np.abs(bm_mixed[sel, bi["apd90"]] - bm64[:, bi["apd90"]]).max()),
+            "qnet_max_rel_err": float((np.abs(bm_mixed[sel, bi["qnet"]] - bm64[:, bi["qnet"]])
+                                       / np.abs(bm64[:, bi["qnet"]])).max()),
+            "fp64_plain_job_cell_beats_per_s": len(sel) * (W + K) / (t3 - t2),
+            "cells_checked": int(len(sel)),
+        }
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("bench: CPU baseline")
+        cpu = cpu_baseline(args.cpu_seconds)
+    out = {
+        "metric": "ORd cell-beats/s on 1 GPU at APD90 err <=1 ms; p50/p99 job latency",
+        "value": value,
+        "unit": "cell-beats/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": 1e3 * elapsed / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32-rates/fp64-state" if prec == cep.PREC_MIXED else "f64",
+        "data": "synthetic (seeded drug/IC50/conductance population)",
+        "config": {"workload": "ORd-CiPA batched population, 10,000 cells = 50 drugs x 4 conc x 50 "
+                               "variants, BCL {500,1000,2000} ms, BDF rtol 1e-6/atol 1e-8, step = 1 beat",
+                   "cells": n, "beats_per_job": nb_total},
+        "accuracy_vs_fp64": chk,
+        "failed_cells": int(np.sum(st != 0)),
+        "latency_ms": {"p50": None, "p99": None},
+        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                     "kernel": f"ord_bdf_kernel<{'float' if prec == cep.PREC_MIXED else 'double'}>",
+                     "kernel_avg_ms": kavg_ms, "flops_per_launch": flops_launch,
+                     "rhs_evals_per_launch": nrhs / max(nl, 1)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,133 @@
+/*
+ * cep_cellsim.h -- C ABI of the MI355X cardiac cell-population engine (libcepgpu.so).
+ *
+ * Drop-in boundary for the reference's CellML cell-model interface:
+ *   class CellML (CEP5Cell/cellmodels/cellml.hpp:7-32): initConsts(type, conc, hill, drug),
+ *   computeRates(TIME, CONSTANTS, RATES, STATES, ALGEBRAIC), solveAnalytical(dt), public
+ *   STATES/RATES/ALGEBRAIC/CONSTANTS arrays;
+ * driven by the CVODE loops of the drug/population drivers
+ *   (CEP7Cell/modules/drugs_herg.cpp:140-300, CEPCell_NEO/modules/cipa_drug_bench.cpp:134-336,
+ *   CEP7Cell/modules/population_tn2006.cpp:70-204).
+ * Where the reference holds one model object per cell and integrates it on the CPU, this
+ * ABI holds a batch of N cells resident in device memory and integrates all of them in
+ * one launch.  Arrays crossing the ABI are host arrays in the reference's per-cell layout
+ * (cell-major, STATES order of the reference enums); no device pointers or framework types
+ * appear in the signatures.
+ *
+ * Errors: every call returns 0 on success, a negative CEP_E* code on failure;
+ * cep_last_error() gives the message.  A cell whose integration fails is flagged in its
+ * status word (CEP_ST_*) and stops; the other cells of the batch continue.
+ * Threading: a batch is used by one host thread at a time; different batches may run
+ * concurrently (each owns a HIP stream).
+ */
+#ifndef CEP_CELLSIM_H
+#define CEP_CELLSIM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CEP_MODEL_ORD_CIPA 1   /* O'Hara-Rudy CiPA v1 2017, 52 states (ohara_rudy_cipa_v1_2017.cpp) */
+#define CEP_MODEL_TT06 2       /* ten Tusscher 2006 IK1Ko endo/epi/M, 19 states */
+
+#define CEP_PREC_FP64 0        /* all arithmetic in FP64 */
+#define CEP_PREC_MIXED 1       /* gating/currents in FP32; state, Newton, LU, error control in FP64 */
+
+#define CEP_E_ARG -1
+#define CEP_E_HIP -2
+#define CEP_E_STATE -3
+
+#define CEP_ST_OK 0
+#define CEP_ST_CONV_FAIL 1     /* Newton failed MAXNCF times (CVODE CV_CONV_FAILURE) */
+#define CEP_ST_ERR_FAIL 2      /* error test failed MAXNEF times (CV_ERR_FAILURE) */
+#define CEP_ST_NONFINITE 3
+#define CEP_ST_MAX_STEPS 4     /* more than 500 steps between two output points (CV_TOO_MUCH_WORK) */
+
+/* biomarker record per cell (doubles), cipa_t of CEPCell_NEO/modules/commons.hpp:18-36 */
+#define CEP_BM_QNET 0          /* integral of INaL+ICaL+Ito+IKr+IKs+IK1 over the selected beat (uA/uF ms) */
+#define CEP_BM_INAL_AUC 1
+#define CEP_BM_ICAL_AUC 2
+#define CEP_BM_DVMDT_REPOL 3
+#define CEP_BM_DVMDT_MAX 4
+#define CEP_BM_VM_PEAK 5
+#define CEP_BM_VM_VALLEY 6
+#define CEP_BM_VM_DIA 7
+#define CEP_BM_APD90 8
+#define CEP_BM_APD50 9
+#define CEP_BM_CA_PEAK 10
+#define CEP_BM_CA_VALLEY 11
+#define CEP_BM_CA_DIA 12
+#define CEP_BM_CAD90 13
+#define CEP_BM_CAD50 14
+#define CEP_BM_STEEPEST_PACE 15
+#define CEP_NBM 16
+
+typedef struct cep_batch cep_batch;
+
+/* Create a batch of ncells cells of one model.  Replaces `new <model>()` per cell
+   (e.g. ohara_rudy_cipa_v1_2017.cpp:525-534). */
+cep_batch *cep_batch_create(int model, int ncells, int precision);
+void cep_batch_destroy(cep_batch *b);
+const char *cep_last_error(const cep_batch *b);
+/* model, ncells, states per cell, constants per cell (CellML layout), algebraic slots */
+int cep_batch_info(const cep_batch *b, int *model, int *ncells, int *nstates, int *nconsts, int *nalg);
+
+/* Per-cell inputs for cells [first, first+count).  celltype: 0 endo, 1 epi, 2 M.
+   ORd-CiPA: herg row = 14 IC50/Hill values in the IC50 CSV column order + Kmax, Ku, n,
+   halfmax, Vhalf (initConsts(celltype, conc, herg, drug), ohara_rudy_cipa_v1_2017.cpp:819-845);
+   gscale row = 7 population scales of GNa, GNaL, PCa, GKr, GKs, GK1, Gto
+   (CEPPatch_Population_Drugs/benchpatch.cpp:211-241 style CONSTANTS[g] *= scale).
+   TT06: herg row = IC50[IKr, ICaL, INa], Hill[IKr, ICaL, INa]; gscale row = 10 scales of
+   GNa, GK1, GKr, GKs, Gto, GCaL, GbNa, GbCa, GpCa, GpK (population_tn2006.cpp:91-100).
+   bcl: basic cycle length (ms) -> stimulus period.  Null gscale = all ones. */
+int cep_set_cells(cep_batch *b, int first, int count, const double *celltype, const double *conc,
+                  const double *herg, int herg_stride, const double *gscale, int gscale_stride,
+                  const double *bcl);
+
+/* Integration / biomarker configuration shared by the batch.
+   dt: output step (ms; also CVODE max step for ORd, the Rush-Larsen step for TT06);
+   rtol/atol: BDF scalar tolerances (CVodeSStolerances); maxord: BDF max order (5);
+   nbeats_total: beats of the whole job (biomarkers scan its trailing nwin beats);
+   stim_amp_scale: multiplies the stimulus amplitude (drugs_herg.cpp:150). */
+int cep_set_run_config(cep_batch *b, double dt, double rtol, double atol, int maxord,
+                       int nbeats_total, int nwin, double stim_amp_scale);
+
+/* Build per-cell constants and initial states, reset integrators and biomarkers
+   (initConsts + CVodeInit/CVodeReInit). */
+int cep_init_consts(cep_batch *b);
+
+/* CONSTANTS of one cell in the reference's CellML layout (206 ORd / 53 TT06). */
+int cep_get_constants(const cep_batch *b, int cell, double *out);
+
+/* computeRates for every cell at time t: states/rates n x nstates, algebraic n x nalg
+   (reference ALGEBRAIC indices; may be null).  Evaluated on the device. */
+int cep_compute_rates(cep_batch *b, double t, const double *states, double *rates, double *algebraic);
+
+/* Advance every cell by nbeats beats of its own BCL (device-resident state, biomarkers
+   updated on the device).  Blocks until done.  The _async form returns after the launch on
+   the batch stream; cep_sync waits for it. */
+int cep_run_beats(cep_batch *b, int nbeats);
+int cep_run_beats_async(cep_batch *b, int nbeats);
+int cep_sync(cep_batch *b);
+
+/* Current STATES (n x nstates); set_states also restarts the integrators at t = 0 from
+   the given states (the reference's Is_Using_Output restart, cipa_drug_bench.cpp:151). */
+int cep_get_states(const cep_batch *b, double *out);
+int cep_set_states(cep_batch *b, const double *in);
+
+int cep_get_biomarkers(const cep_batch *b, double *out);        /* n x CEP_NBM */
+int cep_get_status(const cep_batch *b, int *status);             /* n */
+/* n x 8: steps, rhs evals, jacobians, setups, newton iters, conv fails, err fails, outputs */
+int cep_get_stats(const cep_batch *b, long long *out);
+/* Vm at every output point of the most recent beat (enable before running). */
+int cep_enable_vm_trace(cep_batch *b, int enable);
+int cep_get_vm_trace(const cep_batch *b, double *out, int *nsamp);  /* n x nsamp */
+
+/* average duration (ms) of the integration kernel over the launches since the last
+   call, timed with HIP events on the batch stream; returns launches counted */
+int cep_kernel_time(cep_batch *b, double *avg_ms, double *total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -49,7 +49,8 @@ int oracle_ord_run(const ord_cell_cfg_t *cfg, double *states_io, int use_states_
   bdf_init(m, t0, S);
   const int imax = (int)((cfg->nbeats * ctx.C[15]) / cfg->dt);
   const int beat_steps = (int)(ctx.C[15] / cfg->dt);
-  bm_state_t *b = bm_create(ctx.C[15], cfg->dt, ctx.C[12], cfg->nbeats, cfg->nwin, 1);
+  /* Ca trace sampled every dt_write = 1 ms (reference default, CEP7Cell/commons/helper.cpp:13) */
+  bm_state_t *b = bm_create(ctx.C[15], cfg->dt, ctx.C[12], cfg->nbeats, cfg->nwin, (int)((1. / cfg->dt) * 1.0));
   bm_begin(b, S[0], S[9]);
   double tnext = cfg->dt;
   int iout = 0, rc = 0;
@@ -92,7 +93,7 @@ int oracle_tt_run(const tt_cell_cfg_t *cfg, double *states_io, int use_states_in
   memset(A, 0, sizeof(A));
   const int imax = (int)((cfg->nbeats * C[6]) / cfg->dt);
   const int beat_steps = (int)(C[6] / cfg->dt);
-  bm_state_t *b = bm_create(C[6], cfg->dt, C[5], cfg->nbeats, cfg->nwin, 1);
+  bm_state_t *b = bm_create(C[6], cfg->dt, C[5], cfg->nbeats, cfg->nwin, (int)((1. / cfg->dt) * 1.0));
   bm_begin(b, S[0], S[3]);
   double tcurr = 0.0;
   int iout = 0;
