@@ -8,21 +8,19 @@
 // Newton, DQ Jacobian with CVODE's reuse rules, local error test, order/step selection,
 // initial step estimate) -- the same steps as the FP64 CPU oracle.
 //
-// GPU organisation (one cell per lane):
-//   * The per-lane algorithm is a resumable phase machine that stops every time it needs
-//     a right-hand side.  The wavefront loop evaluates exactly one RHS per lane per
-//     iteration -- a Newton iterate, a Jacobian column or an output point -- so the
-//     dominant cost never diverges even though lanes are in different phases (a lane
-//     building its 48-column Jacobian does not stall the others).
-//   * Newton systems use the Jacobian's structure: 26 Hodgkin-Huxley gates couple only
-//     to themselves and V, nca to jca/cass, the 9-state Markov IKr block to V, the three
-//     charge accumulators to nothing downstream, D is constant.  Eliminating gates and
-//     the Markov block leaves a 12x12 core (CaMKt, 8 concentrations, 2 Jrel, V last).
-//     With V ordered last the Markov block (an M-matrix) and the core factor without
-//     pivoting; the solution is that of the dense system built from the same
-//     finite-difference entries (entries outside the pattern are exactly zero).
-//   * Nordsieck history, error weights, the FD Jacobian and the factors live in HBM as
-//     SoA (element k of cell c at k*pitch + c); scalars stay in registers.
+// This header holds the per-cell algorithm as operations on one cell (`Lane`): the
+// integrator scalars (`Mach`) sit in LDS, the vectors (Nordsieck array, error weights,
+// Newton correction, FD Jacobian, factors) sit in HBM as SoA (element k of cell c at
+// k*pitch + c).  The kernel in cep_gpu.hip schedules these operations so that every pass
+// of a wavefront evaluates one right-hand side per lane.
+//
+// Newton systems use the Jacobian's structure: 26 Hodgkin-Huxley gates couple only to
+// themselves and V, nca to jca/cass, the 9-state Markov IKr block to V, the three charge
+// accumulators to nothing downstream, D is constant.  Eliminating gates and the Markov
+// block leaves a 12x12 core (CaMKt, 8 concentrations, 2 Jrel, V last).  With V ordered
+// last the Markov block (an M-matrix) and the core factor without pivoting; the solution
+// is that of the dense system built from the same finite-difference entries (entries
+// outside the pattern are exactly zero).
 #pragma once
 #include "biomarker.hpp"
 #include "ord_model.hpp"
@@ -37,7 +35,7 @@ constexpr int NJC = 48;     // Jacobian columns evaluated (all states but D and 
 constexpr int NG = 26;      // simple gates
 constexpr int F_RG = 0, F_CG = NG, F_NCA = 2 * NG, F_MLU = 2 * NG + 4, F_W = F_MLU + 81, F_SLU = F_W + 9;
 constexpr int NFAC = F_SLU + 144;
-// scalar save area
+// scalar save area between launches
 constexpr int SD_N = 40;    // doubles
 constexpr int SI_N = 24;    // ints
 
@@ -45,15 +43,19 @@ constexpr int SI_N = 24;    // ints
 constexpr double ETAMX1 = 10000.0, ETAMX2 = 10.0, ETAMX3 = 10.0, ETAMXF = 0.2, ETAMIN = 0.1;
 constexpr double ETACF = 0.25, ADDON = 1e-6, BIAS1 = 6.0, BIAS2 = 6.0, BIAS3 = 10.0, THRESH = 1.5;
 constexpr int SMALL_NST = 10, MXNEF1 = 3, SMALL_NEF = 2, LONG_WAIT = 10, MAXNEF = 7, MAXNCF = 10;
-constexpr int MAXCOR = 3, MSBP = 20, MSBJ = 50;
+constexpr int MAXCOR = 3, MSBP = 20, MSBJ = 50, MAX_ITERS = 4;
 constexpr double NLSCOEF = 0.1, CRDOWN = 0.3, RDIV = 2.0, DGMAX = 0.3, DGMAX_J = 0.2;
 constexpr double HLB_FACTOR = 100.0, HUB_FACTOR = 0.1, H_BIAS = 0.5;
 constexpr double UROUND = 2.220446049250313e-16;
 
 enum { FIRST_CALL = 0, PREV_CONV_FAIL = 1, PREV_ERR_FAIL = 2 };
 enum { NO_FAILURES = 0, FAIL_BAD_J = 1, FAIL_OTHER = 2 };
-// phases: what the pending RHS request is for
-enum { PH_FRESH = 0, PH_INIT_F, PH_HIN, PH_STEP_F0, PH_JAC, PH_NEWTON, PH_OUTPUT, PH_RESTART_F, PH_READY, PH_DONE };
+enum { NLS_OK = 0, NLS_ITER = 1, NLS_TRY_AGAIN = 2, NLS_CONV_FAIL = 3 };
+// where a cell's integrator is; PH_F0 .. PH_HIN end a pass with a right-hand-side request
+enum {
+  PH_FRESH = 0, PH_INIT_F, PH_HIN, PH_START, PH_PREDICT, PH_F0, PH_WAIT_JAC, PH_FACTOR,
+  PH_NEWTON, PH_ITER, PH_CONVFAIL, PH_ETEST, PH_RESTART_F, PH_STEP_DONE, PH_DONE
+};
 
 __device__ constexpr int kSimpleGates[NG] = { 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22,
                                               23, 24, 25, 26, 27, 28, 29, 30, 31, 32, 44, 45, 46 };
@@ -83,9 +85,10 @@ CEP_HD constexpr bool couples(int row, int g)
 
 struct Buffers {
   int ncell, pitch;
-  double *y;        // [52] current output state (CellML STATES)
-  double *zn;       // [6*52]
-  double *ewt, *acor, *f0;   // [52]
+  double *y;        // [52] state at the last output point (CellML STATES)
+  // integrator vectors, in blocks of CPB cells
+  double *zn;       // [6*52] Nordsieck array
+  double *ewt, *acor, *ft;   // [52]: error weights, Newton correction, f at the current iterate
   double *jd;       // [48*52] FD Jacobian, column slot-major
   double *fac;      // [NFAC]
   double *sd;       // [SD_N] scalar save
@@ -99,211 +102,287 @@ struct Buffers {
   long long *stats; // [8] nst nfe nje nsetups nni ncfn netf nout
   int *status;
   double dt, rtol, atol, uround_fd;
-  int maxord, nwin_unused;
+  int maxord, max_steps;
   int cap;
 };
 
-struct Param {
-  const double *p;
-  int pitch;
-  CEP_D double operator()(int k) const { return p[(size_t)k * pitch]; }
+// integrator scalars of one cell (LDS-resident during a launch)
+struct Mach {
+  double tn, h, hprime, eta, hscale, etamax, rl1, gamma, gammap, gamrat, crate, acnrm, del, delp;
+  double saved_t, tq1, tq2, tq3, tq4, tq5, saved_tq5, tout, hg, hub, hlb, hnew, mininc;
+  double tau[7], l[6];
+  int q, qprime, qwait, jcur, phase, callSetup, convfail, nflag, ncf, nef, mnewt;
+  int iout, iout_end, status, hin_count, hin_ok, nstloc, jend;
+  int sw_ops, sw_dq, sw_q1;   // Nordsieck-array work pending for the next attempt
+  long long nst, nstlp, nstlj, nfe, nje, nsetups, nni, ncfn, netf;
 };
 
+// The integrator's per-cell vectors are stored in blocks of CPB cells, one block per
+// workgroup: element k of the cell in block slot `ln` is at base[k*CPB + ln].  They are
+// accessed with buffer instructions through one resource descriptor per vector (scalar
+// registers), the slot offset ln*8 (one vector register) and a compile-time byte offset
+// split into the instruction's immediate (< 4 KiB) and a scalar constant: no per-lane
+// address arithmetic, and accesses outside a block's range read 0 / are dropped.
+constexpr int CPB = 16;
+
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+CEP_D inline Rsrc make_rsrc(const double *base, unsigned bytes)
+{
+  return __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, (int)bytes, 0x00020000);
+}
+
+// one double of a blocked vector, readable and assignable like a reference
+struct BElem {
+  Rsrc r;
+  unsigned voff;
+  int byte;
+  CEP_D operator double() const
+  {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, byte, 0));
+  }
+  CEP_D const BElem &operator=(double v) const
+  {
+    typedef __attribute__((ext_vector_type(2))) unsigned u2;
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, voff, byte, 0);
+    return *this;
+  }
+};
+
+// a lane's 52-double staging vector in LDS, laid out [component][lane]
+struct LVec {
+  double *p;
+  CEP_D double &operator[](int i) const { return p[i * 64]; }
+};
+
+struct BlockBufs {
+  Rsrc zn, ewt, acor, ft, jd, fac;
+};
+
+// one cell's view: buffers (kernel arguments, wave-uniform), its LDS scalars, the block
+// of integrator vectors, its slot in the block, its index and the pitch of the per-cell
+// (SoA: element k of cell c at k*p + c) buffers shared with the host.  The kernel passes
+// the pitch re-read every pass, which keeps the compiler from hoisting loop-invariant
+// addresses of those buffers out of the integration loop.
 template <typename R>
-struct Machine {
+struct Lane {
   const Buffers &B;
-  const int c;
-  const Param P;
-  // scalars
-  double tn, h, hprime, eta, hscale, etamax, rl1, gamma, gammap, gamrat, crate, acnrm, del, delp;
-  double saved_t, tq1, tq2, tq3, tq4, tq5, saved_tq5, tout, hg, hub, hlb, hnew, hs;
-  double tau[7], l[6];
-  int q, qprime, qwait, jcur, phase, jcol, callSetup, convfail, nflag, ncf, nef, mnewt;
-  int iout, iout_end, status, hin_count, hin_ok, nstloc;
-  long long nst, nstlp, nstlj, nfe, nje, nsetups, nni, ncfn, netf;
-  double inc_cur;
+  Mach &m;
+  const BlockBufs &V;
+  unsigned ln;
+  unsigned c;
+  int p;
 
-  CEP_D Machine(const Buffers &b, int cell) : B(b), c(cell), P{b.prm + cell, b.pitch} {}
-
-  // SoA accessors
-  CEP_D double &ZN(int j, int i) const { return B.zn[(size_t)(j * N + i) * B.pitch + c]; }
-  CEP_D double &EWT(int i) const { return B.ewt[(size_t)i * B.pitch + c]; }
-  CEP_D double &ACOR(int i) const { return B.acor[(size_t)i * B.pitch + c]; }
-  CEP_D double &F0(int i) const { return B.f0[(size_t)i * B.pitch + c]; }
-  CEP_D double &JD(int slot, int row) const { return B.jd[(size_t)(slot * N + row) * B.pitch + c]; }
+  // (the mask tells the compiler the slot offset is small, so constants fold into the
+  // instruction's immediate offset)
+  CEP_D BElem blk(Rsrc r, int k) const { return BElem{r, (ln & 63u) * 8u, k * CPB * 8}; }
+  CEP_D BElem ZN(int j, int i) const { return blk(V.zn, j * N + i); }
+  CEP_D BElem EWT(int i) const { return blk(V.ewt, i); }
+  CEP_D BElem ACOR(int i) const { return blk(V.acor, i); }
+  CEP_D BElem FT(int i) const { return blk(V.ft, i); }
+  CEP_D BElem JD(int slot, int row) const { return blk(V.jd, slot * N + row); }
   CEP_D double J(int row, int col) const { return JD(jslot(col), row); }
-  CEP_D double &FAC(int k) const { return B.fac[(size_t)k * B.pitch + c]; }
-  CEP_D double &Y(int i) const { return B.y[(size_t)i * B.pitch + c]; }
+  CEP_D BElem FAC(int k) const { return blk(V.fac, k); }
+  CEP_D double &Y(int i) const { return B.y[i * p + c]; }
 
-  // ---------------- persistence
+  // ---------------- persistence between launches
   CEP_D void load()
   {
     const double *d = B.sd + c;
-    const size_t p = B.pitch;
-    double *dv[SD_N] = { &tn, &h, &hprime, &eta, &hscale, &etamax, &rl1, &gamma, &gammap, &gamrat,
-                         &crate, &acnrm, &del, &delp, &saved_t, &tq1, &tq2, &tq3, &tq4, &tq5,
-                         &saved_tq5, &tout, &hg, &hub, &hlb, &hnew, &hs, &tau[0], &tau[1], &tau[2],
-                         &tau[3], &tau[4], &tau[5], &tau[6], &l[0], &l[1], &l[2], &l[3], &l[4], &l[5] };
+    double *dv[SD_N] = { &m.tn, &m.h, &m.hprime, &m.eta, &m.hscale, &m.etamax, &m.rl1, &m.gamma,
+                         &m.gammap, &m.gamrat, &m.crate, &m.acnrm, &m.del, &m.delp, &m.saved_t,
+                         &m.tq1, &m.tq2, &m.tq3, &m.tq4, &m.tq5, &m.saved_tq5, &m.tout, &m.hg,
+                         &m.hub, &m.hlb, &m.hnew, &m.mininc, &m.tau[0], &m.tau[1], &m.tau[2],
+                         &m.tau[3], &m.tau[4], &m.tau[5], &m.tau[6], &m.l[0], &m.l[1], &m.l[2],
+                         &m.l[3], &m.l[4], &m.l[5] };
 #pragma unroll
     for (int k = 0; k < SD_N; ++k) *dv[k] = d[k * p];
     const int *s = B.si + c;
-    int *iv[15] = { &q, &qprime, &qwait, &jcur, &phase, &jcol, &callSetup, &convfail, &nflag, &ncf,
-                    &nef, &mnewt, &iout, &hin_count, &hin_ok };
+    int *iv[14] = { &m.q, &m.qprime, &m.qwait, &m.jcur, &m.phase, &m.callSetup, &m.convfail,
+                    &m.nflag, &m.ncf, &m.nef, &m.mnewt, &m.iout, &m.hin_count, &m.hin_ok };
 #pragma unroll
-    for (int k = 0; k < 15; ++k) *iv[k] = s[k * p];
-    nst = s[15 * p]; nstlp = s[16 * p]; nstlj = s[17 * p];
-    status = B.status[c];
+    for (int k = 0; k < 14; ++k) *iv[k] = s[k * p];
+    m.nst = s[14 * p]; m.nstlp = s[15 * p]; m.nstlj = s[16 * p];
+    m.status = B.status[c];
     const long long *st = B.stats + c;
-    nfe = st[1 * p]; nje = st[2 * p]; nsetups = st[3 * p]; nni = st[4 * p]; ncfn = st[5 * p]; netf = st[6 * p];
+    m.nfe = st[1 * p]; m.nje = st[2 * p]; m.nsetups = st[3 * p]; m.nni = st[4 * p];
+    m.ncfn = st[5 * p]; m.netf = st[6 * p];
   }
   CEP_D void store() const
   {
     double *d = B.sd + c;
-    const size_t p = B.pitch;
-    const double dv[SD_N] = { tn, h, hprime, eta, hscale, etamax, rl1, gamma, gammap, gamrat, crate,
-                              acnrm, del, delp, saved_t, tq1, tq2, tq3, tq4, tq5, saved_tq5, tout,
-                              hg, hub, hlb, hnew, hs, tau[0], tau[1], tau[2], tau[3], tau[4], tau[5],
-                              tau[6], l[0], l[1], l[2], l[3], l[4], l[5] };
+    const double dv[SD_N] = { m.tn, m.h, m.hprime, m.eta, m.hscale, m.etamax, m.rl1, m.gamma,
+                              m.gammap, m.gamrat, m.crate, m.acnrm, m.del, m.delp, m.saved_t,
+                              m.tq1, m.tq2, m.tq3, m.tq4, m.tq5, m.saved_tq5, m.tout, m.hg, m.hub,
+                              m.hlb, m.hnew, m.mininc, m.tau[0], m.tau[1], m.tau[2], m.tau[3],
+                              m.tau[4], m.tau[5], m.tau[6], m.l[0], m.l[1], m.l[2], m.l[3], m.l[4],
+                              m.l[5] };
 #pragma unroll
     for (int k = 0; k < SD_N; ++k) d[k * p] = dv[k];
     int *s = B.si + c;
-    const int iv[15] = { q, qprime, qwait, jcur, phase, jcol, callSetup, convfail, nflag, ncf, nef,
-                         mnewt, iout, hin_count, hin_ok };
+    const int iv[14] = { m.q, m.qprime, m.qwait, m.jcur, m.phase, m.callSetup, m.convfail,
+                         m.nflag, m.ncf, m.nef, m.mnewt, m.iout, m.hin_count, m.hin_ok };
 #pragma unroll
-    for (int k = 0; k < 15; ++k) s[k * p] = iv[k];
-    s[15 * p] = (int)nst; s[16 * p] = (int)nstlp; s[17 * p] = (int)nstlj;
-    B.status[c] = status;
+    for (int k = 0; k < 14; ++k) s[k * p] = iv[k];
+    s[14 * p] = (int)m.nst; s[15 * p] = (int)m.nstlp; s[16 * p] = (int)m.nstlj;
+    B.status[c] = m.status;
     long long *st = B.stats + c;
-    st[0] = nst; st[1 * p] = nfe; st[2 * p] = nje; st[3 * p] = nsetups; st[4 * p] = nni;
-    st[5 * p] = ncfn; st[6 * p] = netf; st[7 * p] = iout;
+    st[0] = m.nst; st[1 * p] = m.nfe; st[2 * p] = m.nje; st[3 * p] = m.nsetups; st[4 * p] = m.nni;
+    st[5 * p] = m.ncfn; st[6 * p] = m.netf; st[7 * p] = m.iout;
   }
 
-  // ---------------- small helpers
-  CEP_D double wrms_reg(const double *x) const
+  // ---------------- vector sweeps over the 52 components.  A sweep issues the loads of a
+  // chunk of CH components together, computes in registers, then stores the chunk: the
+  // compiler cannot move a load above a store to the same buffers, so per-component
+  // load/store interleaving would serialise every load on memory latency.
+  static constexpr int CH = 13;
+  static_assert(N % CH == 0, "whole chunks");
+
+  // CVODE's weights from zn[0]: 1 / (rtol |y| + atol)
+  CEP_D void set_ewt()
   {
-    double s = 0.0;
+    const double rt = B.rtol, at_ = B.atol;
+    double w[N];
 #pragma unroll
-    for (int i = 0; i < N; ++i) { const double t = x[i] * EWT(i); s += t * t; }
-    return sqrt(s * (1.0 / N));
-  }
-  CEP_D void set_ewt_from_zn0()
-  {
+    for (int i = 0; i < N; ++i) w[i] = 1.0 / (rt * fabs(ZN(0, i)) + at_);
 #pragma unroll
-    for (int i = 0; i < N; ++i) EWT(i) = 1.0 / (B.rtol * fabs(ZN(0, i)) + B.atol);
+    for (int i = 0; i < N; ++i) EWT(i) = w[i];
   }
 
-  CEP_D void predict()
+  // Pascal-triangle update of one component's Nordsieck column (CVODE's prediction,
+  // sign +1, and its undo, sign -1), order q
+  CEP_D static void pascal_reg(double *z, int q, int sign)
   {
-    tn += h;
 #pragma unroll
-    for (int i = 0; i < N; ++i) {
-      double z[QMAX + 1];
+    for (int k = 1; k <= QMAX; ++k)
 #pragma unroll
-      for (int j = 0; j <= QMAX; ++j) z[j] = (j <= q) ? ZN(j, i) : 0.0;
-#pragma unroll
-      for (int k = 1; k <= QMAX; ++k)
-#pragma unroll
-        for (int j = QMAX; j >= k; --j)
-          if (k <= q && j <= q) z[j - 1] += z[j];
-#pragma unroll
-      for (int j = 0; j <= QMAX; ++j) if (j < q) ZN(j, i) = z[j];
-    }
+      for (int j = QMAX; j >= k; --j)
+        if (k <= q && j <= q) z[j - 1] = sign > 0 ? z[j - 1] + z[j] : z[j - 1] - z[j];
   }
-  __device__ __noinline__ void restore()
-  {
-    tn = saved_t;
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      double z[QMAX + 1];
-#pragma unroll
-      for (int j = 0; j <= QMAX; ++j) z[j] = (j <= q) ? ZN(j, i) : 0.0;
-#pragma unroll
-      for (int k = 1; k <= QMAX; ++k)
-#pragma unroll
-        for (int j = QMAX; j >= k; --j)
-          if (k <= q && j <= q) z[j - 1] -= z[j];
-#pragma unroll
-      for (int j = 0; j <= QMAX; ++j) if (j < q) ZN(j, i) = z[j];
-    }
-  }
-  __device__ __noinline__ void rescale()
+
+  // SW_RESTART: after the restore, evaluate f(tn, zn[0]) for zn[1] instead of predicting
+  enum { SW_RESTORE = 1, SW_INC = 2, SW_DEC = 4, SW_RESCALE = 8, SW_PREDICT = 16, SW_RESTART = 32 };
+  // Everything done to the Nordsieck array between two nonlinear solves, in CVODE's order,
+  // in one sweep: undo a failed prediction (order q0), change the order (cvIncreaseBDF /
+  // cvDecreaseBDF with coefficients ll and a1, order q0), rescale row j by eta^j (order
+  // q1), predict (order q1).  y receives the resulting zn[0].
+  template <typename YV>
+  CEP_D void attempt_sweep(int ops, int q0, int q1, const double *ll, double a1, double eta, const YV &y)
   {
     double fac[QMAX + 1];
     fac[0] = 1.0;
 #pragma unroll
     for (int j = 1; j <= QMAX; ++j) fac[j] = fac[j - 1] * eta;
+    const int qs = q0 > q1 ? q0 : q1;          // highest row written
+    const bool inc = (ops & SW_INC) != 0;       // reads the correction saved in zn[QMAX]
 #pragma unroll
-    for (int i = 0; i < N; ++i)
+    for (int c0 = 0; c0 < N; c0 += CH) {
+      double z[CH][QMAX + 1];
 #pragma unroll
-      for (int j = 1; j <= QMAX; ++j)
-        if (j <= q) ZN(j, i) *= fac[j];
-    h = hscale * eta;
-    hscale = h;
+      for (int k = 0; k < CH; ++k)
+#pragma unroll
+        for (int j = 0; j <= QMAX; ++j) z[k][j] = (j <= qs || (inc && j == QMAX)) ? ZN(j, c0 + k) : 0.0;
+#pragma unroll
+      for (int k = 0; k < CH; ++k) {
+        double *zz = z[k];
+        if (ops & SW_RESTORE) pascal_reg(zz, q0, -1);
+        if (inc) {
+          const double zl = a1 * zz[QMAX];
+#pragma unroll
+          for (int j = 2; j <= QMAX; ++j) if (j <= q0) zz[j] += ll[j] * zl;
+#pragma unroll
+          for (int j = 2; j <= QMAX; ++j) if (j == q0 + 1) zz[j] = zl;
+        }
+        if (ops & SW_DEC) {
+          double zq = 0.0;
+#pragma unroll
+          for (int j = 2; j <= QMAX; ++j) if (j == q0) zq = zz[j];
+#pragma unroll
+          for (int j = 2; j < QMAX; ++j) if (j < q0) zz[j] -= ll[j] * zq;
+        }
+        if (ops & SW_RESCALE) {
+#pragma unroll
+          for (int j = 1; j <= QMAX; ++j) if (j <= q1) zz[j] *= fac[j];
+        }
+        if (ops & SW_PREDICT) pascal_reg(zz, q1, 1);
+        y[c0 + k] = zz[0];
+      }
+#pragma unroll
+      for (int k = 0; k < CH; ++k)
+#pragma unroll
+        for (int j = 0; j <= QMAX; ++j) if (j <= qs) ZN(j, c0 + k) = z[k][j];
+    }
   }
 
   CEP_D void set_bdf()
   {
-    double xi_inv = 1.0, xistar_inv = 1.0, alpha0 = -1.0, alpha0_hat = -1.0, hsum = h;
+    const int q = m.q;
+    double l[QMAX + 1];
+    double xi_inv = 1.0, xistar_inv = 1.0, alpha0 = -1.0, alpha0_hat = -1.0, hsum = m.h;
     l[0] = l[1] = 1.0;
 #pragma unroll
     for (int i = 2; i <= QMAX; ++i) l[i] = 0.0;
     if (q > 1) {
       for (int j = 2; j < q; ++j) {
-        hsum += tau[j - 1];
-        xi_inv = h / hsum;
+        hsum += m.tau[j - 1];
+        xi_inv = m.h / hsum;
         alpha0 -= 1.0 / j;
 #pragma unroll
         for (int i = QMAX; i >= 1; --i) if (i <= j) l[i] += l[i - 1] * xi_inv;
       }
       alpha0 -= 1.0 / q;
       xistar_inv = -l[1] - alpha0;
-      hsum += tau[q - 1];
-      xi_inv = h / hsum;
+      hsum += m.tau[q - 1];
+      xi_inv = m.h / hsum;
       alpha0_hat = -l[1] - xi_inv;
 #pragma unroll
       for (int i = QMAX; i >= 1; --i) if (i <= q) l[i] += l[i - 1] * xistar_inv;
     }
+    double lq = l[1];
+#pragma unroll
+    for (int i = 2; i <= QMAX; ++i) if (i == q) lq = l[i];
+#pragma unroll
+    for (int i = 0; i <= QMAX; ++i) m.l[i] = l[i];
     // error-test / order-change coefficients
-    const double lq = l[q < 1 ? 1 : (q > QMAX ? QMAX : q)];
     const double A1 = 1.0 - alpha0_hat + alpha0;
     const double A2 = 1.0 + q * A1;
-    tq2 = fabs(A1 / (alpha0 * A2));
-    tq5 = fabs(A2 * xistar_inv / (lq * xi_inv));
-    if (qwait == 1) {
+    m.tq2 = fabs(A1 / (alpha0 * A2));
+    m.tq5 = fabs(A2 * xistar_inv / (lq * xi_inv));
+    if (m.qwait == 1) {
       if (q > 1) {
         const double Cc = xistar_inv / lq;
         const double A3 = alpha0 + 1.0 / q;
         const double A4 = alpha0_hat + xi_inv;
         const double Cpinv = (1.0 - A4 + A3) / A3;
-        tq1 = fabs(Cc * Cpinv);
+        m.tq1 = fabs(Cc * Cpinv);
       } else {
-        tq1 = 1.0;
+        m.tq1 = 1.0;
       }
-      const double hsum2 = hsum + tau[q];
-      const double xi2 = h / hsum2;
+      const double hsum2 = hsum + m.tau[q];
+      const double xi2 = m.h / hsum2;
       const double A5 = alpha0 - (1.0 / (q + 1));
       const double A6 = alpha0_hat - xi2;
       const double Cppinv = (1.0 - A6 + A5) / A2;
-      tq3 = fabs(Cppinv / (xi2 * (q + 2) * A5));
+      m.tq3 = fabs(Cppinv / (xi2 * (q + 2) * A5));
     }
-    tq4 = NLSCOEF / tq2;
-    rl1 = 1.0 / l[1];
-    gamma = h * rl1;
-    if (nst == 0) gammap = gamma;
-    gamrat = (nst > 0) ? gamma / gammap : 1.0;
+    m.tq4 = NLSCOEF / m.tq2;
+    m.rl1 = 1.0 / l[1];
+    m.gamma = m.h * m.rl1;
+    if (m.nst == 0) m.gammap = m.gamma;
+    m.gamrat = (m.nst > 0) ? m.gamma / m.gammap : 1.0;
   }
 
-  CEP_D void increase_order()
+  // cvIncreaseBDF coefficients at order q: zn[q+1] = a1 zn[qmax], zn[j] += ll[j] zn[q+1]
+  CEP_D double inc_coeffs(double *ll) const
   {
-    double ll[QMAX + 1];
+    const int q = m.q;
 #pragma unroll
     for (int i = 0; i <= QMAX; ++i) ll[i] = 0.0;
-    double alpha1 = 1.0, prod = 1.0, xiold = 1.0, alpha0 = -1.0, hsum = hscale;
+    double alpha1 = 1.0, prod = 1.0, xiold = 1.0, alpha0 = -1.0, hsum = m.hscale;
     ll[2] = 1.0;
     if (q > 1) {
       for (int j = 1; j < q; ++j) {
-        hsum += tau[j + 1];
-        const double xi = hsum / hscale;
+        hsum += m.tau[j + 1];
+        const double xi = hsum / m.hscale;
         prod *= xi;
         alpha0 -= 1.0 / (j + 1);
         alpha1 += 1.0 / xi;
@@ -312,71 +391,67 @@ struct Machine {
         xiold = xi;
       }
     }
-    const double A1 = (-alpha0 - alpha1) / prod;
-    // zn[q+1] = A1 * zn[qmax] (acor saved there); zn[j] += l[j] * zn[q+1]
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      const double zl = A1 * ZN(QMAX, i);
-#pragma unroll
-      for (int j = 2; j <= QMAX; ++j) if (j <= q) ZN(j, i) += ll[j] * zl;
-#pragma unroll
-      for (int j = 2; j <= QMAX; ++j) if (j == q + 1) ZN(j, i) = zl;
-    }
+    return (-alpha0 - alpha1) / prod;
   }
-  CEP_D void decrease_order()
+  // cvDecreaseBDF coefficients at order q: zn[j] -= ll[j] zn[q], 2 <= j < q
+  CEP_D void dec_coeffs(double *ll) const
   {
-    double ll[QMAX + 1];
+    const int q = m.q;
 #pragma unroll
     for (int i = 0; i <= QMAX; ++i) ll[i] = 0.0;
     ll[2] = 1.0;
     double hsum = 0.0;
     for (int j = 1; j <= q - 2; ++j) {
-      hsum += tau[j];
-      const double xi = hsum / hscale;
+      hsum += m.tau[j];
+      const double xi = hsum / m.hscale;
 #pragma unroll
       for (int i = QMAX; i >= 2; --i) if (i <= j + 2) ll[i] = ll[i] * xi + ll[i - 1];
     }
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      double zq = 0.0;
-#pragma unroll
-      for (int j = 0; j <= QMAX; ++j) if (j == q) zq = ZN(j, i);
-#pragma unroll
-      for (int j = 2; j < QMAX; ++j) if (j < q) ZN(j, i) -= ll[j] * zq;
-    }
   }
-  CEP_D void adjust_order(int dq)
+  // order change dq at the current order (CVODE's cvAdjustOrder: no decrease below 2),
+  // folded into `ops`/`ll`/`a1` for attempt_sweep
+  CEP_D int order_ops(int dq, double *ll, double &a1) const
   {
-    if ((q == 2) && (dq != 1)) return;
-    if (dq == 1) increase_order(); else if (dq == -1) decrease_order();
+    a1 = 0.0;
+    if ((m.q == 2) && (dq != 1)) return 0;
+    if (dq == 1) { a1 = inc_coeffs(ll); return SW_INC; }
+    if (dq == -1) { dec_coeffs(ll); return SW_DEC; }
+    return 0;
   }
 
-  // ---------------- linear algebra: structured factorisation of M = I - gamma*J
+  // ---------------- linear algebra: structured factorisation of M = I - gamma*J.
+  // Each section loads its Jacobian entries, computes in registers and stores its factors.
   CEP_D bool factor()
   {
-    const double g = gamma;
-    // simple gates
+    const double g = m.gamma;
+    // simple gates: x_g = r_g b_g - c_g x_V
+    double rg[NG], cg[NG];
 #pragma unroll
     for (int k = 0; k < NG; ++k) {
       const int s = kSimpleGates[k];
       const double r = 1.0 / (1.0 - g * J(s, s));
-      FAC(F_RG + k) = r;
-      FAC(F_CG + k) = (-g * J(s, 0)) * r;
+      rg[k] = r;
+      cg[k] = (-g * J(s, 0)) * r;
     }
     // nca: row {cass, jca, nca}
     const double rn = 1.0 / (1.0 - g * J(33, 33));
     const double e_nj = rn * (-g * J(33, 30));
-    const double c_jca = FAC(F_CG + 20);   // jca is simple gate #20
+    const double cnV = -e_nj * cg[20];           // jca is simple gate #20
+    const double cnC = rn * (-g * J(33, 2));
+#pragma unroll
+    for (int k = 0; k < NG; ++k) { FAC(F_RG + k) = rg[k]; FAC(F_CG + k) = cg[k]; }
     FAC(F_NCA + 0) = rn;
     FAC(F_NCA + 1) = e_nj;
-    FAC(F_NCA + 2) = -e_nj * c_jca;            // cnV
-    FAC(F_NCA + 3) = rn * (-g * J(33, 2));     // cnC
-    // Markov 9x9 LU (no pivoting: column diagonally dominant M-matrix)
-    double A[9][9];
+    FAC(F_NCA + 2) = cnV;
+    FAC(F_NCA + 3) = cnC;
+    // Markov 9x9 LU (no pivoting: column diagonally dominant M-matrix), w = M^-1 (-g J_mV)
+    double A[9][9], w[9];
 #pragma unroll
-    for (int i = 0; i < 9; ++i)
+    for (int i = 0; i < 9; ++i) {
 #pragma unroll
       for (int j = 0; j < 9; ++j) A[i][j] = (i == j ? 1.0 : 0.0) - g * J(34 + i, 34 + j);
+      w[i] = -g * J(34 + i, 0);
+    }
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
       const double pinv = 1.0 / A[k][k];
@@ -387,9 +462,6 @@ struct Machine {
         for (int j = k + 1; j < 9; ++j) A[i][j] -= A[i][k] * A[k][j];
       }
     }
-    double w[9];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) w[i] = -g * J(34 + i, 0);
 #pragma unroll
     for (int i = 0; i < 9; ++i)
 #pragma unroll
@@ -400,33 +472,33 @@ struct Machine {
       for (int j = i + 1; j < 9; ++j) w[i] -= A[i][j] * w[j];
       w[i] /= A[i][i];
     }
-#pragma unroll
-    for (int i = 0; i < 9; ++i) {
-#pragma unroll
-      for (int j = 0; j < 9; ++j) FAC(F_MLU + i * 9 + j) = A[i][j];
-      FAC(F_W + i) = w[i];
-    }
-    // core Schur complement (12x12, V last)
+    const double w4 = w[4];
+    // core Schur complement (12x12, V last), its loads before the Markov stores
     double S[12][12];
 #pragma unroll
     for (int a = 0; a < 12; ++a)
 #pragma unroll
       for (int b = 0; b < 12; ++b) S[a][b] = (a == b ? 1.0 : 0.0) - g * J(kCore[a], kCore[b]);
-    const double cnV = FAC(F_NCA + 2), cnC = FAC(F_NCA + 3);
 #pragma unroll
     for (int a = 0; a < 12; ++a) {
       const int row = kCore[a];
       double sv = 0.0;
 #pragma unroll
       for (int k = 0; k < NG; ++k)
-        if (couples(row, kSimpleGates[k])) sv += (-g * J(row, kSimpleGates[k])) * FAC(F_CG + k);
+        if (couples(row, kSimpleGates[k])) sv += (-g * J(row, kSimpleGates[k])) * cg[k];
       if (couples(row, 33)) {
         const double mn = -g * J(row, 33);
         sv += mn * cnV;
         S[a][CORE_CASS] -= mn * cnC;
       }
-      if (couples(row, 38)) sv += (-g * J(row, 38)) * w[4];
+      if (couples(row, 38)) sv += (-g * J(row, 38)) * w4;
       S[a][CORE_V] -= sv;
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+#pragma unroll
+      for (int j = 0; j < 9; ++j) FAC(F_MLU + i * 9 + j) = A[i][j];
+      FAC(F_W + i) = w[i];
     }
 #pragma unroll
     for (int k = 0; k < 12; ++k) {
@@ -451,7 +523,7 @@ struct Machine {
   // solve M x = b in place (b in registers), then CVODE's 2/(1+gamrat) scaling
   CEP_D void solve(double *b) const
   {
-    const double g = gammap;
+    const double g = m.gammap;
     double beta[NG];
 #pragma unroll
     for (int k = 0; k < NG; ++k) beta[k] = FAC(F_RG + k) * b[kSimpleGates[k]];
@@ -512,442 +584,272 @@ struct Machine {
       if (couples(r, 38)) s += J(r, 38) * b[38];
       b[r] += g * s;
     }
-    if (gamrat != 1.0) {
-      const double sc = 2.0 / (1.0 + gamrat);
+    if (m.gamrat != 1.0) {
+      const double sc = 2.0 / (1.0 + m.gamrat);
 #pragma unroll
       for (int i = 0; i < N; ++i) b[i] *= sc;
     }
   }
 
-  // ---------------- phase machine.  Actions chain inside one dispatcher loop (drive) so
-  // every phase body is instantiated once; ACT_REQ means (treq, yreq) holds the next RHS
-  // request, ACT_DONE that the lane has finished this launch.
-  enum { ACT_DONE = 0, ACT_REQ, ACT_OUTPUT_CHECK, ACT_STEP_ATTEMPT, ACT_SETUP, ACT_NEWTON_START,
-         ACT_NEWTON_ITER, ACT_ERROR_TEST, ACT_CONV_FAIL, ACT_ERR_FAIL, ACT_HIN_FINISH, ACT_REQ_F0 };
-
-  // tout reached: interpolate y(tout) and request an output RHS; else start a new step
-  CEP_D int output_check(double *yreq, double &treq)
+  // One modified-Newton iteration with f(tn, y_iterate) in FT; writes the next iterate
+  // into `y`.  Returns NLS_OK, NLS_ITER (evaluate f(y) and call again), NLS_TRY_AGAIN or
+  // NLS_CONV_FAIL (CVODE's cvNewtonIteration).
+  template <typename YV>
+  CEP_D int newton(const YV &y)
   {
-    if ((tn - tout) * h >= 0.0) {
-      const double s = (tout - tn) / h;
-#pragma unroll
-      for (int i = 0; i < N; ++i) {
-        double acc = 0.0;
-#pragma unroll
-        for (int j = QMAX; j >= 0; --j) if (j <= q) acc = ZN(j, i) + s * acc;
-        yreq[i] = acc;
-      }
-      treq = tout;
-      phase = PH_OUTPUT;
-      return ACT_REQ;
-    }
-    if (nst > 0) set_ewt_from_zn0();
-    if (++nstloc > 500) { status = ST_MAX_STEPS; phase = PH_DONE; return ACT_DONE; }
-    saved_t = tn;
-    ncf = nef = 0;
-    nflag = FIRST_CALL;
-    if ((nst > 0) && (hprime != h)) {
-      if (qprime != q) {
-        adjust_order(qprime - q);
-        q = qprime; qwait = q + 1;
-      }
-      rescale();
-    }
-    return ACT_STEP_ATTEMPT;
-  }
-
-  CEP_D int step_attempt()
-  {
-    predict();
-    set_bdf();
-    convfail = ((nflag == FIRST_CALL) || (nflag == PREV_ERR_FAIL)) ? NO_FAILURES : FAIL_OTHER;
-    callSetup = (nflag == PREV_CONV_FAIL) || (nflag == PREV_ERR_FAIL) || (nst == 0) ||
-                (nst >= nstlp + MSBP) || (fabs(gamrat - 1.0) > DGMAX);
-    return ACT_REQ_F0;
-  }
-
-  CEP_D int request_f0(double *yreq, double &treq)
-  {
-#pragma unroll
-    for (int i = 0; i < N; ++i) yreq[i] = ZN(0, i);
-    treq = tn;
-    phase = PH_STEP_F0;
-    return ACT_REQ;
-  }
-
-  CEP_D void jac_request(double *yreq, double &treq)
-  {
-    const int s = jstate(jcol);
-    double ys = 0.0, ew = 1.0;
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      const double z = ZN(0, i);
-      yreq[i] = z;
-      if (i == s) { ys = z; ew = EWT(i); }
-    }
-    const double srur = sqrt(B.uround_fd);
-    double inc = srur * fabs(ys);
-    const double alt = hg / ew;   // hg holds CVODE's minInc during the Jacobian phase
-    if (alt > inc) inc = alt;
-    inc_cur = inc;
-#pragma unroll
-    for (int i = 0; i < N; ++i) if (i == s) yreq[i] = ys + inc;
-    treq = tn;
-    phase = PH_JAC;
-  }
-
-  // consume f = f(tn, zn0) of a step attempt
-  CEP_D int on_step_f0(const double *f, double *yreq, double &treq)
-  {
-    nfe++;
-    if (!callSetup) return ACT_NEWTON_START;
-    const double dgamma = fabs((gamma / gammap) - 1.0);
-    const bool jbad = (nst == 0) || (nst > nstlj + MSBJ) ||
-                      ((convfail == FAIL_BAD_J) && (dgamma < DGMAX_J)) || (convfail == FAIL_OTHER);
-    if (!jbad) { jcur = 0; return ACT_SETUP; }
-#pragma unroll
-    for (int i = 0; i < N; ++i) F0(i) = f[i];
-    nstlj = nst;
-    jcur = 1;
-    const double fnorm = wrms_reg(f);
-    hg = (fnorm != 0.0) ? (1000.0 * fabs(h) * B.uround_fd * N * fnorm) : 1.0;
-    jcol = 0;
-    jac_request(yreq, treq);
-    return ACT_REQ;
-  }
-
-  // consume one Jacobian column; after the last, f is reloaded with f(tn, zn0)
-  CEP_D int on_jac(double *f, double *yreq, double &treq)
-  {
-    nfe++;
-    const double iinv = 1.0 / inc_cur;
-#pragma unroll
-    for (int i = 0; i < N; ++i) JD(jcol, i) = iinv * f[i] - iinv * F0(i);
-    jcol++;
-    if (jcol < NJC) { jac_request(yreq, treq); return ACT_REQ; }
-    nje++;
-#pragma unroll
-    for (int i = 0; i < N; ++i) f[i] = F0(i);
-    return ACT_SETUP;
-  }
-
-  CEP_D int setup()
-  {
-    const bool ok = factor();
-    nsetups++;
-    callSetup = 0;
-    gamrat = crate = 1.0;
-    gammap = gamma;
-    nstlp = nst;
-    return ok ? ACT_NEWTON_START : ACT_CONV_FAIL;
-  }
-
-  CEP_D int newton_iter(const double *ftemp, double *yreq, double &treq)
-  {
+    const int mm = m.mnewt;
+    const double gm = m.gamma, r1 = m.rl1;
     double b[N];
 #pragma unroll
-    for (int i = 0; i < N; ++i) b[i] = gamma * ftemp[i] - (rl1 * ZN(1, i) + ACOR(i));
+    for (int i = 0; i < N; ++i) {
+      const double a = mm > 0 ? ACOR(i) : 0.0;
+      b[i] = gm * FT(i) - (r1 * ZN(1, i) + a);
+    }
     solve(b);
-    nni++;
-    del = wrms_reg(b);
+    m.nni++;
+    double s2 = 0.0, sa = 0.0;
 #pragma unroll
-    for (int i = 0; i < N; ++i) {
-      const double a = ACOR(i) + b[i];
-      ACOR(i) = a;
-      yreq[i] = ZN(0, i) + a;
+    for (int c0 = 0; c0 < N; c0 += CH) {
+      double a[CH], z[CH], w[CH];
+#pragma unroll
+      for (int k = 0; k < CH; ++k) {
+        a[k] = mm > 0 ? ACOR(c0 + k) : 0.0;
+        z[k] = ZN(0, c0 + k);
+        w[k] = EWT(c0 + k);
+      }
+#pragma unroll
+      for (int k = 0; k < CH; ++k) {
+        const int i = c0 + k;
+        a[k] += b[i];
+        y[i] = z[k] + a[k];
+        const double t = b[i] * w[k], u = a[k] * w[k];
+        s2 += t * t;
+        sa += u * u;
+      }
+#pragma unroll
+      for (int k = 0; k < CH; ++k) ACOR(c0 + k) = a[k];
     }
-    if (mnewt > 0) { const double r = del / delp; crate = (CRDOWN * crate > r) ? CRDOWN * crate : r; }
-    const double dcon = del * (crate < 1.0 ? crate : 1.0) / tq4;
+    const double del = sqrt(s2 / N);
+    if (mm > 0) { const double r = del / m.delp; m.crate = (CRDOWN * m.crate > r) ? CRDOWN * m.crate : r; }
+    const double dcon = del * (m.crate < 1.0 ? m.crate : 1.0) / m.tq4;
     if (dcon <= 1.0) {
-      if (mnewt == 0) {
-        acnrm = del;
-      } else {
-        double s = 0.0;
+      m.acnrm = (mm == 0) ? del : sqrt(sa / N);
+      m.jcur = 0;
+      return NLS_OK;
+    }
+    m.mnewt = mm + 1;
+    if ((mm + 1 == MAXCOR) || ((mm + 1 >= 2) && (del > RDIV * m.delp)))
+      return m.jcur ? NLS_CONV_FAIL : NLS_TRY_AGAIN;
+    m.delp = del;
+    return NLS_ITER;
+  }
+
+  // A successful step (CVODE's cvCompleteStep + cvPrepareNextStep), with one sweep over
+  // the components for: zn[j] += l[j] acor; the saved correction zn[QMAX] = acor when
+  // qwait reaches 1; the norms of the order-selection test (zn[q] and acor - cquot
+  // zn[QMAX], weighted with this step's ewt); the error weights of the next step.
+  CEP_D void complete_step(double dsm)
+  {
+    m.nst++;
+    const int q = m.q;
 #pragma unroll
-        for (int i = 0; i < N; ++i) { const double t = ACOR(i) * EWT(i); s += t * t; }
-        acnrm = sqrt(s * (1.0 / N));
+    for (int i = QMAX; i >= 2; --i) if (i <= q) m.tau[i] = m.tau[i - 1];
+    if ((q == 1) && (m.nst > 1)) m.tau[2] = m.tau[1];
+    m.tau[1] = m.h;
+    const bool save = (m.qwait == 2) && (q != B.maxord);   // qwait becomes 1
+    m.qwait--;
+    // prepare_next needs the norms only when it may change the order
+    const bool norms = (m.etamax != 1.0) && (m.qwait == 0);
+    const bool dn = norms && q > 1;
+    const bool up = norms && (q != B.maxord) && (m.saved_tq5 != 0.0);
+    const double cquot = up ? (m.tq5 / m.saved_tq5) * pow(m.h / m.tau[2], (double)(q + 1)) : 0.0;
+    double l[QMAX + 1];
+#pragma unroll
+    for (int j = 0; j <= QMAX; ++j) l[j] = m.l[j];
+    const double rt = B.rtol, at_ = B.atol;
+    double s2 = 0.0, s3 = 0.0;
+#pragma unroll
+    for (int c0 = 0; c0 < N; c0 += CH) {
+      double z[CH][QMAX + 1], a[CH], w[CH];
+#pragma unroll
+      for (int k = 0; k < CH; ++k) {
+#pragma unroll
+        for (int j = 0; j <= QMAX; ++j) z[k][j] = (j <= q || (up && j == QMAX)) ? ZN(j, c0 + k) : 0.0;
+        a[k] = ACOR(c0 + k);
+        w[k] = EWT(c0 + k);
       }
-      jcur = 0;
-      return ACT_ERROR_TEST;
-    }
-    mnewt++;
-    if ((mnewt == MAXCOR) || ((mnewt >= 2) && (del > RDIV * delp))) {
-      if (!jcur) {   // retry the nonlinear solve with a fresh Jacobian
-        callSetup = 1;
-        convfail = FAIL_BAD_J;
-        return ACT_REQ_F0;
+#pragma unroll
+      for (int k = 0; k < CH; ++k) {
+        const double zqm = z[k][QMAX];
+#pragma unroll
+        for (int j = 0; j <= QMAX; ++j) if (j <= q) z[k][j] += l[j] * a[k];
+        if (dn) {
+          double zq = 0.0;
+#pragma unroll
+          for (int j = 2; j <= QMAX; ++j) if (j == q) zq = z[k][j];
+          const double t = zq * w[k];
+          s2 += t * t;
+        }
+        if (up) { const double t = (a[k] - cquot * zqm) * w[k]; s3 += t * t; }
+        w[k] = 1.0 / (rt * fabs(z[k][0]) + at_);
       }
-      return ACT_CONV_FAIL;
-    }
-    delp = del;
-    treq = tn;
-    phase = PH_NEWTON;
-    return ACT_REQ;
-  }
-
-  CEP_D int conv_fail()
-  {
-    ncfn++;
-    restore();
-    ncf++;
-    etamax = 1.0;
-    if (ncf == MAXNCF) { status = ST_CONV_FAIL; phase = PH_DONE; return ACT_DONE; }
-    eta = ETACF;
-    nflag = PREV_CONV_FAIL;
-    rescale();
-    return ACT_STEP_ATTEMPT;
-  }
-
-  CEP_D int error_test()
-  {
-    const double dsm = acnrm * tq2;
-    if (dsm <= 1.0) {
-      complete_step();
-      prepare_next(dsm);
-      etamax = (nst <= SMALL_NST) ? ETAMX2 : ETAMX3;
-      return ACT_OUTPUT_CHECK;
-    }
-    nef++;
-    netf++;
-    nflag = PREV_ERR_FAIL;
-    if (nef == MAXNEF || !(dsm == dsm)) { restore(); status = ST_ERR_FAIL; phase = PH_DONE; return ACT_DONE; }
-    etamax = 1.0;
-    restore();
-    if (nef <= MXNEF1) {
-      double e = 1.0 / (pow(BIAS2 * dsm, 1.0 / (q + 1)) + ADDON);
-      if (e < ETAMIN) e = ETAMIN;
-      if (nef >= SMALL_NEF && e > ETAMXF) e = ETAMXF;
-      eta = e;
-      rescale();
-      return ACT_STEP_ATTEMPT;
-    }
-    if (q > 1) {
-      eta = ETAMIN;
-      adjust_order(-1);
-      q--;
-      qwait = q + 1;
-      rescale();
-      return ACT_STEP_ATTEMPT;
-    }
-    // order 1: restart from f(tn, zn0) with a smaller step
-    eta = ETAMIN;
-    h *= eta;
-    hscale = h;
-    qwait = LONG_WAIT;
-    return ACT_ERR_FAIL;
-  }
-
-  CEP_D int drive(double *f, double *yreq, double &treq, int act)
-  {
-    while (act > ACT_REQ) {
-      switch (act) {
-        case ACT_OUTPUT_CHECK: act = output_check(yreq, treq); break;
-        case ACT_STEP_ATTEMPT: act = step_attempt(); break;
-        case ACT_REQ_F0: act = request_f0(yreq, treq); break;
-        case ACT_SETUP: act = setup(); break;
-        case ACT_NEWTON_START:
 #pragma unroll
-          for (int i = 0; i < N; ++i) ACOR(i) = 0.0;
-          mnewt = 0;
-          del = delp = 0.0;
-          act = ACT_NEWTON_ITER;
-          break;
-        case ACT_NEWTON_ITER: act = newton_iter(f, yreq, treq); break;
-        case ACT_ERROR_TEST: act = error_test(); break;
-        case ACT_CONV_FAIL: act = conv_fail(); break;
-        case ACT_ERR_FAIL:
+      for (int k = 0; k < CH; ++k) {
 #pragma unroll
-          for (int i = 0; i < N; ++i) yreq[i] = ZN(0, i);
-          treq = tn;
-          phase = PH_RESTART_F;
-          act = ACT_REQ;
-          break;
-        case ACT_HIN_FINISH: act = hin_finish(); break;
-        default: act = ACT_DONE; break;
+        for (int j = 0; j <= QMAX; ++j) if (j <= q) ZN(j, c0 + k) = z[k][j];
+        if (save) ZN(QMAX, c0 + k) = a[k];
+        EWT(c0 + k) = w[k];
       }
     }
-    return act;
-  }
-
-  // consume the RHS just evaluated for the pending phase and run to the next request
-  CEP_D bool consume(double *f, double *yreq, double &treq)
-  {
-    int act = ACT_DONE;
-    switch (phase) {
-      case PH_INIT_F:
-#pragma unroll
-        for (int i = 0; i < N; ++i) ZN(1, i) = f[i];
-        nfe = 1;
-        set_ewt_from_zn0();
-        act = hin_begin(yreq, treq);
-        break;
-      case PH_HIN: act = on_hin(f, yreq, treq); break;
-      case PH_STEP_F0: act = on_step_f0(f, yreq, treq); break;
-      case PH_JAC: act = on_jac(f, yreq, treq); break;
-      case PH_NEWTON: nfe++; act = ACT_NEWTON_ITER; break;
-      case PH_RESTART_F:
-        nfe++;
-#pragma unroll
-        for (int i = 0; i < N; ++i) ZN(1, i) = h * f[i];
-        act = ACT_STEP_ATTEMPT;
-        break;
-      case PH_READY: act = ACT_OUTPUT_CHECK; break;
-      default: act = ACT_DONE; break;
-    }
-    return drive(f, yreq, treq, act) == ACT_REQ;
-  }
-
-  CEP_D void complete_step()
-  {
-    nst++;
-#pragma unroll
-    for (int i = QMAX; i >= 2; --i) if (i <= q) tau[i] = tau[i - 1];
-    if ((q == 1) && (nst > 1)) tau[2] = tau[1];
-    tau[1] = h;
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      const double a = ACOR(i);
-#pragma unroll
-      for (int j = 0; j <= QMAX; ++j) if (j <= q) ZN(j, i) += l[j] * a;
-    }
-    qwait--;
-    if ((qwait == 1) && (q != B.maxord)) {
-#pragma unroll
-      for (int i = 0; i < N; ++i) ZN(QMAX, i) = ACOR(i);
-      saved_tq5 = tq5;
-    }
+    if (save) m.saved_tq5 = m.tq5;
+    prepare_next(dsm, s2, s3, dn, up);
+    m.etamax = (m.nst <= SMALL_NST) ? ETAMX2 : ETAMX3;
   }
 
   CEP_D void set_eta()
   {
-    if (eta < THRESH) {
-      eta = 1.0;
-      hprime = h;
+    if (m.eta < THRESH) {
+      m.eta = 1.0;
+      m.hprime = m.h;
     } else {
-      if (eta > etamax) eta = etamax;
-      const double r = fabs(h) * (1.0 / B.dt) * eta;
-      eta /= (r > 1.0 ? r : 1.0);
-      hprime = h * eta;
+      if (m.eta > m.etamax) m.eta = m.etamax;
+      const double r = fabs(m.h) * (1.0 / B.dt) * m.eta;
+      m.eta /= (r > 1.0 ? r : 1.0);
+      m.hprime = m.h * m.eta;
     }
   }
 
-  CEP_D void prepare_next(double dsm)
+  CEP_D void prepare_next(double dsm, double s2, double s3, bool dn, bool up)
   {
-    if (etamax == 1.0) {
-      if (qwait < 2) qwait = 2;
-      qprime = q;
-      hprime = h;
-      eta = 1.0;
+    if (m.etamax == 1.0) {
+      if (m.qwait < 2) m.qwait = 2;
+      m.qprime = m.q;
+      m.hprime = m.h;
+      m.eta = 1.0;
       return;
     }
+    const int q = m.q;
     const double etaq = 1.0 / (pow(BIAS2 * dsm, 1.0 / (q + 1)) + ADDON);
-    if (qwait != 0) {
-      eta = etaq;
-      qprime = q;
+    if (m.qwait != 0) {
+      m.eta = etaq;
+      m.qprime = q;
       set_eta();
       return;
     }
-    qwait = 2;
+    m.qwait = 2;
     double etaqm1 = 0.0, etaqp1 = 0.0;
-    if (q > 1) {
-      double zq[N];
-#pragma unroll
-      for (int i = 0; i < N; ++i) {
-        double v = 0.0;
-#pragma unroll
-        for (int j = 0; j <= QMAX; ++j) if (j == q) v = ZN(j, i);
-        zq[i] = v;
-      }
-      const double ddn = wrms_reg(zq) * tq1;
+    if (dn) {
+      const double ddn = sqrt(s2 / N) * m.tq1;
       etaqm1 = 1.0 / (pow(BIAS1 * ddn, 1.0 / q) + ADDON);
     }
-    if (q != B.maxord && saved_tq5 != 0.0) {
-      const double cquot = (tq5 / saved_tq5) * pow(h / tau[2], (double)(q + 1));
-      double tv[N];
-#pragma unroll
-      for (int i = 0; i < N; ++i) tv[i] = ACOR(i) - cquot * ZN(QMAX, i);
-      const double dup = wrms_reg(tv) * tq3;
+    if (up) {
+      const double dup = sqrt(s3 / N) * m.tq3;
       etaqp1 = 1.0 / (pow(BIAS3 * dup, 1.0 / (q + 2)) + ADDON);
     }
     double etam = etaq;
     if (etaqm1 > etam) etam = etaqm1;
     if (etaqp1 > etam) etam = etaqp1;
     if (etam < THRESH) {
-      eta = 1.0; qprime = q;
+      m.eta = 1.0; m.qprime = q;
     } else if (etam == etaq) {
-      eta = etaq; qprime = q;
+      m.eta = etaq; m.qprime = q;
     } else if (etam == etaqm1) {
-      eta = etaqm1; qprime = q - 1;
+      m.eta = etaqm1; m.qprime = q - 1;
     } else {
-      eta = etaqp1; qprime = q + 1;
+      m.eta = etaqp1; m.qprime = q + 1;
+      double a[N];
 #pragma unroll
-      for (int i = 0; i < N; ++i) ZN(QMAX, i) = ACOR(i);
+      for (int i = 0; i < N; ++i) a[i] = ACOR(i);
+#pragma unroll
+      for (int i = 0; i < N; ++i) ZN(QMAX, i) = a[i];
     }
     set_eta();
   }
 
-  // ---- initial step size (CVODE's cvHin), one RHS per trial step
-  CEP_D int hin_begin(double *yreq, double &treq)
+  // y(t) from the Nordsieck array (CVodeGetDky with k = 0)
+  template <typename YV>
+  CEP_D void interpolate(double t, const YV &y) const
   {
-    const double tdist = fabs(B.dt);     // first output point is t0 + dt
-    const double tround = UROUND * fabs(tout);
-    hlb = HLB_FACTOR * tround;
-    double hub_inv = 0.0;
+    const double s = (t - m.tn) / m.h;
+    const int q = m.q;
+#pragma unroll 4
+    for (int i = 0; i < N; ++i) {
+      double z[QMAX + 1];
 #pragma unroll
+      for (int j = 0; j <= QMAX; ++j) z[j] = (j <= q) ? ZN(j, i) : 0.0;
+      double v = 0.0;
+#pragma unroll
+      for (int j = QMAX; j >= 0; --j) {
+        if (j == q) v = z[j];
+        else if (j < q) v = z[j] + s * v;
+      }
+      y[i] = v;
+    }
+  }
+
+  // ---- initial step size (CVODE's cvHin): bounds, then up to MAX_ITERS trial RHS
+  // evaluations at tn + hg.  Returns true when h is known without a trial.
+  CEP_D bool hin_begin()
+  {
+    const double tdist = fabs(m.tout - m.tn);
+    const double tround = UROUND * (fabs(m.tn) > fabs(m.tout) ? fabs(m.tn) : fabs(m.tout));
+    m.hlb = HLB_FACTOR * tround;
+    double hub_inv = 0.0;
+#pragma unroll 4
     for (int i = 0; i < N; ++i) {
       const double tol = HUB_FACTOR * fabs(ZN(0, i)) + B.atol;
       const double r = fabs(ZN(1, i)) / tol;
       if (r > hub_inv) hub_inv = r;
     }
-    hub = HUB_FACTOR * tdist;
+    double hub = HUB_FACTOR * tdist;
     if (hub * hub_inv > 1.0) hub = 1.0 / hub_inv;
-    hg = sqrt(hlb * hub);
-    if (hub < hlb) { hnew = hg; hin_ok = 2; return ACT_HIN_FINISH; }
-    hin_count = 1;
-    hin_ok = 0;
-    hs = hg;
-    return hin_request(yreq, treq);
+    m.hub = hub;
+    m.hg = sqrt(m.hlb * hub);
+    m.hin_count = 1;
+    m.hin_ok = 0;
+    if (hub < m.hlb) { hin_set(m.hg); return true; }
+    return false;
   }
-  CEP_D int hin_request(double *yreq, double &treq)
+  // consume f(tn + hg, zn0 + hg zn1); returns true when the estimate is final
+  template <typename FV>
+  CEP_D bool hin_step(const FV &f)
   {
+    const double hg = m.hg;
+    double s = 0.0;
 #pragma unroll
-    for (int i = 0; i < N; ++i) yreq[i] = hg * ZN(1, i) + ZN(0, i);
-    treq = tn + hg;
-    phase = PH_HIN;
-    return ACT_REQ;
-  }
-  CEP_D int on_hin(const double *f, double *yreq, double &treq)
-  {
-    nfe++;
-    double d[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) d[i] = (f[i] - ZN(1, i)) * (1.0 / hg);
-    const double yddnrm = wrms_reg(d);
-    hs = hg;
-    if (hin_ok || hin_count == 4) { hnew = hg; hin_ok = 1; return ACT_HIN_FINISH; }
-    hnew = (yddnrm * hub * hub > 2.0) ? sqrt(2.0 / yddnrm) : sqrt(hg * hub);
-    const double hrat = hnew / hg;
-    if ((hrat > 0.5) && (hrat < 2.0)) hin_ok = 1;
-    if ((hin_count > 1) && (hrat > 2.0)) { hnew = hg; hin_ok = 1; }
-    hg = hnew;
-    hin_count++;
-    return hin_request(yreq, treq);
-  }
-  // hin_ok == 2: hub < hlb, take hg directly
-  CEP_D int hin_finish()
-  {
-    const bool direct = hin_ok == 2;
-    double h0 = direct ? hnew : H_BIAS * hnew;
-    if (!direct) {
-      if (h0 < hlb) h0 = hlb;
-      if (h0 > hub) h0 = hub;
+    for (int i = 0; i < N; ++i) {
+      const double t = (f[i] - ZN(1, i)) * (1.0 / hg) * EWT(i);
+      s += t * t;
     }
-    const double rh = fabs(h0) / B.dt;   // hmax = dt
-    if (rh > 1.0) h0 /= rh;
-    h = h0; hscale = h0; hprime = h0;
+    const double yddnrm = sqrt(s / N);
+    double hnew;
+    if (m.hin_ok || m.hin_count == MAX_ITERS) {
+      hnew = hg;
+    } else {
+      hnew = (yddnrm * m.hub * m.hub > 2.0) ? sqrt(2.0 / yddnrm) : sqrt(hg * m.hub);
+      const double hrat = hnew / hg;
+      if ((hrat > 0.5) && (hrat < 2.0)) m.hin_ok = 1;
+      if ((m.hin_count > 1) && (hrat > 2.0)) { hnew = hg; m.hin_ok = 1; }
+      m.hg = hnew;
+      m.hin_count++;
+      return false;
+    }
+    double h0 = H_BIAS * hnew;
+    if (h0 < m.hlb) h0 = m.hlb;
+    if (h0 > m.hub) h0 = m.hub;
+    hin_set(h0);
+    return true;
+  }
+  CEP_D void hin_set(double h)
+  {
+    const double rh = fabs(h) / B.dt;    // max step = output dt
+    if (rh > 1.0) h /= rh;
+    m.h = h; m.hscale = h; m.hprime = h;
+    double z[N];
 #pragma unroll
-    for (int i = 0; i < N; ++i) ZN(1, i) *= h0;
-    return ACT_OUTPUT_CHECK;
+    for (int i = 0; i < N; ++i) z[i] = ZN(1, i) * h;
+#pragma unroll
+    for (int i = 0; i < N; ++i) ZN(1, i) = z[i];
   }
 };
 

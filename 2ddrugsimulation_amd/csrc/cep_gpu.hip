@@ -17,14 +17,13 @@ __constant__ MarkovTable g_markov;
 }
 
 using ordbdf::Buffers;
-using ordbdf::Machine;
 
 // ============================================================================ kernels
 
 struct PrmAcc {
   const double *p;
   int pitch;
-  CEP_D double operator()(int k) const { return p[(size_t)k * pitch]; }
+  CEP_D double operator()(int k) const { return p[k * pitch]; }
 };
 
 // computeRates for a batch: AoS states in, AoS rates / algebraic slots out
@@ -93,76 +92,416 @@ CEP_D void bm_config(Biomarker &bm, double bcl, double stim_start, const RunCfg 
   bm.cap = cap;
 }
 
-// ---- ORd-CiPA: persistent BDF + biomarkers, one cell per lane
-template <typename R>
-__global__ void __launch_bounds__(64) ord_bdf_kernel(Buffers B, RunCfg rc, int nbeats)
+// ---- ORd-CiPA: persistent CVODE-style BDF + biomarkers, CPW cells per wavefront.
+//
+// Lane roles inside the 64-lane workgroup (one wavefront):
+//   lanes [0, CPW)        integrator of cell slot s = lane (BDF step logic, Newton)
+//   lanes [CPW, 2*CPW)    output lane of slot s = lane - CPW: evaluates the right-hand side
+//                         at each output point (CVode's y(tout) + computeRates) and runs
+//                         the biomarker state machine
+//   every lane            takes finite-difference Jacobian columns from a wave-level task
+//                         queue whenever it has nothing else to evaluate in a pass
+// Every pass runs the bookkeeping blocks (each masked to the lanes in that phase), then
+// exactly one right-hand-side evaluation per busy lane, then consumes the results.  A
+// cell's arithmetic never depends on its neighbours: results are the same alone and in
+// any batch.
+enum { RQ_NONE = 0, RQ_STEP, RQ_OUT, RQ_JAC };
+
+CEP_D int lanes_below(unsigned long long mask)
+{
+  return __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0));
+}
+
+template <typename R, int CPW>
+__global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunCfg rc, const int nbeats,
+                                                     const long long max_passes)
 {
   using namespace ordbdf;
-  const int c = blockIdx.x * 64 + threadIdx.x;
-  if (c >= B.ncell) return;
-  Machine<R> m(B, c);
-  m.load();
-  m.nstloc = 0;
-  if (m.status != ST_OK) return;
-  Biomarker bm;
-  const double bcl = m.P(ord::OP_BCL);
-  bm_config(bm, bcl, m.P(ord::OP_STIM_START), rc, B.cap);
-  double *ct = B.ct + c, *cv = B.cv + c;
-  double y[N], f[N], treq = 0.0;
-  bool need_rhs;
-  if (m.phase == PH_FRESH) {
-    // CVodeInit at t0 = 0 with the initial states
-#pragma unroll
-    for (int i = 0; i < N; ++i) { y[i] = m.Y(i); m.ZN(0, i) = y[i]; }
-    m.tn = 0.0; m.tout = rc.dt; m.q = 1; m.qprime = 1; m.qwait = 2; m.etamax = ETAMX1;
-    m.h = 0.0; m.hprime = 0.0; m.eta = 1.0; m.nst = m.nstlp = m.nstlj = 0; m.crate = 1.0;
-    m.gamrat = 1.0; m.gammap = 0.0; m.saved_tq5 = 0.0; m.jcur = 0; m.iout = 0;
-    m.nfe = m.nje = m.nsetups = m.nni = m.ncfn = m.netf = 0;
-    for (int i = 0; i < 7; ++i) m.tau[i] = 0.0;
-    m.tq1 = m.tq2 = m.tq3 = m.tq4 = m.tq5 = 0.0;
-    bm.begin(y[0], y[9]);
-    treq = 0.0;
-    m.phase = PH_INIT_F;
-    need_rhs = true;
-  } else {
-    bm.load(B.bmd, B.bmi, B.pitch, c);
-    m.phase = PH_READY;
-    need_rhs = false;
+  static_assert(2 * CPW <= 64, "an integrator and an output lane per cell");
+  static_assert(CPW == CPB, "one block of integrator vectors per workgroup");
+  constexpr int QCAP = CPW * NJC;                 // each cell has at most one Jacobian pending
+  __shared__ Mach s_m[CPW];
+  __shared__ Biomarker s_bm[CPW];
+  // each lane's right-hand-side argument, then its result: [component][lane].  The RHS
+  // reads the argument into registers before it writes the result.
+  __shared__ double s_v[N][64];
+  __shared__ double s_tout[CPW];
+  __shared__ int s_oiout[CPW], s_opend[CPW], s_ofail[CPW];
+  __shared__ unsigned short s_jq[QCAP];           // Jacobian tasks: (slot << 6) | column slot
+
+  const int lane = threadIdx.x;
+  const int cell0 = blockIdx.x * CPW;
+  const bool is_int = lane < CPW && cell0 + lane < B.ncell;
+  const bool is_out = lane >= CPW && lane < 2 * CPW && cell0 + lane - CPW < B.ncell;
+  const int slot = lane < CPW ? lane : (lane < 2 * CPW ? lane - CPW : 0);
+  const int c = cell0 + slot;
+  const LVec LV{&s_v[0][lane]};                    // this lane's staging vector
+  const double srur = sqrt(B.uround_fd);
+  const size_t blk0 = (size_t)blockIdx.x * CPB;     // first cell of this workgroup's block
+  const BlockBufs V0{make_rsrc(B.zn + blk0 * ((QMAX + 1) * N), CPB * (QMAX + 1) * N * 8),
+                     make_rsrc(B.ewt + blk0 * N, CPB * N * 8), make_rsrc(B.acor + blk0 * N, CPB * N * 8),
+                     make_rsrc(B.ft + blk0 * N, CPB * N * 8), make_rsrc(B.jd + blk0 * (NJC * N), CPB * NJC * N * 8),
+                     make_rsrc(B.fac + blk0 * NFAC, CPB * NFAC * 8)};
+
+  // ---- launch prologue
+  int trace_start = 0;
+  if (is_int) {
+    Lane<R> L{B, s_m[slot], V0, (unsigned)slot, (unsigned)c, B.pitch};
+    Mach &m = s_m[slot];
+    L.load();
+    int ph = m.phase;
+    if (m.status != ST_OK) {
+      ph = PH_DONE;
+    } else if (ph == PH_FRESH) {
+      // CVodeInit at t0 = 0 with the current states
+      for (int i = 0; i < N; ++i) L.ZN(0, i) = L.Y(i);
+      m.tn = 0.0; m.tout = rc.dt; m.q = 1; m.qprime = 1; m.qwait = 2; m.etamax = ETAMX1;
+      m.h = 0.0; m.hprime = 0.0; m.hscale = 0.0; m.eta = 1.0; m.nst = m.nstlp = m.nstlj = 0;
+      m.crate = 1.0; m.gamrat = 1.0; m.gammap = 0.0; m.saved_tq5 = 0.0; m.jcur = 0; m.iout = 0;
+      m.nfe = m.nje = m.nsetups = m.nni = m.ncfn = m.netf = 0;
+      for (int i = 0; i < 7; ++i) m.tau[i] = 0.0;
+      m.tq1 = m.tq2 = m.tq3 = m.tq4 = m.tq5 = 0.0;
+      m.ncf = m.nef = 0; m.nflag = FIRST_CALL; m.callSetup = 0; m.convfail = NO_FAILURES;
+      ph = PH_INIT_F;
+    } else {
+      ph = PH_STEP_DONE;
+    }
+    m.phase = ph;
+    m.nstloc = 0;
+    m.jend = 0;
+    m.sw_ops = 0;
+    m.sw_dq = 0;
+    m.sw_q1 = m.q;
+    const int beat_steps = (int)(B.prm[ord::OP_BCL * B.pitch + c] / rc.dt);
+    m.iout_end = m.iout + nbeats * beat_steps;
   }
-  m.iout_end = m.iout + nbeats * bm.beat_steps;
-  const int trace_start = m.iout_end - bm.beat_steps;
-  const PrmAcc P{B.prm + c, B.pitch};
-  for (;;) {
-    if (need_rhs) {
-      ord::Currents cur;
-      ord::rhs<R>(treq, y, f, P, &cur);
-      bool fin = true;
-#pragma unroll
-      for (int i = 0; i < N; ++i) fin = fin && isfinite(f[i]);
-      if (!fin && m.phase != PH_JAC) { m.status = ST_NONFINITE; m.phase = PH_DONE; break; }
-      if (m.phase == PH_OUTPUT) {
-        // CVode returned y(tout); computeRates at tout feeds the biomarkers
-        m.iout++;
-        bm.sample(m.iout, m.tout, y[0], y[9], f[0],
-                  cur.INaL + cur.ICaL + cur.Ito + cur.IKr + cur.IKs + cur.IK1, cur.INaL, cur.ICaL,
-                  ct, cv, B.pitch);
-        if (B.vtrace && m.iout > trace_start && m.iout - trace_start <= B.tcap)
-          B.vtrace[(size_t)(m.iout - trace_start - 1) * B.pitch + c] = y[0];
-        m.tout += rc.dt;
-        m.nstloc = 0;
-        m.phase = PH_READY;
-        if (m.iout >= m.iout_end) {
-#pragma unroll
-          for (int i = 0; i < N; ++i) m.Y(i) = y[i];
-          break;
+  if (is_out) {
+    Biomarker &bm = s_bm[slot];
+    bm_config(bm, B.prm[ord::OP_BCL * B.pitch + c], B.prm[ord::OP_STIM_START * B.pitch + c], rc, B.cap);
+    const int iout0 = B.si[11 * B.pitch + c];
+    if (B.si[4 * B.pitch + c] == PH_FRESH) bm.begin(B.y[c], B.y[9 * B.pitch + c]);
+    else bm.load(B.bmd, B.bmi, B.pitch, c);
+    trace_start = iout0 + (nbeats - 1) * bm.beat_steps;
+    s_opend[slot] = 0;
+    s_ofail[slot] = 0;
+  }
+  __syncthreads();
+
+  int jq_head = 0, jq_tail = 0;                   // wave-uniform task queue cursors
+  for (long long pass = 0;; ++pass) {
+    int pp = B.pitch;
+    const BlockBufs &V = V0;
+    asm volatile("" : "+s"(pp));
+    int req = RQ_NONE;
+    double treq = 0.0;
+    // ================= integrator bookkeeping (each block masked to its phase)
+    if (is_int) {
+      Lane<R> L{B, s_m[slot], V, (unsigned)slot, (unsigned)c, pp};
+      Mach &m = s_m[slot];
+      int ph = m.phase;
+      if (s_ofail[slot] && ph != PH_DONE) { m.status = ST_NONFINITE; ph = PH_DONE; }
+      if (pass >= max_passes && ph != PH_DONE) { m.status = ST_MAX_STEPS; ph = PH_DONE; }
+      if (ph == PH_WAIT_JAC && jq_head >= m.jend) {
+        m.nje++;
+        ph = PH_FACTOR;
+      }
+      if (ph == PH_FACTOR) {
+        const bool ok = L.factor();
+        m.nsetups++;
+        m.callSetup = 0;
+        m.gamrat = m.crate = 1.0;
+        m.gammap = m.gamma;
+        m.nstlp = m.nst;
+        ph = ok ? PH_NEWTON : PH_CONVFAIL;
+      }
+      if (ph == PH_NEWTON) {
+        const int r = L.newton(LV);
+        if (r == NLS_OK) ph = PH_ETEST;
+        else if (r == NLS_ITER) ph = PH_ITER;
+        else if (r == NLS_TRY_AGAIN) { m.callSetup = 1; m.convfail = FAIL_BAD_J; ph = PH_F0; }
+        else ph = PH_CONVFAIL;
+      }
+      if (ph == PH_CONVFAIL) {
+        m.ncfn++;
+        m.tn = m.saved_t;                     // restore; the array part runs in PREDICT
+        m.ncf++;
+        m.etamax = 1.0;
+        if (m.ncf == MAXNCF) {
+          m.status = ST_CONV_FAIL; ph = PH_DONE;
+        } else {
+          m.eta = ETACF;
+          m.nflag = PREV_CONV_FAIL;
+          m.sw_ops = Lane<R>::SW_RESTORE | Lane<R>::SW_RESCALE;
+          m.sw_dq = 0; m.sw_q1 = m.q;
+          ph = PH_PREDICT;
         }
       }
+      if (ph == PH_ETEST) {
+        const double dsm = m.acnrm * m.tq2;
+        if (dsm <= 1.0) {
+          L.complete_step(dsm);
+          ph = PH_STEP_DONE;
+        } else {
+          m.nef++;
+          m.netf++;
+          m.nflag = PREV_ERR_FAIL;
+          m.tn = m.saved_t;                   // restore; the array part runs in PREDICT
+          m.sw_ops = Lane<R>::SW_RESTORE;
+          m.sw_dq = 0; m.sw_q1 = m.q;
+          if (m.nef == MAXNEF) {
+            m.status = ST_ERR_FAIL; ph = PH_DONE;
+          } else {
+            m.etamax = 1.0;
+            if (m.nef <= MXNEF1) {
+              double eta = 1.0 / (pow(BIAS2 * dsm, 1.0 / (m.q + 1)) + ADDON);
+              if (eta < ETAMIN) eta = ETAMIN;
+              if (m.nef >= SMALL_NEF && eta > ETAMXF) eta = ETAMXF;
+              m.eta = eta;
+              m.sw_ops |= Lane<R>::SW_RESCALE;
+              ph = PH_PREDICT;
+            } else if (m.q > 1) {
+              m.eta = ETAMIN;
+              m.sw_dq = -1;                    // cvAdjustOrder(-1), then q = q - 1
+              m.sw_q1 = m.q - 1;
+              m.qwait = m.q;
+              m.sw_ops |= Lane<R>::SW_RESCALE;
+              ph = PH_PREDICT;
+            } else {
+              m.eta = ETAMIN;
+              m.h *= m.eta;
+              m.hscale = m.h;
+              m.qwait = LONG_WAIT;
+              m.sw_ops |= Lane<R>::SW_RESTART;   // restore only, then f(tn, zn[0]) -> zn[1]
+              ph = PH_PREDICT;
+            }
+          }
+        }
+      }
+      if (ph == PH_STEP_DONE) {
+        if ((m.tn - m.tout) * m.h >= 0.0) {
+          if (!s_opend[slot]) {              // hand y(tout) to the output lane
+            const LVec yo{&s_v[0][CPW + slot]};   // the output lane's staging vector
+            L.interpolate(m.tout, yo);
+            s_tout[slot] = m.tout;
+            m.iout++;
+            if (m.iout >= m.iout_end)              // CellML STATES at the end of the run
+              for (int i = 0; i < N; ++i) L.Y(i) = yo[i];
+            s_oiout[slot] = m.iout;
+            s_opend[slot] = 1;
+            m.nstloc = 0;
+            m.tout += rc.dt;
+            if (m.iout >= m.iout_end) ph = PH_DONE;
+            else if ((m.tn - m.tout) * m.h < 0.0) ph = PH_START;
+          }
+        } else {
+          ph = PH_START;
+        }
+      }
+      if (ph == PH_START) {
+        if (m.nstloc >= B.max_steps) {
+          m.status = ST_MAX_STEPS; ph = PH_DONE;
+        } else {
+          // (the error weights for this step were set when the last step completed)
+          m.nstloc++;
+          m.sw_ops = 0; m.sw_dq = 0; m.sw_q1 = m.q;
+          if (m.nst > 0 && m.hprime != m.h) {
+            if (m.qprime != m.q) {
+              m.sw_dq = m.qprime - m.q;
+              m.sw_q1 = m.qprime;
+              m.qwait = m.qprime + 1;
+            }
+            m.sw_ops = Lane<R>::SW_RESCALE;
+          }
+          m.saved_t = m.tn;
+          m.ncf = m.nef = 0;
+          m.nflag = FIRST_CALL;
+          ph = PH_PREDICT;
+        }
+      }
+      if (ph == PH_PREDICT) {
+        // one sweep: [restore] [order change] [rescale] predict, y = zn[0]
+        const int q0 = m.q, q1 = m.sw_q1;
+        const bool restart = (m.sw_ops & Lane<R>::SW_RESTART) != 0;
+        double ll[QMAX + 1], a1 = 0.0;
+#pragma unroll
+        for (int j = 0; j <= QMAX; ++j) ll[j] = 0.0;
+        int ops = m.sw_ops;
+        if (m.sw_dq != 0) ops |= L.order_ops(m.sw_dq, ll, a1);   // at the old order and hscale
+        m.q = q1;
+        if (ops & Lane<R>::SW_RESCALE) { m.h = m.hscale * m.eta; m.hscale = m.h; }
+        if (!restart) { ops |= Lane<R>::SW_PREDICT; m.tn += m.h; }
+        L.attempt_sweep(ops, q0, q1, ll, a1, m.eta, LV);
+        m.sw_ops = 0; m.sw_dq = 0; m.sw_q1 = m.q;
+        if (restart) {
+          ph = PH_RESTART_F;
+        } else {
+          L.set_bdf();
+          m.callSetup = (m.nflag == PREV_CONV_FAIL) || (m.nflag == PREV_ERR_FAIL) || (m.nst == 0) ||
+                        (m.nst >= m.nstlp + MSBP) || (fabs(m.gamrat - 1.0) > DGMAX);
+          m.convfail = ((m.nflag == FIRST_CALL) || (m.nflag == PREV_ERR_FAIL)) ? NO_FAILURES : FAIL_OTHER;
+          ph = PH_F0;
+        }
+        treq = m.tn;
+        req = RQ_STEP;                         // y = zn[0] from the sweep
+      } else if (ph == PH_F0 || ph == PH_INIT_F) {
+        double z[N];
+        for (int i = 0; i < N; ++i) z[i] = L.ZN(0, i);
+        for (int i = 0; i < N; ++i) LV[i] = z[i];
+        treq = m.tn;
+        req = RQ_STEP;
+      } else if (ph == PH_ITER) {
+        treq = m.tn;                         // y was set by the Newton block
+        req = RQ_STEP;
+      } else if (ph == PH_HIN) {
+        double z[N];
+        for (int i = 0; i < N; ++i) z[i] = m.hg * L.ZN(1, i) + L.ZN(0, i);
+        for (int i = 0; i < N; ++i) LV[i] = z[i];
+        treq = m.tn + m.hg;
+        req = RQ_STEP;
+      }
+      m.phase = ph;
     }
-    need_rhs = true;
-    if (!m.consume(f, y, treq)) break;
+    __syncthreads();
+    // ================= output lanes
+    if (is_out && s_opend[slot]) {                 // y(tout) is already in s_v
+      treq = s_tout[slot];
+      req = RQ_OUT;
+    }
+    // ================= Jacobian columns for every other lane
+    const unsigned long long freem = __ballot(req == RQ_NONE);
+    const int navail = jq_tail - jq_head;
+    int jown = 0, jcol = 0;
+    double jinc = 0.0;
+    if (req == RQ_NONE) {
+      const int r = lanes_below(freem);
+      if (r < navail) {
+        const int t = s_jq[(jq_head + r) % QCAP];
+        jown = t >> 6;
+        jcol = t & 63;
+        const int co = cell0 + jown;
+        Lane<R> Lo{B, s_m[jown], V, (unsigned)jown, (unsigned)co, pp};
+        const int k = jstate(jcol);
+        const double ysave = Lo.ZN(0, k);
+        double inc = srur * fabs(ysave);
+        const double alt = s_m[jown].mininc / Lo.EWT(k);
+        if (alt > inc) inc = alt;
+        // no runtime index into the register array: select the perturbed component
+        double z[N];
+        for (int i = 0; i < N; ++i) z[i] = (i == k) ? ysave + inc : Lo.ZN(0, i);
+        for (int i = 0; i < N; ++i) LV[i] = z[i];
+        jinc = inc;
+        treq = s_m[jown].tn;
+        req = RQ_JAC;
+      }
+    }
+    {
+      const int nfree = __popcll(freem);
+      jq_head += nfree < navail ? nfree : navail;
+    }
+    if (__ballot(req != RQ_NONE) == 0ull) break;      // every cell finished
+    // ================= one right-hand side per busy lane
+    ord::Currents cur;
+    double v0 = 0.0, ca0 = 0.0;
+    if (req != RQ_NONE) {
+      const int cc = req == RQ_JAC ? cell0 + jown : c;
+      const PrmAcc P{B.prm + cc, pp};
+      double y[N];
+      for (int i = 0; i < N; ++i) y[i] = LV[i];
+      v0 = y[0];
+      ca0 = y[9];
+      ord::rhs<R>(treq, y, LV, P, &cur);          // f overwrites the argument in LDS
+    }
+    const LVec &f = LV;
+    // ================= consume
+    bool publish = false;
+    if (req == RQ_STEP) {
+      Lane<R> L{B, s_m[slot], V, (unsigned)slot, (unsigned)c, pp};
+      Mach &m = s_m[slot];
+      m.nfe++;
+      int ph = m.phase;
+      bool fin = true;
+      for (int i = 0; i < N; ++i) fin = fin && isfinite(f[i]);
+      if (!fin) {
+        m.status = ST_NONFINITE; ph = PH_DONE;
+      } else if (ph == PH_F0) {
+        m.mnewt = 0;
+        if (m.callSetup) {
+          const double dgamma = fabs((m.gamma / m.gammap) - 1.0);
+          const bool jbad = (m.nst == 0) || (m.nst > m.nstlj + MSBJ) ||
+                            ((m.convfail == FAIL_BAD_J) && (dgamma < DGMAX_J)) || (m.convfail == FAIL_OTHER);
+          if (jbad) {
+            m.nstlj = m.nst;
+            m.jcur = 1;
+            double w[N];
+            for (int i = 0; i < N; ++i) w[i] = L.EWT(i);
+            double s = 0.0;
+            for (int i = 0; i < N; ++i) { const double t = f[i] * w[i]; s += t * t; }
+            for (int i = 0; i < N; ++i) L.FT(i) = f[i];
+            const double fnorm = sqrt(s / N);
+            m.mininc = (fnorm != 0.0) ? (1000.0 * fabs(m.h) * B.uround_fd * N * fnorm) : 1.0;
+            m.nfe += NJC;
+            publish = true;
+            ph = PH_WAIT_JAC;
+          } else {
+            m.jcur = 0;
+            for (int i = 0; i < N; ++i) L.FT(i) = f[i];
+            ph = PH_FACTOR;
+          }
+        } else {
+          for (int i = 0; i < N; ++i) L.FT(i) = f[i];
+          ph = PH_NEWTON;
+        }
+      } else if (ph == PH_ITER) {
+        for (int i = 0; i < N; ++i) L.FT(i) = f[i];
+        ph = PH_NEWTON;
+      } else if (ph == PH_RESTART_F) {
+        for (int i = 0; i < N; ++i) L.ZN(1, i) = m.h * f[i];
+        ph = PH_PREDICT;
+      } else if (ph == PH_INIT_F) {
+        for (int i = 0; i < N; ++i) L.ZN(1, i) = f[i];
+        L.set_ewt();
+        ph = L.hin_begin() ? PH_START : PH_HIN;
+      } else if (ph == PH_HIN) {
+        ph = L.hin_step(f) ? PH_START : PH_HIN;
+      }
+      m.phase = ph;
+    } else if (req == RQ_OUT) {
+      // CVode returned y(tout); computeRates at tout feeds the biomarkers
+      bool fin = true;
+      for (int i = 0; i < N; ++i) fin = fin && isfinite(f[i]);
+      if (!fin) s_ofail[slot] = 1;
+      const int iout = s_oiout[slot];
+      s_bm[slot].sample(iout, treq, v0, ca0, f[0],
+                        cur.INaL + cur.ICaL + cur.Ito + cur.IKr + cur.IKs + cur.IK1, cur.INaL, cur.ICaL,
+                        B.ct + c, B.cv + c, pp);
+      if (B.vtrace && iout > trace_start && iout - trace_start <= B.tcap)
+        B.vtrace[(iout - trace_start - 1) * pp + c] = v0;
+      s_opend[slot] = 0;
+    } else if (req == RQ_JAC) {
+      Lane<R> Lo{B, s_m[jown], V, (unsigned)jown, (unsigned)(cell0 + jown), pp};
+      const double iinv = 1.0 / jinc;
+      double ft[N];
+      for (int i = 0; i < N; ++i) ft[i] = Lo.FT(i);
+      for (int i = 0; i < N; ++i) Lo.JD(jcol, i) = iinv * f[i] - iinv * ft[i];
+    }
+    // integrators that need a fresh Jacobian queue their 48 columns
+    const unsigned long long pubm = __ballot(publish);
+    if (publish) {
+      const int base = jq_tail + NJC * lanes_below(pubm);
+      for (int k = 0; k < NJC; ++k) s_jq[(base + k) % QCAP] = (unsigned short)((slot << 6) | k);
+      s_m[slot].jend = base + NJC;
+    }
+    jq_tail += NJC * __popcll(pubm);
+    __syncthreads();
   }
-  m.store();
-  bm.store(B.bmd, B.bmi, B.pitch, c);
+  // ---- launch epilogue
+  if (is_int) {
+    Lane<R> L{B, s_m[slot], V0, (unsigned)slot, (unsigned)c, B.pitch};
+    Mach &m = s_m[slot];
+    if (m.phase == PH_DONE && m.status == ST_OK) m.phase = PH_STEP_DONE;
+    L.store();
+  }
+  if (is_out) s_bm[slot].store(B.bmd, B.bmi, B.pitch, c);
 }
 
 // ---- TT06: fixed-dt Rush-Larsen + biomarkers, one cell per lane
@@ -411,7 +750,7 @@ struct cep_batch {
   std::vector<host::TtCellIn> tt_in;
   // device memory
   double *d_y = nullptr, *d_prm = nullptr, *d_bmd = nullptr, *d_ct = nullptr, *d_cv = nullptr;
-  double *d_zn = nullptr, *d_ewt = nullptr, *d_acor = nullptr, *d_f0 = nullptr, *d_jd = nullptr;
+  double *d_zn = nullptr, *d_ewt = nullptr, *d_acor = nullptr, *d_ft = nullptr, *d_jd = nullptr;
   double *d_fac = nullptr, *d_sd = nullptr, *d_tcur = nullptr, *d_vtrace = nullptr;
   int *d_si = nullptr, *d_bmi = nullptr, *d_status = nullptr, *d_iout = nullptr;
   long long *d_stats = nullptr;
@@ -498,7 +837,7 @@ cep_batch *cep_batch_create(int model, int ncells, int precision)
     rc |= dalloc(b, &b->d_zn, P * 6 * ordbdf::N);
     rc |= dalloc(b, &b->d_ewt, P * ordbdf::N);
     rc |= dalloc(b, &b->d_acor, P * ordbdf::N);
-    rc |= dalloc(b, &b->d_f0, P * ordbdf::N);
+    rc |= dalloc(b, &b->d_ft, P * ordbdf::N);
     rc |= dalloc(b, &b->d_jd, P * ordbdf::NJC * ordbdf::N);
     rc |= dalloc(b, &b->d_fac, P * ordbdf::NFAC);
     rc |= dalloc(b, &b->d_sd, P * ordbdf::SD_N);
@@ -702,17 +1041,21 @@ int cep_run_beats_async(cep_batch *b, int nbeats)
   if (b->model == CEP_MODEL_ORD_CIPA) {
     Buffers B;
     B.ncell = b->n; B.pitch = b->pitch; B.y = b->d_y; B.zn = b->d_zn; B.ewt = b->d_ewt;
-    B.acor = b->d_acor; B.f0 = b->d_f0; B.jd = b->d_jd; B.fac = b->d_fac; B.sd = b->d_sd;
+    B.acor = b->d_acor; B.ft = b->d_ft; B.jd = b->d_jd; B.fac = b->d_fac; B.sd = b->d_sd;
     B.si = b->d_si; B.prm = b->d_prm; B.bmd = b->d_bmd; B.bmi = b->d_bmi; B.ct = b->d_ct;
     B.cv = b->d_cv; B.vtrace = b->vtrace_on ? b->d_vtrace : nullptr; B.tcap = b->tcap;
     B.stats = b->d_stats; B.status = b->d_status; B.dt = b->dt; B.rtol = b->rtol; B.atol = b->atol;
     B.uround_fd = b->prec == CEP_PREC_FP64 ? ordbdf::UROUND : 1.1920928955078125e-07;
-    B.maxord = b->maxord; B.nwin_unused = 0; B.cap = b->cap;
-    const int blocks = (b->n + 63) / 64;
+    B.maxord = b->maxord; B.max_steps = 500; B.cap = b->cap;
+    // safety net only: a step takes one pass when Newton converges at once, a few more on
+    // failures; cells still running after this many passes are flagged CEP_ST_MAX_STEPS
+    const long long max_passes = 64LL * nbeats * (long long)(max_bcl(b) / b->dt) + 100000;
+    constexpr int CPW = ordbdf::CPB;
+    const int blocks = (b->n + CPW - 1) / CPW;
     if (b->prec == CEP_PREC_FP64)
-      hipLaunchKernelGGL(ord_bdf_kernel<double>, dim3(blocks), dim3(64), 0, b->stream, B, rc, nbeats);
+      hipLaunchKernelGGL((ord_bdf_kernel<double, CPW>), dim3(blocks), dim3(64), 0, b->stream, B, rc, nbeats, max_passes);
     else
-      hipLaunchKernelGGL(ord_bdf_kernel<float>, dim3(blocks), dim3(64), 0, b->stream, B, rc, nbeats);
+      hipLaunchKernelGGL((ord_bdf_kernel<float, CPW>), dim3(blocks), dim3(64), 0, b->stream, B, rc, nbeats, max_passes);
   } else {
     TtBuffers B;
     B.ncell = b->n; B.pitch = b->pitch; B.cap = b->cap; B.tcap = b->tcap; B.y = b->d_y;

@@ -104,10 +104,10 @@ CEP_D double stimulus(double t, double start, double end, double amp, double per
   return (t >= start && t <= end && (t - start) - floor((t - start) / period) * period <= dur) ? amp : 0.0;
 }
 
-// Full RHS.  y/f: 52 doubles (registers after inlining).  P(k): per-cell parameter k.
-template <typename R, typename PA>
-CEP_D void rhs(double t, const double *__restrict__ y, double *__restrict__ f, const PA &P,
-               Currents *cur)
+// Full RHS.  y/f: 52 doubles, indexable as y[i] / f[i] (register arrays or an LDS view).
+// P(k): per-cell parameter k.
+template <typename R, typename PA, typename YV, typename FV>
+CEP_D void rhs(double t, const YV &y, FV &&f, const PA &P, Currents *cur)
 {
   using Mx = M<R>;
   const R v = (R)y[0];
