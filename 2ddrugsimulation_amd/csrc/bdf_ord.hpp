@@ -126,7 +126,7 @@ struct Mach {
 constexpr int CPB = 16;
 
 typedef __amdgpu_buffer_rsrc_t Rsrc;
-CEP_D inline Rsrc make_rsrc(const double *base, unsigned bytes)
+CEP_D Rsrc make_rsrc(const double *base, unsigned bytes)
 {
   return __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, (int)bytes, 0x00020000);
 }
@@ -261,10 +261,15 @@ struct Lane {
 
   // SW_RESTART: after the restore, evaluate f(tn, zn[0]) for zn[1] instead of predicting
   enum { SW_RESTORE = 1, SW_INC = 2, SW_DEC = 4, SW_RESCALE = 8, SW_PREDICT = 16, SW_RESTART = 32 };
-  // Everything done to the Nordsieck array between two nonlinear solves, in CVODE's order,
-  // in one sweep: undo a failed prediction (order q0), change the order (cvIncreaseBDF /
-  // cvDecreaseBDF with coefficients ll and a1, order q0), rescale row j by eta^j (order
-  // q1), predict (order q1).  y receives the resulting zn[0].
+  // Everything done to the Nordsieck array between two nonlinear solves, in CVODE's order:
+  // undo a failed prediction (order q0), change the order (cvIncreaseBDF / cvDecreaseBDF
+  // with coefficients ll and a1, order q0), rescale row j by eta^j (order q1), predict
+  // (order q1).  All of it is linear in a component's column (zn[0..5][i]), so the
+  // operations are first composed into one upper-triangular 6x6 map M (applied to the unit
+  // columns, in registers), then one sweep computes M z for the 52 components with no
+  // order-dependent branches.  Rows above the new order, except the saved correction in
+  // row QMAX, leave the map as zeros: the array keeps rows q+1..QMAX-1 at zero, which the
+  // interpolation and the other sweeps rely on.  y receives the new zn[0].
   template <typename YV>
   CEP_D void attempt_sweep(int ops, int q0, int q1, const double *ll, double a1, double eta, const YV &y)
   {
@@ -272,44 +277,62 @@ struct Lane {
     fac[0] = 1.0;
 #pragma unroll
     for (int j = 1; j <= QMAX; ++j) fac[j] = fac[j - 1] * eta;
-    const int qs = q0 > q1 ? q0 : q1;          // highest row written
-    const bool inc = (ops & SW_INC) != 0;       // reads the correction saved in zn[QMAX]
+    double M[QMAX + 1][QMAX + 1];
+#pragma unroll
+    for (int col = 0; col <= QMAX; ++col) {
+      double zz[QMAX + 1];
+#pragma unroll
+      for (int r = 0; r <= QMAX; ++r) zz[r] = (r == col) ? 1.0 : 0.0;
+      if (ops & SW_RESTORE) pascal_reg(zz, q0, -1);
+      if (ops & SW_INC) {
+        const double zl = a1 * zz[QMAX];
+#pragma unroll
+        for (int j = 2; j <= QMAX; ++j) if (j <= q0) zz[j] += ll[j] * zl;
+#pragma unroll
+        for (int j = 2; j <= QMAX; ++j) if (j == q0 + 1) zz[j] = zl;
+      }
+      if (ops & SW_DEC) {
+        double zq = 0.0;
+#pragma unroll
+        for (int j = 2; j <= QMAX; ++j) if (j == q0) zq = zz[j];
+#pragma unroll
+        for (int j = 2; j < QMAX; ++j) if (j < q0) zz[j] -= ll[j] * zq;
+      }
+      if (ops & SW_RESCALE) {
+#pragma unroll
+        for (int j = 1; j <= QMAX; ++j) if (j <= q1) zz[j] *= fac[j];
+      }
+      if (ops & SW_PREDICT) pascal_reg(zz, q1, 1);
+#pragma unroll
+      for (int r = 0; r < QMAX; ++r) if (r > q1) zz[r] = 0.0;
+#pragma unroll
+      for (int r = 0; r <= QMAX; ++r) M[r][col] = zz[r];
+    }
 #pragma unroll
     for (int c0 = 0; c0 < N; c0 += CH) {
       double z[CH][QMAX + 1];
 #pragma unroll
       for (int k = 0; k < CH; ++k)
 #pragma unroll
-        for (int j = 0; j <= QMAX; ++j) z[k][j] = (j <= qs || (inc && j == QMAX)) ? ZN(j, c0 + k) : 0.0;
+        for (int j = 0; j <= QMAX; ++j) z[k][j] = ZN(j, c0 + k);
 #pragma unroll
       for (int k = 0; k < CH; ++k) {
-        double *zz = z[k];
-        if (ops & SW_RESTORE) pascal_reg(zz, q0, -1);
-        if (inc) {
-          const double zl = a1 * zz[QMAX];
+        double w[QMAX + 1];
 #pragma unroll
-          for (int j = 2; j <= QMAX; ++j) if (j <= q0) zz[j] += ll[j] * zl;
+        for (int r = 0; r <= QMAX; ++r) {
+          double acc = 0.0;
 #pragma unroll
-          for (int j = 2; j <= QMAX; ++j) if (j == q0 + 1) zz[j] = zl;
+          for (int col = QMAX; col >= r; --col) acc += M[r][col] * z[k][col];
+          w[r] = acc;
         }
-        if (ops & SW_DEC) {
-          double zq = 0.0;
 #pragma unroll
-          for (int j = 2; j <= QMAX; ++j) if (j == q0) zq = zz[j];
-#pragma unroll
-          for (int j = 2; j < QMAX; ++j) if (j < q0) zz[j] -= ll[j] * zq;
-        }
-        if (ops & SW_RESCALE) {
-#pragma unroll
-          for (int j = 1; j <= QMAX; ++j) if (j <= q1) zz[j] *= fac[j];
-        }
-        if (ops & SW_PREDICT) pascal_reg(zz, q1, 1);
-        y[c0 + k] = zz[0];
+        for (int r = 0; r <= QMAX; ++r) z[k][r] = w[r];
+        y[c0 + k] = w[0];
       }
 #pragma unroll
       for (int k = 0; k < CH; ++k)
 #pragma unroll
-        for (int j = 0; j <= QMAX; ++j) if (j <= qs) ZN(j, c0 + k) = z[k][j];
+        for (int j = 0; j <= QMAX; ++j) ZN(j, c0 + k) = z[k][j];
     }
   }
 
@@ -663,9 +686,11 @@ struct Lane {
     const bool dn = norms && q > 1;
     const bool up = norms && (q != B.maxord) && (m.saved_tq5 != 0.0);
     const double cquot = up ? (m.tq5 / m.saved_tq5) * pow(m.h / m.tau[2], (double)(q + 1)) : 0.0;
-    double l[QMAX + 1];
+    // coefficients masked to the current order (no order-dependent branches in the sweep);
+    // oh selects row q for the order-decrease norm
+    double l[QMAX + 1], oh[QMAX + 1];
 #pragma unroll
-    for (int j = 0; j <= QMAX; ++j) l[j] = m.l[j];
+    for (int j = 0; j <= QMAX; ++j) { l[j] = (j <= q) ? m.l[j] : 0.0; oh[j] = (j == q) ? 1.0 : 0.0; }
     const double rt = B.rtol, at_ = B.atol;
     double s2 = 0.0, s3 = 0.0;
 #pragma unroll
@@ -674,30 +699,28 @@ struct Lane {
 #pragma unroll
       for (int k = 0; k < CH; ++k) {
 #pragma unroll
-        for (int j = 0; j <= QMAX; ++j) z[k][j] = (j <= q || (up && j == QMAX)) ? ZN(j, c0 + k) : 0.0;
+        for (int j = 0; j <= QMAX; ++j) z[k][j] = ZN(j, c0 + k);
         a[k] = ACOR(c0 + k);
         w[k] = EWT(c0 + k);
       }
 #pragma unroll
       for (int k = 0; k < CH; ++k) {
         const double zqm = z[k][QMAX];
+        double zq = 0.0;
 #pragma unroll
-        for (int j = 0; j <= QMAX; ++j) if (j <= q) z[k][j] += l[j] * a[k];
-        if (dn) {
-          double zq = 0.0;
-#pragma unroll
-          for (int j = 2; j <= QMAX; ++j) if (j == q) zq = z[k][j];
-          const double t = zq * w[k];
-          s2 += t * t;
+        for (int j = 0; j <= QMAX; ++j) {
+          z[k][j] += l[j] * a[k];
+          zq += oh[j] * z[k][j];
         }
+        if (dn) { const double t = zq * w[k]; s2 += t * t; }
         if (up) { const double t = (a[k] - cquot * zqm) * w[k]; s3 += t * t; }
+        if (save) z[k][QMAX] = a[k];
         w[k] = 1.0 / (rt * fabs(z[k][0]) + at_);
       }
 #pragma unroll
       for (int k = 0; k < CH; ++k) {
 #pragma unroll
-        for (int j = 0; j <= QMAX; ++j) if (j <= q) ZN(j, c0 + k) = z[k][j];
-        if (save) ZN(QMAX, c0 + k) = a[k];
+        for (int j = 0; j <= QMAX; ++j) ZN(j, c0 + k) = z[k][j];
         EWT(c0 + k) = w[k];
       }
     }
@@ -771,19 +794,21 @@ struct Lane {
   CEP_D void interpolate(double t, const YV &y) const
   {
     const double s = (t - m.tn) / m.h;
-    const int q = m.q;
-#pragma unroll 4
-    for (int i = 0; i < N; ++i) {
-      double z[QMAX + 1];
+    const bool q5 = m.q == QMAX;     // below order 5 row QMAX holds the saved correction
 #pragma unroll
-      for (int j = 0; j <= QMAX; ++j) z[j] = (j <= q) ? ZN(j, i) : 0.0;
-      double v = 0.0;
+    for (int c0 = 0; c0 < N; c0 += CH) {
+      double z[CH][QMAX + 1];
 #pragma unroll
-      for (int j = QMAX; j >= 0; --j) {
-        if (j == q) v = z[j];
-        else if (j < q) v = z[j] + s * v;
+      for (int k = 0; k < CH; ++k)
+#pragma unroll
+        for (int j = 0; j <= QMAX; ++j) z[k][j] = ZN(j, c0 + k);
+#pragma unroll
+      for (int k = 0; k < CH; ++k) {
+        double v = q5 ? z[k][QMAX] : 0.0;
+#pragma unroll
+        for (int j = QMAX - 1; j >= 0; --j) v = z[k][j] + s * v;   // rows above q are zero
+        y[c0 + k] = v;
       }
-      y[i] = v;
     }
   }
 

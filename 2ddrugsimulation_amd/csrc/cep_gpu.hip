@@ -420,7 +420,7 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
       m.nfe++;
       int ph = m.phase;
       bool fin = true;
-      for (int i = 0; i < N; ++i) fin = fin && isfinite(f[i]);
+      for (int i = 0; i < N; ++i) fin &= (bool)isfinite(f[i]);   // no short-circuit branches
       if (!fin) {
         m.status = ST_NONFINITE; ph = PH_DONE;
       } else if (ph == PH_F0) {
@@ -468,7 +468,7 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
     } else if (req == RQ_OUT) {
       // CVode returned y(tout); computeRates at tout feeds the biomarkers
       bool fin = true;
-      for (int i = 0; i < N; ++i) fin = fin && isfinite(f[i]);
+      for (int i = 0; i < N; ++i) fin &= (bool)isfinite(f[i]);   // no short-circuit branches
       if (!fin) s_ofail[slot] = 1;
       const int iout = s_oiout[slot];
       s_bm[slot].sample(iout, treq, v0, ca0, f[0],
@@ -480,9 +480,10 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
     } else if (req == RQ_JAC) {
       Lane<R> Lo{B, s_m[jown], V, (unsigned)jown, (unsigned)(cell0 + jown), pp};
       const double iinv = 1.0 / jinc;
-      double ft[N];
+      double ft[N], fv[N];
       for (int i = 0; i < N; ++i) ft[i] = Lo.FT(i);
-      for (int i = 0; i < N; ++i) Lo.JD(jcol, i) = iinv * f[i] - iinv * ft[i];
+      for (int i = 0; i < N; ++i) fv[i] = f[i];
+      for (int i = 0; i < N; ++i) Lo.JD(jcol, i) = iinv * fv[i] - iinv * ft[i];
     }
     // integrators that need a fresh Jacobian queue their 48 columns
     const unsigned long long pubm = __ballot(publish);
@@ -951,6 +952,8 @@ static int reset_integrators(cep_batch *b)
   HIPCHK(b, hipMemsetAsync(b->d_bmi, 0, P * BM_I_N * sizeof(int), b->stream));
   if (b->model == CEP_MODEL_ORD_CIPA) {
     HIPCHK(b, hipMemsetAsync(b->d_si, 0, P * ordbdf::SI_N * sizeof(int), b->stream));   // phase = FRESH
+    // the integrator keeps Nordsieck rows above the current order at zero
+    HIPCHK(b, hipMemsetAsync(b->d_zn, 0, P * (ordbdf::QMAX + 1) * ordbdf::N * sizeof(double), b->stream));
     HIPCHK(b, hipMemsetAsync(b->d_sd, 0, P * ordbdf::SD_N * sizeof(double), b->stream));
     HIPCHK(b, hipMemsetAsync(b->d_stats, 0, P * 8 * sizeof(long long), b->stream));
   } else {
