@@ -33,7 +33,7 @@ EXPORTED = (
     "cep_set_run_config", "cep_init_consts", "cep_get_constants", "cep_compute_rates",
     "cep_run_beats", "cep_run_beats_async", "cep_sync", "cep_get_states", "cep_set_states",
     "cep_get_biomarkers", "cep_get_status", "cep_get_stats", "cep_enable_vm_trace",
-    "cep_get_vm_trace", "cep_kernel_time",
+    "cep_get_vm_trace", "cep_kernel_time", "cep_query", "cep_batch_device_bytes",
 )
 
 _lib = None
@@ -78,6 +78,9 @@ def library(path: str | None = None):
     L.cep_enable_vm_trace.argtypes = [C.c_void_p, C.c_int]
     L.cep_get_vm_trace.argtypes = [C.c_void_p, dp, C.POINTER(C.c_int)]
     L.cep_kernel_time.argtypes = [C.c_void_p, dp, dp]
+    L.cep_query.argtypes = [C.c_void_p]
+    L.cep_batch_device_bytes.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int]
+    L.cep_batch_device_bytes.restype = C.c_longlong
     if path is None:
         _lib = L
     return L
@@ -183,6 +186,13 @@ class CellBatch:
     def run_beats_async(self, nbeats: int):
         self._chk(self.L.cep_run_beats_async(self.h, nbeats))
 
+    def query(self) -> bool:
+        """True while the last run_beats_async launch is still running."""
+        rc = self.L.cep_query(self.h)
+        if rc < 0:
+            self._chk(rc)
+        return rc == 1
+
     def sync(self):
         self._chk(self.L.cep_sync(self.h))
 
@@ -220,6 +230,14 @@ class CellBatch:
         avg, tot = C.c_double(), C.c_double()
         n = self.L.cep_kernel_time(self.h, C.byref(avg), C.byref(tot))
         return n, avg.value, tot.value
+
+
+def device_bytes(model: int, ncells: int, max_bcl: float, dt: float, vm_trace: bool = False) -> int:
+    """Device memory a batch of `ncells` cells takes (memory-based admission)."""
+    v = library().cep_batch_device_bytes(model, ncells, float(max_bcl), float(dt), int(vm_trace))
+    if v < 0:
+        raise CepError("cep_batch_device_bytes: bad arguments")
+    return int(v)
 
 
 def qinward(inal_auc_drug, ical_auc_drug, inal_auc_ctl, ical_auc_ctl):

@@ -1089,6 +1089,42 @@ int cep_sync(cep_batch *b)
   return 0;
 }
 
+int cep_query(cep_batch *b)
+{
+  if (!b) return CEP_E_ARG;
+  if (!b->pending) return 0;
+  const hipError_t e = hipEventQuery(b->ev1);
+  if (e == hipErrorNotReady) return 1;
+  if (e != hipSuccess) return fail(b, CEP_E_HIP, std::string("hipEventQuery: ") + hipGetErrorString(e));
+  return cep_sync(b) == 0 ? 0 : CEP_E_HIP;
+}
+
+long long cep_batch_device_bytes(int model, int ncells, double max_bcl, double dt, int vm_trace)
+{
+  if ((model != CEP_MODEL_ORD_CIPA && model != CEP_MODEL_TT06) || ncells <= 0 || !(dt > 0) || !(max_bcl > 0))
+    return CEP_E_ARG;
+  const long long P = (ncells + 63) / 64 * 64, pad = 64;
+  const bool ord = model == CEP_MODEL_ORD_CIPA;
+  const long long ns = ord ? ord::NS : tt::NS, np = ord ? ord::NP : tt::NP;
+  long long d = P * ns + P * np + P * BM_TOTAL_D;            // doubles
+  long long i4 = P * BM_I_N + P;                              // ints
+  if (ord) {
+    d += P * ((ordbdf::QMAX + 1) * ordbdf::N + 3 * ordbdf::N + ordbdf::NJC * ordbdf::N + ordbdf::NFAC + ordbdf::SD_N);
+    i4 += P * ordbdf::SI_N;
+    d += P * 8;                                               // stats (long long)
+  } else {
+    d += P;
+    i4 += P;
+  }
+  const int print_freq = (int)((1. / dt) * 1.0);
+  const long long beat_steps = (long long)(max_bcl / dt);
+  const long long cap = beat_steps / (print_freq > 0 ? print_freq : 1) + 4;
+  d += 2 * P * cap;
+  if (vm_trace) d += P * beat_steps;
+  const int nalloc = ord ? 14 : 10;
+  return d * 8 + i4 * 4 + nalloc * pad;
+}
+
 int cep_run_beats(cep_batch *b, int nbeats)
 {
   int rc = cep_run_beats_async(b, nbeats);

@@ -135,12 +135,24 @@ CEP_D void ionic(double t, const double *__restrict__ y, const PA &P, double *__
   const R EKs = c<R>(RTF) * Mx::lg(Mx::dv(c<R>(Ko + P_kna * Nao), Ki + c<R>(P_kna) * Nai));
   const R xs = (R)y[6];
   const R IKs = (R)P(TP_GKS) * xs * xs * (V - EKs);
-  // ICaL: GHK form with the reference's singular point at V = 15 mV kept
+  // ICaL: GHK form.  FP64 keeps the reference's expression, 0/0 at exactly V = 15 mV
+  // included (:327).  In FP32, e15 - 1 rounds to 0 for |V - 15| below ~1e-6 mV, far more
+  // often than V hits 15 exactly in FP64, so the FP32 path evaluates (V-15)/(e15-1) by its
+  // series 1/(2FRT) (1 - x/2), x = 2FRT (V-15), for |x| < 1e-4.
   const R v15 = V - c<R>(15);
-  const R e15 = Mx::ex(c<R>(2.0 * FRT) * v15);
-  const R ICaL = Mx::dv((R)P(TP_GCAL) * (R)y[11] * (R)y[12] * (R)y[13] * (R)y[14] * c<R>(4.0 * Fc * FRT) * v15
-                            * (c<R>(0.25) * Cass * e15 - c<R>(Cao)),
-                        e15 - c<R>(1));
+  const R x15 = c<R>(2.0 * FRT) * v15;
+  const R e15 = Mx::ex(x15);
+  R ICaL;
+  if constexpr (sizeof(R) == sizeof(float)) {
+    const R q = (fabsf(x15) < 1e-4f) ? c<R>(1.0 / (2.0 * FRT)) * (c<R>(1) - c<R>(0.5) * x15)
+                                     : Mx::dv(v15, e15 - c<R>(1));
+    ICaL = (R)P(TP_GCAL) * (R)y[11] * (R)y[12] * (R)y[13] * (R)y[14] * c<R>(4.0 * Fc * FRT) * q
+           * (c<R>(0.25) * Cass * e15 - c<R>(Cao));
+  } else {
+    ICaL = Mx::dv((R)P(TP_GCAL) * (R)y[11] * (R)y[12] * (R)y[13] * (R)y[14] * c<R>(4.0 * Fc * FRT) * v15
+                      * (c<R>(0.25) * Cass * e15 - c<R>(Cao)),
+                  e15 - c<R>(1));
+  }
   const R ECa = c<R>(0.5 * RTF) * Mx::lg(Mx::dv(c<R>(Cao), Cai));
   const R IbCa = (R)P(TP_GBCA) * (V - ECa);
   const R IpK = Mx::dv((R)P(TP_GPK) * dvk, c<R>(1) + Mx::ex((c<R>(25) - V) * c<R>(1 / 5.98)));

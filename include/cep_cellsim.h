@@ -109,6 +109,15 @@ int cep_compute_rates(cep_batch *b, double t, const double *states, double *rate
 int cep_run_beats(cep_batch *b, int nbeats);
 int cep_run_beats_async(cep_batch *b, int nbeats);
 int cep_sync(cep_batch *b);
+/* Non-blocking completion test of the last cep_run_beats_async: 1 while it runs, 0 when
+   done (then equivalent to cep_sync), <0 on error.  No reference counterpart: the
+   reference drivers run one cell to completion per process; the serving loop uses this to
+   overlap jobs on the device. */
+int cep_query(cep_batch *b);
+/* Device bytes a batch of ncells cells takes (state, integrator workspace, biomarker
+   traces for beats up to max_bcl at output step dt, optional Vm trace); used for
+   memory-based admission of jobs.  No reference counterpart. */
+long long cep_batch_device_bytes(int model, int ncells, double max_bcl, double dt, int vm_trace);
 
 /* Current STATES (n x nstates); set_states also restarts the integrators at t = 0 from
    the given states (the reference's Is_Using_Output restart, cipa_drug_bench.cpp:151). */

@@ -69,8 +69,12 @@ def test_tt06_rhs_matches_reference(prec, tol):
             R[m] = b.compute_rates(t, s)[m]
     Rg = g["R"]
     ok = np.isfinite(Rg).all(axis=1)
-    # the reference's 0/0 at V = 15 mV is reproduced, not masked
-    np.testing.assert_array_equal(np.isfinite(R).all(axis=1), ok)
+    if prec == cep.PREC_FP64:
+        # the reference's 0/0 at V = 15 mV is reproduced, not masked
+        np.testing.assert_array_equal(np.isfinite(R).all(axis=1), ok)
+    else:
+        # FP32 gating evaluates the GHK factor by its series near V = 15 mV: finite there
+        assert np.isfinite(R).all()
     scale = np.abs(Rg[ok]).max(axis=0) + 1e-300
     assert (np.abs(R[ok] - Rg[ok]) / scale).max() <= tol
 
