@@ -776,7 +776,9 @@ static int dalloc(cep_batch *b, T **p, size_t count)
   void *q = nullptr;
   hipError_t e = hipMalloc(&q, count * sizeof(T) + 64);
   if (e != hipSuccess) return fail(b, CEP_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
-  e = hipMemset(q, 0, count * sizeof(T) + 64);
+  // on the batch stream: a memset on the null stream is not ordered with the batch's
+  // (non-blocking) stream and could land after the first uploads into the buffer
+  e = hipMemsetAsync(q, 0, count * sizeof(T) + 64, b->stream);
   if (e != hipSuccess) return fail(b, CEP_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
   b->allocs.push_back(q);
   *p = (T *)q;
@@ -794,6 +796,8 @@ static int ensure_markov_table()
     t.B[k] = Bc;
   }
   if (hipMemcpyToSymbol(HIP_SYMBOL(ord::g_markov), &t, sizeof(t)) != hipSuccess) return CEP_E_HIP;
+  // batches run on non-blocking streams: make the table visible before any of them launches
+  if (hipDeviceSynchronize() != hipSuccess) return CEP_E_HIP;
   done = true;
   return 0;
 }
@@ -1180,7 +1184,9 @@ int cep_get_biomarkers(const cep_batch *b, double *out)
 int cep_get_status(const cep_batch *b, int *status)
 {
   if (!b || !status) return CEP_E_ARG;
-  if (hipMemcpy(status, b->d_status, (size_t)b->n * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return CEP_E_HIP;
+  if (hipMemcpyAsync(status, b->d_status, (size_t)b->n * sizeof(int), hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
+      hipStreamSynchronize(b->stream) != hipSuccess)
+    return CEP_E_HIP;
   return 0;
 }
 
@@ -1189,7 +1195,9 @@ int cep_get_stats(const cep_batch *b, long long *out)
   if (!b || !out) return CEP_E_ARG;
   if (!b->d_stats) { memset(out, 0, sizeof(long long) * 8 * b->n); return 0; }
   std::vector<long long> tmp((size_t)b->pitch * 8);
-  if (hipMemcpy(tmp.data(), b->d_stats, tmp.size() * sizeof(long long), hipMemcpyDeviceToHost) != hipSuccess) return CEP_E_HIP;
+  if (hipMemcpyAsync(tmp.data(), b->d_stats, tmp.size() * sizeof(long long), hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
+      hipStreamSynchronize(b->stream) != hipSuccess)
+    return CEP_E_HIP;
   for (int i = 0; i < b->n; ++i)
     for (int k = 0; k < 8; ++k) out[(size_t)i * 8 + k] = tmp[(size_t)k * b->pitch + i];
   return 0;

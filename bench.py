@@ -1,12 +1,14 @@
-"""Headline benchmark: ORd-CiPA cell-beats/s on one MI355X (batched population config).
+"""Headline benchmark: ORd-CiPA cell-beats/s on one MI355X (batched population), plus p50/p99
+job latency of the serving loop.
 
 Workload: 10,000 ORd-CiPA endo cells = 50 seeded drugs x 4 concentrations x 50 variants with
 7 lognormal(0, 0.2) conductance scales, BCL drawn from {500, 1000, 2000} ms, CVODE-style BDF
-(rtol 1e-6, atol 1e-8, max step = output dt = 0.1 ms), on-device biomarkers every output
+(rtol 1e-6, atol 1e-8, max step = output dt = 0.1 ms), on-device biomarkers at every output
 point.  A "step" is one beat of the whole population (every cell advances one beat of its
 own BCL); inputs and states are resident in HBM.  The timed solver is the mixed-precision
-one (FP32 gating/currents, FP64 state/Newton/LU); an FP64 run of a subsample checks APD90
-and qNet against it.
+one (FP32 gating/currents, FP64 state/Newton/LU); an FP64 run of a subsample, as one plain
+job, checks APD90 and qNet and gives the plain per-job FP64 rate.  A short serving session
+(Poisson arrivals of mixed ORd-CiPA / TT06 jobs) gives the p50/p99 latency.
 
 Prints one JSON line.  --gpus N > 1 is not supported (single-GPU build).
 """
@@ -24,15 +26,17 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 cep = importlib.import_module("2ddrugsimulation_amd")
+serving = importlib.import_module("2ddrugsimulation_amd.serving")
 
-# algorithmic work per unit for the BDF kernel (flops: + - * / each 1; transcendentals apart).
-# RHS counted by oracle/opcount.cpp on the ORd-CiPA model; setup/solve/step from the
-# structured Newton algebra (see the kernel header): factor ~2.7 kflop, solve+residual
-# ~1.45 kflop, Nordsieck predict/correct/error/order logic ~1.1 kflop per step.
-F_RHS, X_RHS = 1292, 172
+# Algorithmic work per unit of the BDF kernel (flops: + - * / each 1; transcendentals apart).
+# RHS: counted by oracle/opcount.cpp on the ORd-CiPA model (1292 flops, 172 exp/log/pow).
+# Newton solve with the structured factors: ~1450 flops; factorisation: ~2700 flops; per
+# accepted step the Nordsieck map, correction update, error test and weights: ~1100 flops.
+F_RHS = 1292
 F_SETUP, F_NEWTON, F_STEP = 2700, 1450, 1100
 PEAK_FP32 = 157.3e12
 PEAK_FP64 = 78.6e12
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "ord_bdf_population_10k_traffic.txt")
 
 
 def log(*a):
@@ -59,21 +63,39 @@ def make_batch(pop, prec, nbeats_total):
 
 
 def flops_of(stats: np.ndarray) -> float:
+    """Counted flops of the integration work recorded in the per-cell statistics
+    (steps, rhs evals incl. Jacobian columns, jacobians, setups, Newton iterations, ...,
+    output points: one rhs each)."""
     nst, nfe, nje, nsetups, nni = (stats[:, k].astype(float) for k in range(5))
     nout = stats[:, 7].astype(float)
     return float(np.sum((nfe + nout) * F_RHS + nsetups * F_SETUP + nni * F_NEWTON + nst * F_STEP))
 
 
+def read_traffic():
+    """HBM bytes per launch of the BDF kernel from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), or None."""
+    try:
+        kv = {}
+        for line in open(TRAFFIC_FILE):
+            if "=" in line and not line.startswith("#"):
+                k, v = line.split("=", 1)
+                kv[k.strip()] = v.strip()
+        return float(kv["hbm_bytes_per_beat_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(seconds: float):
+    """The FP64 CPU oracle (test infrastructure) on a bounded sample of the same population:
+    one beat from rest of a spread of cells, one cell stream per host thread."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
-    import ctypes as C
     threads = min(16, os.cpu_count() or 1)
     pop = make_population(10000)
-    # bounded sample: one beat of a spread of cells from the same population, from rest
     rng = np.random.default_rng(7)
-    ncell = max(threads, int(seconds * threads / 0.2))
-    idx = rng.choice(len(pop["bcl"]), size=ncell, replace=False)
+    ncell = max(threads, int(seconds * threads / 0.25))
+    idx = rng.choice(len(pop["bcl"]), size=min(ncell, len(pop["bcl"])), replace=False)
+    ncell = len(idx)
     cfgs = (O.OrdCellCfg * ncell)()
     for k, i in enumerate(idx):
         cfgs[k] = O.make_ord_cfg(celltype=pop["celltype"][i], conc=pop["conc"][i], herg=pop["herg"][i],
@@ -81,8 +103,23 @@ def cpu_baseline(seconds: float):
     res = (O.BmResult * ncell)()
     wall = O.lib().oracle_ord_run_many(cfgs, ncell, threads, res, None)
     return {"value": ncell / wall, "unit": "cell-beats/s", "cores": threads, "kind": "port",
-            "sample": f"{ncell} cells x 1 beat of the same population (FP64 BDF oracle, "
-                      f"{threads} threads, {wall:.1f} s)"}
+            "sample": f"{ncell} cells x 1 beat (from rest) of the same population, FP64 BDF oracle, "
+                      f"{threads} threads, {wall:.1f} s"}
+
+
+def serving_latency(njobs: int, rate: float):
+    """Short serving session: Poisson arrivals of screening jobs (1-400 cells, ORd-CiPA or
+    TT06, 2-4 beats each) into the continuous-batching loop."""
+    jobs = serving.synthetic_jobs(20251206, njobs, beats=(2, 4))
+    srv = serving.Server(mem_budget=32 << 30, max_active=16)
+    fin, wall = serving.run_poisson(srv, jobs, rate, seed=1, timeout=300)
+    p = serving.latency_percentiles(fin)
+    cb = sum(j.ncells * j.nbeats for j in fin)
+    bad = int(sum(int(np.sum(j.status != 0)) for j in fin))
+    return {"p50": p["p50"], "p99": p["p99"], "jobs": len(fin), "rate_jobs_per_s": rate,
+            "cell_beats_per_s": cb / wall, "failed_cells": bad,
+            "workload": f"{njobs} jobs, Poisson {rate}/s, cells log-uniform 1-400, 2-4 beats, "
+                        "ORd-CiPA or TT06 (p=0.5), mixed precision"}
 
 
 def main():
@@ -95,10 +132,12 @@ def main():
     ap.add_argument("--check-cells", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=24)
+    ap.add_argument("--serve-rate", type=float, default=4.0)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "single-GPU build: run with --gpus 1 (cells shard trivially across "
-                          "GPUs, see DESIGN.md)"}))
+        print(json.dumps({"error": "single-GPU build: run with --gpus 1 (cells shard trivially "
+                          "across GPUs; see the design notes)"}))
         sys.exit(2)
     prec = cep.PREC_MIXED if args.precision == "mixed" else cep.PREC_FP64
     K, W = args.steps, args.warmup
@@ -128,12 +167,12 @@ def main():
     nrhs = float(np.sum((stats1 - stats0)[:, [1, 7]]))
     b.close()
 
-    # accuracy of the timed solver vs the FP64 solver on a subsample (same beats)
+    # accuracy of the timed solver vs the FP64 solver on a subsample run as one plain job
     chk = {}
     if args.check_cells > 0:
         sel = np.linspace(0, n - 1, args.check_cells).astype(int)
         sub = {k: v[sel] for k, v in pop.items()}
-        log(f"bench: FP64 check on {len(sel)} cells")
+        log(f"bench: FP64 plain job of {len(sel)} cells")
         b64 = make_batch(sub, cep.PREC_FP64, nb_total)
         t2 = time.perf_counter()
         b64.run_beats(W + K)
@@ -141,17 +180,24 @@ def main():
         bm64 = b64.biomarker_array()
         b64.close()
         bi = {k: i for i, k in enumerate(cep.BM_FIELDS)}
+        plain = len(sel) * (W + K) / (t3 - t2)
         chk = {
             "apd90_max_abs_err_ms": float(np.abs(bm_mixed[sel, bi["apd90"]] - bm64[:, bi["apd90"]]).max()),
             "qnet_max_rel_err": float((np.abs(bm_mixed[sel, bi["qnet"]] - bm64[:, bi["qnet"]])
                                        / np.abs(bm64[:, bi["qnet"]])).max()),
-            "fp64_plain_job_cell_beats_per_s": len(sel) * (W + K) / (t3 - t2),
+            "fp64_plain_job_cell_beats_per_s": plain,
+            "batched_mixed_over_plain_fp64": value / plain,
             "cells_checked": int(len(sel)),
         }
+    lat = {"p50": None, "p99": None}
+    if args.serve_jobs > 0:
+        log(f"bench: serving session, {args.serve_jobs} jobs")
+        lat = serving_latency(args.serve_jobs, args.serve_rate)
     cpu = None
     if not args.no_cpu_baseline:
         log("bench: CPU baseline")
         cpu = cpu_baseline(args.cpu_seconds)
+    traffic = read_traffic()
     out = {
         "metric": "ORd cell-beats/s on 1 GPU at APD90 err <=1 ms; p50/p99 job latency",
         "value": value,
@@ -163,17 +209,18 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32-rates/fp64-state" if prec == cep.PREC_MIXED else "f64",
+        "dtype": "f32-rates/f64-state" if prec == cep.PREC_MIXED else "f64",
         "data": "synthetic (seeded drug/IC50/conductance population)",
         "config": {"workload": "ORd-CiPA batched population, 10,000 cells = 50 drugs x 4 conc x 50 "
                                "variants, BCL {500,1000,2000} ms, BDF rtol 1e-6/atol 1e-8, step = 1 beat",
                    "cells": n, "beats_per_job": nb_total},
         "accuracy_vs_fp64": chk,
         "failed_cells": int(np.sum(st != 0)),
-        "latency_ms": {"p50": None, "p99": None},
+        "latency_ms": lat,
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
-                     "kernel": f"ord_bdf_kernel<{'float' if prec == cep.PREC_MIXED else 'double'}>",
+                     "unit": "TFLOP/s", "frac": achieved / peak,
+                     "traffic": traffic,
+                     "kernel": f"ord_bdf_kernel<{'float' if prec == cep.PREC_MIXED else 'double'}, 16>",
                      "kernel_avg_ms": kavg_ms, "flops_per_launch": flops_launch,
                      "rhs_evals_per_launch": nrhs / max(nl, 1)},
         "cpu_baseline": cpu,

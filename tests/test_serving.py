@@ -128,3 +128,18 @@ def test_job_result_same_alone_and_in_the_mix():
     np.testing.assert_array_equal(alone.biomarkers, mixed.biomarkers)
     np.testing.assert_array_equal(alone.states, mixed.states)
     assert len(srv2.finished) == len(mix) + 1
+
+
+@pytest.mark.gpu
+def test_burst_of_jobs_all_start_from_their_initial_states():
+    """Sixteen jobs admitted at once while others run: every cell starts from its model's
+    initial states (buffers are cleared on the job's own stream, before its uploads) and
+    finishes without failure."""
+    jobs = serving.synthetic_jobs(11, 16, max_cells=30, beats=(1, 2))
+    srv = serving.Server(mem_budget=8 << 30, max_active=16)
+    for j in jobs:
+        srv.submit(j)
+    srv.drain()
+    assert len(srv.finished) == 16
+    for j in srv.finished:
+        assert (j.status == 0).all(), (j.jid, j.model, j.status)
