@@ -34,7 +34,13 @@ EXPORTED = (
     "cep_run_beats", "cep_run_beats_async", "cep_sync", "cep_get_states", "cep_set_states",
     "cep_get_biomarkers", "cep_get_status", "cep_get_stats", "cep_enable_vm_trace",
     "cep_get_vm_trace", "cep_kernel_time", "cep_query", "cep_batch_device_bytes",
+    "cep_tissue_create", "cep_tissue_destroy", "cep_tissue_last_error", "cep_tissue_set_cells",
+    "cep_tissue_set_conductivity", "cep_tissue_set_stimulus", "cep_tissue_step", "cep_tissue_diffuse",
+    "cep_tissue_get_vm", "cep_tissue_get_states", "cep_tissue_set_states", "cep_tissue_time",
+    "cep_tissue_kernel_times",
 )
+SPLIT_GODUNOV = 0
+SPLIT_STRANG = 1
 
 _lib = None
 dp = C.POINTER(C.c_double)
@@ -81,6 +87,25 @@ def library(path: str | None = None):
     L.cep_query.argtypes = [C.c_void_p]
     L.cep_batch_device_bytes.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int]
     L.cep_batch_device_bytes.restype = C.c_longlong
+    L.cep_tissue_create.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int]
+    L.cep_tissue_create.restype = C.c_void_p
+    L.cep_tissue_destroy.argtypes = [C.c_void_p]
+    L.cep_tissue_destroy.restype = None
+    L.cep_tissue_last_error.argtypes = [C.c_void_p]
+    L.cep_tissue_last_error.restype = C.c_char_p
+    L.cep_tissue_set_cells.argtypes = [C.c_void_p, dp, dp, dp, C.c_int, dp, C.c_int]
+    L.cep_tissue_set_conductivity.argtypes = [C.c_void_p, dp, dp, dp]
+    L.cep_tissue_set_stimulus.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.c_int, C.c_double, C.c_double,
+                                          C.c_double, C.c_double]
+    L.cep_tissue_step.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int,
+                                  C.POINTER(C.c_longlong)]
+    L.cep_tissue_diffuse.argtypes = [C.c_void_p, C.c_double, dp, dp, C.c_double, C.c_int, C.POINTER(C.c_int)]
+    L.cep_tissue_get_vm.argtypes = [C.c_void_p, dp]
+    L.cep_tissue_get_states.argtypes = [C.c_void_p, dp]
+    L.cep_tissue_set_states.argtypes = [C.c_void_p, dp]
+    L.cep_tissue_time.argtypes = [C.c_void_p]
+    L.cep_tissue_time.restype = C.c_double
+    L.cep_tissue_kernel_times.argtypes = [C.c_void_p, dp, C.POINTER(C.c_longlong)]
     if path is None:
         _lib = L
     return L
@@ -230,6 +255,97 @@ class CellBatch:
         avg, tot = C.c_double(), C.c_double()
         n = self.L.cep_kernel_time(self.h, C.byref(avg), C.byref(tot))
         return n, avg.value, tot.value
+
+
+class Tissue:
+    """2-D monodomain tissue of ORd-CiPA nodes on a structured Q1 grid (nx x ny elements of
+    hx x hy mm): the reference's Cheart reaction step + Cfem diffusion step (heart.cpp,
+    fem.cpp) on the device."""
+
+    def __init__(self, nx: int, ny: int, hx: float, hy: float, precision: int = PREC_FP64):
+        self.L = library()
+        h = self.L.cep_tissue_create(nx, ny, hx, hy, precision)
+        if not h:
+            raise CepError(f"cep_tissue_create({nx}, {ny}, {hx}, {hy}) failed")
+        self.h = h
+        self.nx, self.ny = nx, ny
+        self.nn = (nx + 1) * (ny + 1)
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise CepError(self.L.cep_tissue_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.cep_tissue_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def node(self, ix: int, iy: int) -> int:
+        return iy * (self.nx + 1) + ix
+
+    def set_cells(self, celltype=0.0, conc=0.0, herg=None, gscale=None):
+        n = self.nn
+        ct = np.ascontiguousarray(np.broadcast_to(np.asarray(celltype, dtype=np.float64), (n,)))
+        cc = np.ascontiguousarray(np.broadcast_to(np.asarray(conc, dtype=np.float64), (n,)))
+        hg = None if herg is None else np.ascontiguousarray(np.broadcast_to(np.asarray(herg, np.float64), (n, 19)))
+        gs = None if gscale is None else np.ascontiguousarray(np.broadcast_to(np.asarray(gscale, np.float64), (n, 7)))
+        self._chk(self.L.cep_tissue_set_cells(self.h, _p(ct), _p(cc), _p(hg), 19, _p(gs), 7))
+
+    def set_conductivity(self, sxx, sxy, syy):
+        ne = self.nx * self.ny
+        a = [np.ascontiguousarray(np.broadcast_to(np.asarray(v, np.float64), (ne,))) for v in (sxx, sxy, syy)]
+        self._chk(self.L.cep_tissue_set_conductivity(self.h, _p(a[0]), _p(a[1]), _p(a[2])))
+
+    def set_stimulus(self, nodes, amp_scale=1.0, bcl=1000.0, start=0.0, duration=0.5):
+        idx = np.ascontiguousarray(np.asarray(nodes, dtype=np.int32))
+        self._chk(self.L.cep_tissue_set_stimulus(self.h, idx.ctypes.data_as(C.POINTER(C.c_int)), len(idx),
+                                                 amp_scale, bcl, start, duration))
+
+    def step(self, nsteps: int, dt: float, split: int = SPLIT_GODUNOV, cg_rtol: float = 1e-5, cg_maxit: int = 200):
+        it = C.c_longlong()
+        self._chk(self.L.cep_tissue_step(self.h, nsteps, dt, split, cg_rtol, cg_maxit, C.byref(it)))
+        return it.value
+
+    def diffuse(self, dt: float, v, cg_rtol: float = 1e-5, cg_maxit: int = 500):
+        a = np.ascontiguousarray(v, dtype=np.float64)
+        x = np.empty(self.nn)
+        it = C.c_int()
+        self._chk(self.L.cep_tissue_diffuse(self.h, dt, _p(a), _p(x), cg_rtol, cg_maxit, C.byref(it)))
+        return x, it.value
+
+    def vm(self) -> np.ndarray:
+        out = np.empty(self.nn)
+        self._chk(self.L.cep_tissue_get_vm(self.h, _p(out)))
+        return out
+
+    def states(self) -> np.ndarray:
+        out = np.empty((self.nn, 52))
+        self._chk(self.L.cep_tissue_get_states(self.h, _p(out)))
+        return out
+
+    def set_states(self, s):
+        a = np.ascontiguousarray(s, dtype=np.float64)
+        self._chk(self.L.cep_tissue_set_states(self.h, _p(a)))
+
+    @property
+    def time(self) -> float:
+        return self.L.cep_tissue_time(self.h)
+
+    def kernel_times(self):
+        """(avg ms, launches) of the reaction, SpMV and CG-update kernels since the last call."""
+        avg = np.zeros(3)
+        n = (C.c_longlong * 3)()
+        self._chk(self.L.cep_tissue_kernel_times(self.h, _p(avg), n))
+        return avg, np.array(list(n))
 
 
 def device_bytes(model: int, ncells: int, max_bcl: float, dt: float, vm_trace: bool = False) -> int:

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/cep_cellsim.h"
+#include "../../include/cep_tissue.h"
 #include "bdf_ord.hpp"
 #include "tt06_model.hpp"
 
@@ -806,7 +807,17 @@ static int alloc_trace_buffers(cep_batch *b);
 
 extern "C" {
 
+// integrator = false: parameters and states only (no BDF workspace), for the tissue nodes
+static cep_batch *batch_create(int model, int ncells, int precision, bool integrator);
+
 cep_batch *cep_batch_create(int model, int ncells, int precision)
+{
+  return batch_create(model, ncells, precision, true);
+}
+
+}  // extern "C"
+
+static cep_batch *batch_create(int model, int ncells, int precision, bool integrator)
 {
   if ((model != CEP_MODEL_ORD_CIPA && model != CEP_MODEL_TT06) || ncells <= 0 ||
       (precision != CEP_PREC_FP64 && precision != CEP_PREC_MIXED))
@@ -838,7 +849,7 @@ cep_batch *cep_batch_create(int model, int ncells, int precision)
   rc |= dalloc(b, &b->d_bmd, P * BM_TOTAL_D);
   rc |= dalloc(b, &b->d_bmi, P * BM_I_N);
   rc |= dalloc(b, &b->d_status, P);
-  if (model == CEP_MODEL_ORD_CIPA) {
+  if (model == CEP_MODEL_ORD_CIPA && integrator) {
     rc |= dalloc(b, &b->d_zn, P * 6 * ordbdf::N);
     rc |= dalloc(b, &b->d_ewt, P * ordbdf::N);
     rc |= dalloc(b, &b->d_acor, P * ordbdf::N);
@@ -848,7 +859,7 @@ cep_batch *cep_batch_create(int model, int ncells, int precision)
     rc |= dalloc(b, &b->d_sd, P * ordbdf::SD_N);
     rc |= dalloc(b, &b->d_si, P * ordbdf::SI_N);
     rc |= dalloc(b, &b->d_stats, P * 8);
-  } else {
+  } else if (model == CEP_MODEL_TT06) {
     rc |= dalloc(b, &b->d_tcur, P);
     rc |= dalloc(b, &b->d_iout, P);
   }
@@ -858,6 +869,8 @@ cep_batch *cep_batch_create(int model, int ncells, int precision)
   }
   return b;
 }
+
+extern "C" {
 
 void cep_batch_destroy(cep_batch *b)
 {
@@ -1231,3 +1244,5 @@ int cep_kernel_time(cep_batch *b, double *avg_ms, double *total_ms)
 }
 
 }  // extern "C"
+
+#include "tissue.hpp"

@@ -104,10 +104,17 @@ CEP_D double stimulus(double t, double start, double end, double amp, double per
   return (t >= start && t <= end && (t - start) - floor((t - start) / period) * period <= dur) ? amp : 0.0;
 }
 
+// sink for the gate relaxation rates of rhs() when the caller does not want them
+struct NoGateRates {
+  double d;
+  CEP_D double &operator[](int) { return d; }
+};
+
 // Full RHS.  y/f: 52 doubles, indexable as y[i] / f[i] (register arrays or an LDS view).
-// P(k): per-cell parameter k.
-template <typename R, typename PA, typename YV, typename FV>
-CEP_D void rhs(double t, const YV &y, FV &&f, const PA &P, Currents *cur)
+// P(k): per-cell parameter k.  ga (optional): receives 1/tau of every Hodgkin-Huxley gate
+// and of the two Jrel relaxations (f[i] = (inf - y[i]) * ga[i]), for Rush-Larsen steps.
+template <typename R, typename PA, typename YV, typename FV, typename GA = NoGateRates>
+CEP_D void rhs(double t, const YV &y, FV &&f, const PA &P, Currents *cur, GA &&ga = GA())
 {
   using Mx = M<R>;
   const R v = (R)y[0];
@@ -317,8 +324,8 @@ CEP_D void rhs(double t, const YV &y, FV &&f, const PA &P, Currents *cur)
   f[43] = 0.0;
 
   // ---- gates (:1069-1098): (inf - state) formed in double
-#define GRATE_DIV(i, inf, tau) f[i] = ((double)(inf) - y[i]) * (double)Mx::rcp(tau)
-#define GRATE_MUL(i, inf, tauinv) f[i] = ((double)(inf) - y[i]) * (double)(tauinv)
+#define GRATE_DIV(i, inf, tau) (ga[i] = (double)Mx::rcp(tau), f[i] = ((double)(inf) - y[i]) * ga[i])
+#define GRATE_MUL(i, inf, tauinv) (ga[i] = (double)(tauinv), f[i] = ((double)(inf) - y[i]) * ga[i])
   GRATE_MUL(17, hLss, c<R>(1 / thL));
   GRATE_MUL(18, hLssp, c<R>(1 / thLp));
   GRATE_MUL(10, mss, tm_inv);
@@ -391,6 +398,35 @@ CEP_D void rhs(double t, const YV &y, FV &&f, const PA &P, Currents *cur)
     cur->IpCa = dIpCa; cur->ICab = dICab; cur->Istim = Istim; cur->Jrel = Jrel; cur->Jup = Jup;
     cur->CaMKa = CaMKa_d;
   }
+}
+
+// The 9x9 transition matrix of the Markov IKr block (states 34..42: IC1, IC2, C1, C2, O,
+// IO, IObound, Obound, Cbound) at voltage v: d/dt S = A S, the fluxes of rhs() above
+// (:1059-1068) written as a matrix (columns sum to zero), in FP64.
+template <typename PA>
+CEP_D void markov_matrix(double v, const PA &P, double (&A)[9][9])
+{
+  double k[14];
+#pragma unroll
+  for (int i = 0; i < 14; ++i) k[i] = g_markov.AQ[i] * exp(g_markov.B[i] * v);
+  const double k1 = k[0], k2 = k[1], k3m = k[2], k4m = k[3], k11 = k[4], k21 = k[5], k31 = k[6], k41 = k[7];
+  const double k51 = k[8], k52 = k[9], k53 = k[10], k61 = k[11], k62 = k[12], k63 = k[13];
+  const double Ku = P(OP_KU), Kt = P(OP_KT), bind = P(OP_BIND);
+  const double trap = Kt / (1.0 + exp(-(v - P(OP_VHALF)) * (1 / 6.789)));
+  const double unbindIO = Ku * k53 / k63;
+#pragma unroll
+  for (int i = 0; i < 9; ++i)
+#pragma unroll
+    for (int j = 0; j < 9; ++j) A[i][j] = 0.0;
+  A[0][0] = -k11 - k61; A[0][1] = k21; A[0][2] = k51;
+  A[1][0] = k11; A[1][1] = -k21 - k3m - k62; A[1][3] = k52; A[1][5] = k4m;
+  A[2][0] = k61; A[2][2] = -k1 - k51; A[2][3] = k2;
+  A[3][1] = k62; A[3][2] = k1; A[3][3] = -k2 - k31 - k52; A[3][4] = k41;
+  A[4][3] = k31; A[4][4] = -k41 - k53 - bind; A[4][5] = k63; A[4][7] = Ku;
+  A[5][1] = k3m; A[5][4] = k53; A[5][5] = -k4m - k63 - bind; A[5][6] = unbindIO;
+  A[6][5] = bind; A[6][6] = -unbindIO - Kt; A[6][8] = trap;
+  A[7][4] = bind; A[7][7] = -Ku - Kt; A[7][8] = trap;
+  A[8][6] = Kt; A[8][7] = Kt; A[8][8] = -2.0 * trap;
 }
 
 }  // namespace ord

@@ -1,0 +1,629 @@
+// tissue.hpp -- 2-D monodomain tissue step on gfx950: ORd-CiPA reaction per node + Q1 FEM
+// diffusion solved by Jacobi-preconditioned CG.  Included once, at the end of cep_gpu.hip
+// (it uses the batch internals defined there).
+//
+// Reference: Cheart::cellular_reaction (heart.cpp:274-430) and Cfem::set_amat_bmat /
+// assemble_Amat_Bmat / solve_Bmat (fem.cpp:628-661,721-763,705-718), Q1 shape functions and
+// 2x2 Gauss points (fem.cpp:256-271,398-426).  Layout and kernels:
+//   * nodes (nx+1) x (ny+1), index iy*(nx+1)+ix; the states of the node cells are the SoA
+//     state planes of an ORd batch (plane 0 = Vm, read and written in place by the solver);
+//   * A = M/dt + K and B = M/dt assembled on the host as nine FP64 coefficient planes
+//     (stencil slot s = (dy+1)*3 + (dx+1)), zero outside the grid;
+//   * one CG iteration = three kernels: SpMV q = A p on 16x16 node tiles with the p tile and
+//     its halo staged in LDS, fused with the block partial of p.q; the update x += a p,
+//     r -= a q, z = D^-1 r fused with the partials of r.z and z.z; the direction p = z + b p.
+//     Scalars (a, b, the stop test) are reduced on the device by the last block to finish
+//     (fixed summation order: results do not depend on scheduling), so iterations run
+//     without host round trips; the host checks the stop flag every few iterations.
+// Algorithmic HBM bytes per CG iteration and node: SpMV 9*8 (A) + 8 (p) + 8 (q) = 88;
+// update 8*5 read (x, p, r, q, dinv) + 8*3 written = 64; direction 16 read + 8 written =
+// 24; total 176 B/node/iteration.
+#pragma once
+
+namespace cep {
+namespace tis {
+
+constexpr int TX = 16, TY = 16, TB = TX * TY;   // node tile per workgroup (4 waves)
+
+struct Scal {            // device-resident CG state
+  double rtol2;
+  int done, iters, maxit;
+};
+
+// partial sums of the block reductions: [PQ | RZ(even) | RZ(odd) | ZZ | ZZ0] x nblk
+enum { P_PQ = 0, P_RZ0 = 1, P_RZ1 = 2, P_ZZ = 3, P_ZZ0 = 4, P_N = 5 };
+
+// workgroup sum of NV values; the result is valid in thread 0
+template <int NV>
+CEP_D void block_sum(double (&v)[NV], double *sm)
+{
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) sm[wid * NV + k] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double s = 0.0;
+      for (int w = 0; w < TB / 64; ++w) s += sm[w * NV + k];
+      v[k] = s;
+    }
+}
+
+// the total of one partial array (written by the previous kernel), computed identically by
+// every workgroup: fixed strided order per thread, fixed tree; broadcast through LDS
+CEP_D double grid_total(const double *part, int nblk, double *sm, double *bc)
+{
+  double v[1] = { 0.0 };
+  for (int i = threadIdx.x; i < nblk; i += TB) v[0] += part[i];
+  block_sum<1>(v, sm);
+  if (threadIdx.x == 0) *bc = v[0];
+  __syncthreads();
+  return *bc;
+}
+
+struct Grid {
+  int nxn, nyn, nn, nblk;   // nodes per row / column, nodes, workgroups
+};
+
+CEP_D int block_id() { return blockIdx.y * gridDim.x + blockIdx.x; }
+
+// q = A p over a 16x16 tile (LDS-staged p with halo); partial p.q (mode 1)
+__global__ void __launch_bounds__(TB) cg_spmv(Grid g, const double *__restrict__ A, const double *__restrict__ p,
+                                              double *__restrict__ q, double *__restrict__ part, const Scal *S,
+                                              int mode)
+{
+  if (mode && S->done) return;
+  __shared__ double tile[TY + 2][TX + 2];
+  __shared__ double sm[TB / 64 * 2];
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+  const int ix = blockIdx.x * TX + tx, iy = blockIdx.y * TY + ty;
+  // stage p for the tile and its one-node halo (zero outside the grid)
+  for (int k = threadIdx.x; k < (TX + 2) * (TY + 2); k += TB) {
+    const int hx = k % (TX + 2), hy = k / (TX + 2);
+    const int gx = blockIdx.x * TX + hx - 1, gy = blockIdx.y * TY + hy - 1;
+    tile[hy][hx] = (gx >= 0 && gx < g.nxn && gy >= 0 && gy < g.nyn) ? p[gy * g.nxn + gx] : 0.0;
+  }
+  __syncthreads();
+  double v[1] = { 0.0 };
+  if (ix < g.nxn && iy < g.nyn) {
+    const int i = iy * g.nxn + ix;
+    double s = 0.0;
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+      for (int dx = -1; dx <= 1; ++dx) s += A[((dy + 1) * 3 + (dx + 1)) * g.nn + i] * tile[ty + 1 + dy][tx + 1 + dx];
+    q[i] = s;
+    v[0] = tile[ty + 1][tx + 1] * s;
+  }
+  if (mode == 0) return;                        // plain product (b = B v)
+  block_sum<1>(v, sm);
+  if (threadIdx.x == 0) part[P_PQ * g.nblk + block_id()] = v[0];
+}
+
+// CG start from x = 0: r = b, z = D^-1 r, p = z; partials r.z (into RZ(even)) and z.z (ZZ0)
+__global__ void __launch_bounds__(TB) cg_init(Grid g, const double *__restrict__ b, const double *__restrict__ dinv,
+                                              double *__restrict__ x, double *__restrict__ r, double *__restrict__ z,
+                                              double *__restrict__ p, double *__restrict__ part)
+{
+  __shared__ double sm[TB / 64 * 2];
+  const int ix = blockIdx.x * TX + threadIdx.x % TX, iy = blockIdx.y * TY + threadIdx.x / TX;
+  double v[2] = { 0.0, 0.0 };
+  if (ix < g.nxn && iy < g.nyn) {
+    const int i = iy * g.nxn + ix;
+    const double ri = b[i], zi = dinv[i] * ri;
+    x[i] = 0.0; r[i] = ri; z[i] = zi; p[i] = zi;
+    v[0] = ri * zi; v[1] = zi * zi;
+  }
+  block_sum<2>(v, sm);
+  if (threadIdx.x == 0) {
+    part[P_RZ0 * g.nblk + block_id()] = v[0];
+    part[P_ZZ0 * g.nblk + block_id()] = v[1];
+  }
+}
+
+// iteration k: a = RZ(k) / PQ; x += a p, r -= a q, z = D^-1 r; partials of r.z (RZ(k+1)) and z.z
+__global__ void __launch_bounds__(TB) cg_update(Grid g, const double *__restrict__ p, const double *__restrict__ q,
+                                                const double *__restrict__ dinv, double *__restrict__ x,
+                                                double *__restrict__ r, double *__restrict__ z,
+                                                double *__restrict__ part, const Scal *S, int k)
+{
+  if (S->done) return;
+  __shared__ double sm[TB / 64 * 2];
+  __shared__ double bc;
+  const double rz = grid_total(part + ((k & 1) ? P_RZ1 : P_RZ0) * g.nblk, g.nblk, sm, &bc);
+  const double pq = grid_total(part + P_PQ * g.nblk, g.nblk, sm, &bc);
+  const double a = (pq != 0.0) ? rz / pq : 0.0;    // b = 0: x stays 0
+  const int ix = blockIdx.x * TX + threadIdx.x % TX, iy = blockIdx.y * TY + threadIdx.x / TX;
+  double v[2] = { 0.0, 0.0 };
+  if (ix < g.nxn && iy < g.nyn) {
+    const int i = iy * g.nxn + ix;
+    x[i] += a * p[i];
+    const double ri = r[i] - a * q[i];
+    const double zi = dinv[i] * ri;
+    r[i] = ri; z[i] = zi;
+    v[0] = ri * zi; v[1] = zi * zi;
+  }
+  block_sum<2>(v, sm);
+  if (threadIdx.x == 0) {
+    part[((k & 1) ? P_RZ0 : P_RZ1) * g.nblk + block_id()] = v[0];
+    part[P_ZZ * g.nblk + block_id()] = v[1];
+  }
+}
+
+// iteration k: stop test ||z|| <= rtol ||z0|| (preconditioned norm, as PETSc KSPCG) or
+// k+1 == maxit; else b = RZ(k+1) / RZ(k), p = z + b p
+__global__ void __launch_bounds__(TB) cg_dir(Grid g, const double *__restrict__ z, double *__restrict__ p,
+                                             const double *__restrict__ part, Scal *S, int k)
+{
+  if (S->done) return;
+  __shared__ double sm[TB / 64 * 2];
+  __shared__ double bc;
+  const double zz = grid_total(part + P_ZZ * g.nblk, g.nblk, sm, &bc);
+  const double zz0 = grid_total(part + P_ZZ0 * g.nblk, g.nblk, sm, &bc);
+  if (zz <= S->rtol2 * zz0 || k + 1 >= S->maxit) {
+    if (block_id() == 0 && threadIdx.x == 0) { S->done = 1; S->iters = (zz0 > 0.0) ? k + 1 : 0; }
+    return;
+  }
+  const double rz_new = grid_total(part + ((k & 1) ? P_RZ0 : P_RZ1) * g.nblk, g.nblk, sm, &bc);
+  const double rz_old = grid_total(part + ((k & 1) ? P_RZ1 : P_RZ0) * g.nblk, g.nblk, sm, &bc);
+  const double bt = rz_new / rz_old;
+  const int ix = blockIdx.x * TX + threadIdx.x % TX, iy = blockIdx.y * TY + threadIdx.x / TX;
+  if (ix < g.nxn && iy < g.nyn) {
+    const int i = iy * g.nxn + ix;
+    p[i] = z[i] + bt * p[i];
+  }
+  if (block_id() == 0 && threadIdx.x == 0) S->iters = k + 1;
+}
+
+// reaction step of every node over h (the reference's solveAnalytical role; ORd-CiPA has
+// none there, :1113-1115): Rush-Larsen for the Hodgkin-Huxley gates and the Jrel
+// relaxations, backward Euler for the Markov IKr block (linear in its states; drug binding
+// and trapping make it stiff: rates up to Kmax*Ku), forward Euler for V, concentrations,
+// CaMK and the charge accumulators.  All rates at the start of the step.
+template <typename R>
+__global__ void __launch_bounds__(64) tissue_react(int n, int pitch, double t, double h, double *__restrict__ y,
+                                                   const double *__restrict__ prm)
+{
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  double s[ord::NS], f[ord::NS], a[ord::NS];
+#pragma unroll
+  for (int i = 0; i < ord::NS; ++i) { s[i] = y[(size_t)i * pitch + c]; a[i] = 0.0; }
+  ord::Currents cur;
+  const PrmAcc P{prm + c, pitch};
+  ord::rhs<R>(t, s, f, P, &cur, a);
+  // Markov block: (I - h A) S1 = S0, no pivoting (an M-matrix)
+  double M[9][9], b[9];
+  ord::markov_matrix(s[0], P, M);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+#pragma unroll
+    for (int j = 0; j < 9; ++j) M[i][j] = (i == j ? 1.0 : 0.0) - h * M[i][j];
+    b[i] = s[34 + i];
+  }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    const double pinv = 1.0 / M[k][k];
+#pragma unroll
+    for (int i = k + 1; i < 9; ++i) {
+      const double l = M[i][k] * pinv;
+#pragma unroll
+      for (int j = k + 1; j < 9; ++j) M[i][j] -= l * M[k][j];
+      b[i] -= l * b[k];
+    }
+  }
+#pragma unroll
+  for (int i = 8; i >= 0; --i) {
+#pragma unroll
+    for (int j = i + 1; j < 9; ++j) b[i] -= M[i][j] * b[j];
+    b[i] /= M[i][i];
+  }
+#pragma unroll
+  for (int i = 0; i < ord::NS; ++i) {
+    double v;
+    if (i >= 34 && i <= 42) v = b[i - 34];
+    else if (a[i] > 0.0) v = s[i] + f[i] * (-expm1(-h * a[i]) / a[i]);   // Rush-Larsen
+    else v = s[i] + h * f[i];
+    y[(size_t)i * pitch + c] = v;
+  }
+}
+
+}  // namespace tis
+}  // namespace cep
+
+struct cep_tissue {
+  cep_batch *cells = nullptr;
+  int nx = 0, ny = 0, nxn = 0, nyn = 0, nn = 0;
+  double hx = 0, hy = 0, t = 0, dt_built = -1;
+  std::string err;
+  std::vector<double> sxx, sxy, syy;       // per element
+  std::vector<int> stim_nodes;
+  double stim_scale = 1.0, stim_bcl = 1000.0, stim_start = 0.0, stim_dur = 0.5;
+  double *d_A = nullptr, *d_B = nullptr, *d_dinv = nullptr;
+  double *d_b = nullptr, *d_r = nullptr, *d_z = nullptr, *d_p = nullptr, *d_q = nullptr, *d_part = nullptr;
+  cep::tis::Scal *d_S = nullptr;
+  std::vector<void *> allocs;
+  std::vector<hipEvent_t> ev;              // pairs for kernel timing
+  double tsum[3] = { 0, 0, 0 };
+  long long tcount[3] = { 0, 0, 0 };
+};
+
+static int tfail(cep_tissue *t, int code, const std::string &msg)
+{
+  if (t) t->err = msg;
+  return code;
+}
+
+#define TCHK(t, call)                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess) return tfail(t, CEP_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+static int talloc(cep_tissue *t, T **p, size_t count)
+{
+  void *q = nullptr;
+  hipError_t e = hipMalloc(&q, count * sizeof(T) + 64);
+  if (e == hipSuccess) e = hipMemsetAsync(q, 0, count * sizeof(T) + 64, t->cells->stream);
+  if (e != hipSuccess) return tfail(t, CEP_E_HIP, std::string("tissue alloc: ") + hipGetErrorString(e));
+  t->allocs.push_back(q);
+  *p = (T *)q;
+  return 0;
+}
+
+// global 9-point stencils of A = M/dt + K and B = M/dt from the Q1 element matrices
+static void tissue_assemble(cep_tissue *t, double dt, std::vector<double> &A, std::vector<double> &Bm,
+                            std::vector<double> &dinv)
+{
+  const int nn = t->nn, nxn = t->nxn;
+  A.assign((size_t)9 * nn, 0.0);
+  Bm.assign((size_t)9 * nn, 0.0);
+  // element node order (0,0), (1,0), (1,1), (0,1) as the reference's set_Phi_dPhiS_square
+  const int ox[4] = { 0, 1, 1, 0 }, oy[4] = { 0, 0, 1, 1 };
+  const double g[2] = { -0.57735, 0.57735 };             // the reference's Gauss points (fem.cpp:256-270)
+  const double gs[4] = { g[0], g[1], g[1], g[0] }, gt[4] = { g[0], g[0], g[1], g[1] };
+  const double hx = t->hx, hy = t->hy, detJ = 0.25 * hx * hy;
+  double Me[16] = { 0 }, Ks[3][16] = { { 0 } };            // mass and the three tensor parts of K
+  for (int gp = 0; gp < 4; ++gp) {
+    const double s = gs[gp], u = gt[gp];
+    const double phi[4] = { (1. - s) * (1. - u) / 4., (1. + s) * (1. - u) / 4., (1. + s) * (1. + u) / 4., (1. - s) * (1. + u) / 4. };
+    const double dps[4] = { -(1. - u) / 4., (1. - u) / 4., (1. + u) / 4., -(1. + u) / 4. };
+    const double dpt[4] = { -(1. - s) / 4., -(1. + s) / 4., (1. + s) / 4., (1. - s) / 4. };
+    double ddx[4], ddy[4];
+    for (int a = 0; a < 4; ++a) { ddx[a] = dps[a] * (2.0 / hx); ddy[a] = dpt[a] * (2.0 / hy); }
+    for (int j = 0; j < 4; ++j)
+      for (int k = 0; k < 4; ++k) {
+        Me[j * 4 + k] += phi[j] * phi[k] * detJ;
+        Ks[0][j * 4 + k] += ddx[j] * ddx[k] * detJ;                          // sxx part
+        Ks[1][j * 4 + k] += (ddx[j] * ddy[k] + ddy[j] * ddx[k]) * detJ;      // sxy part
+        Ks[2][j * 4 + k] += ddy[j] * ddy[k] * detJ;                          // syy part
+      }
+  }
+  for (int ey = 0; ey < t->ny; ++ey)
+    for (int ex = 0; ex < t->nx; ++ex) {
+      const int e = ey * t->nx + ex;
+      for (int j = 0; j < 4; ++j) {
+        const int i = (ey + oy[j]) * nxn + (ex + ox[j]);
+        for (int k = 0; k < 4; ++k) {
+          const int dx = ox[k] - ox[j], dy = oy[k] - oy[j];
+          const int slot = (dy + 1) * 3 + (dx + 1);
+          const double kk = t->sxx[e] * Ks[0][j * 4 + k] + t->sxy[e] * Ks[1][j * 4 + k] + t->syy[e] * Ks[2][j * 4 + k];
+          A[(size_t)slot * nn + i] += Me[j * 4 + k] / dt + kk;
+          Bm[(size_t)slot * nn + i] += Me[j * 4 + k] / dt;
+        }
+      }
+    }
+  dinv.resize(nn);
+  for (int i = 0; i < nn; ++i) dinv[i] = 1.0 / A[(size_t)4 * nn + i];
+}
+
+static int tissue_build(cep_tissue *t, double dt)
+{
+  if (t->dt_built == dt) return 0;
+  std::vector<double> A, Bm, dinv;
+  tissue_assemble(t, dt, A, Bm, dinv);
+  hipStream_t s = t->cells->stream;
+  TCHK(t, hipMemcpyAsync(t->d_A, A.data(), A.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  TCHK(t, hipMemcpyAsync(t->d_B, Bm.data(), Bm.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  TCHK(t, hipMemcpyAsync(t->d_dinv, dinv.data(), dinv.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  TCHK(t, hipStreamSynchronize(s));
+  t->dt_built = dt;
+  return 0;
+}
+
+static void tissue_timed_begin(cep_tissue *t, int slot)
+{
+  (void)hipEventRecord(t->ev[2 * slot], t->cells->stream);
+}
+static void tissue_timed_end(cep_tissue *t, int slot)
+{
+  (void)hipEventRecord(t->ev[2 * slot + 1], t->cells->stream);
+}
+static void tissue_timed_collect(cep_tissue *t, int slot)
+{
+  float ms = 0.f;
+  if (hipEventSynchronize(t->ev[2 * slot + 1]) == hipSuccess &&
+      hipEventElapsedTime(&ms, t->ev[2 * slot], t->ev[2 * slot + 1]) == hipSuccess) {
+    t->tsum[slot] += ms;
+    t->tcount[slot] += 1;
+  }
+}
+
+// one diffusion solve A x = B v with v = x on entry (x: device vector of nn)
+static int tissue_solve(cep_tissue *t, double dt, double *x, double rtol, int maxit, int *iters)
+{
+  using namespace cep::tis;
+  if (int rc = tissue_build(t, dt)) return rc;
+  hipStream_t s = t->cells->stream;
+  const Grid g{ t->nxn, t->nyn, t->nn, ((t->nxn + TX - 1) / TX) * ((t->nyn + TY - 1) / TY) };
+  const dim3 grid((t->nxn + TX - 1) / TX, (t->nyn + TY - 1) / TY), blk(TB);
+  Scal init{};
+  init.rtol2 = rtol * rtol;
+  init.maxit = maxit;
+  TCHK(t, hipMemcpyAsync(t->d_S, &init, sizeof(Scal), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(cg_spmv, grid, blk, 0, s, g, t->d_B, x, t->d_b, t->d_part, t->d_S, 0);     // b = B v
+  hipLaunchKernelGGL(cg_init, grid, blk, 0, s, g, t->d_b, t->d_dinv, x, t->d_r, t->d_z, t->d_p, t->d_part);
+  TCHK(t, hipGetLastError());
+  constexpr int CHUNK = 4;
+  Scal h{};
+  for (int it = 0; it < maxit; it += CHUNK) {
+    for (int k = it; k < it + CHUNK && k < maxit; ++k) {
+      // time the first iteration of each chunk (later ones may exit early once converged)
+      const bool tm = k == it;
+      if (tm) tissue_timed_begin(t, 1);
+      hipLaunchKernelGGL(cg_spmv, grid, blk, 0, s, g, t->d_A, t->d_p, t->d_q, t->d_part, t->d_S, 1);
+      if (tm) { tissue_timed_end(t, 1); tissue_timed_begin(t, 2); }
+      hipLaunchKernelGGL(cg_update, grid, blk, 0, s, g, t->d_p, t->d_q, t->d_dinv, x, t->d_r, t->d_z, t->d_part,
+                         t->d_S, k);
+      if (tm) tissue_timed_end(t, 2);
+      hipLaunchKernelGGL(cg_dir, grid, blk, 0, s, g, t->d_z, t->d_p, t->d_part, t->d_S, k);
+    }
+    TCHK(t, hipGetLastError());
+    TCHK(t, hipMemcpyAsync(&h, t->d_S, sizeof(Scal), hipMemcpyDeviceToHost, s));
+    TCHK(t, hipStreamSynchronize(s));
+    tissue_timed_collect(t, 1);
+    tissue_timed_collect(t, 2);
+    if (h.done) break;
+  }
+  if (iters) *iters = h.iters;
+  return 0;
+}
+
+static int tissue_react_launch(cep_tissue *t, double h)
+{
+  cep_batch *b = t->cells;
+  const int blocks = (t->nn + 63) / 64;
+  tissue_timed_begin(t, 0);
+  if (b->prec == CEP_PREC_FP64)
+    hipLaunchKernelGGL(cep::tis::tissue_react<double>, dim3(blocks), dim3(64), 0, b->stream, t->nn, b->pitch, t->t, h,
+                       b->d_y, b->d_prm);
+  else
+    hipLaunchKernelGGL(cep::tis::tissue_react<float>, dim3(blocks), dim3(64), 0, b->stream, t->nn, b->pitch, t->t, h,
+                       b->d_y, b->d_prm);
+  tissue_timed_end(t, 0);
+  TCHK(t, hipGetLastError());
+  return 0;
+}
+
+// node parameters: the cell parameters with the stimulus kept only on the pacing nodes
+static int tissue_upload_params(cep_tissue *t)
+{
+  cep_batch *b = t->cells;
+  const size_t P = b->pitch;
+  std::vector<double> prm(P * b->np, 0.0);
+  std::vector<char> paced(t->nn, 0);
+  for (int k : t->stim_nodes) paced[k] = 1;
+  double p[64];
+  for (int i = 0; i < t->nn; ++i) {
+    cep::host::ord_params(b->ord_in[i], paced[i] ? t->stim_scale : 0.0, p);
+    p[cep::ord::OP_BCL] = t->stim_bcl;
+    p[cep::ord::OP_STIM_START] = t->stim_start;
+    p[cep::ord::OP_STIM_DUR] = t->stim_dur;
+    for (int k = 0; k < b->np; ++k) prm[k * P + i] = p[k];
+  }
+  TCHK(t, hipMemcpyAsync(b->d_prm, prm.data(), prm.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  TCHK(t, hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+extern "C" {
+
+cep_tissue *cep_tissue_create(int nx, int ny, double hx, double hy, int precision)
+{
+  if (nx < 1 || ny < 1 || !(hx > 0) || !(hy > 0)) return nullptr;
+  const long long nn = (long long)(nx + 1) * (ny + 1);
+  if (nn > 50000000LL) return nullptr;
+  cep_tissue *t = new cep_tissue();
+  t->nx = nx; t->ny = ny; t->nxn = nx + 1; t->nyn = ny + 1; t->nn = (int)nn; t->hx = hx; t->hy = hy;
+  t->cells = batch_create(CEP_MODEL_ORD_CIPA, t->nn, precision, false);   // no BDF workspace
+  if (!t->cells) { delete t; return nullptr; }
+  t->sxx.assign((size_t)nx * ny, 0.0);
+  t->sxy.assign((size_t)nx * ny, 0.0);
+  t->syy.assign((size_t)nx * ny, 0.0);
+  const int nblk = ((t->nxn + cep::tis::TX - 1) / cep::tis::TX) * ((t->nyn + cep::tis::TY - 1) / cep::tis::TY);
+  int rc = 0;
+  rc |= talloc(t, &t->d_A, (size_t)9 * t->nn);
+  rc |= talloc(t, &t->d_B, (size_t)9 * t->nn);
+  rc |= talloc(t, &t->d_dinv, t->nn);
+  rc |= talloc(t, &t->d_b, t->nn);
+  rc |= talloc(t, &t->d_r, t->nn);
+  rc |= talloc(t, &t->d_z, t->nn);
+  rc |= talloc(t, &t->d_p, t->nn);
+  rc |= talloc(t, &t->d_q, t->nn);
+  rc |= talloc(t, &t->d_part, (size_t)cep::tis::P_N * nblk);
+  rc |= talloc(t, &t->d_S, 1);
+  t->ev.resize(6);
+  for (auto &e : t->ev) rc |= hipEventCreate(&e) != hipSuccess;
+  if (rc || hipStreamSynchronize(t->cells->stream) != hipSuccess) {
+    cep_tissue_destroy(t);
+    return nullptr;
+  }
+  return t;
+}
+
+void cep_tissue_destroy(cep_tissue *t)
+{
+  if (!t) return;
+  if (t->cells) (void)hipStreamSynchronize(t->cells->stream);
+  for (void *p : t->allocs) (void)hipFree(p);
+  for (auto &e : t->ev) if (e) (void)hipEventDestroy(e);
+  if (t->cells) cep_batch_destroy(t->cells);
+  delete t;
+}
+
+const char *cep_tissue_last_error(const cep_tissue *t)
+{
+  if (!t) return "null tissue";
+  if (!t->err.empty()) return t->err.c_str();
+  return t->cells ? cep_last_error(t->cells) : "";
+}
+
+int cep_tissue_set_cells(cep_tissue *t, const double *celltype, const double *conc, const double *herg,
+                         int herg_stride, const double *gscale, int gscale_stride)
+{
+  if (!t) return CEP_E_ARG;
+  if (int rc = cep_set_cells(t->cells, 0, t->nn, celltype, conc, herg, herg_stride, gscale, gscale_stride, nullptr))
+    return tfail(t, rc, cep_last_error(t->cells));
+  // initial states of every node (initConsts), D = the node's concentration
+  cep_batch *b = t->cells;
+  const size_t P = b->pitch;
+  std::vector<double> y(P * b->ns, 0.0);
+  for (int i = 0; i < t->nn; ++i) {
+    for (int k = 0; k < 49; ++k) y[k * P + i] = cep::host::kOrdInitStates[k];
+    y[43 * P + i] = b->ord_in[i].conc;
+  }
+  TCHK(t, hipMemcpyAsync(b->d_y, y.data(), y.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  TCHK(t, hipStreamSynchronize(b->stream));
+  t->t = 0.0;
+  return tissue_upload_params(t);
+}
+
+int cep_tissue_set_conductivity(cep_tissue *t, const double *sxx, const double *sxy, const double *syy)
+{
+  if (!t || !sxx || !syy) return tfail(t, CEP_E_ARG, "conductivity arrays");
+  const size_t ne = (size_t)t->nx * t->ny;
+  for (size_t e = 0; e < ne; ++e) {
+    t->sxx[e] = sxx[e];
+    t->sxy[e] = sxy ? sxy[e] : 0.0;
+    t->syy[e] = syy[e];
+    if (!(sxx[e] >= 0) || !(syy[e] >= 0) || t->sxy[e] * t->sxy[e] > sxx[e] * syy[e])
+      return tfail(t, CEP_E_ARG, "diffusion tensor not positive semi-definite");
+  }
+  t->dt_built = -1;
+  return 0;
+}
+
+int cep_tissue_set_stimulus(cep_tissue *t, const int *nodes, int count, double amp_scale, double bcl,
+                            double start, double duration)
+{
+  if (!t || count < 0 || (count && !nodes) || !(bcl > 0) || !(duration >= 0)) return tfail(t, CEP_E_ARG, "stimulus");
+  t->stim_nodes.clear();
+  for (int k = 0; k < count; ++k) {
+    if (nodes[k] < 0 || nodes[k] >= t->nn) return tfail(t, CEP_E_ARG, "stimulus node out of range");
+    t->stim_nodes.push_back(nodes[k]);
+  }
+  t->stim_scale = amp_scale; t->stim_bcl = bcl; t->stim_start = start; t->stim_dur = duration;
+  return tissue_upload_params(t);
+}
+
+int cep_tissue_step(cep_tissue *t, int nsteps, double dt, int split, double cg_rtol, int cg_maxit,
+                    long long *cg_iters)
+{
+  if (!t || nsteps < 0 || !(dt > 0) || (split != CEP_SPLIT_GODUNOV && split != CEP_SPLIT_STRANG) ||
+      !(cg_rtol > 0) || cg_maxit < 1)
+    return tfail(t, CEP_E_ARG, "tissue step arguments");
+  long long total = 0;
+  double *vm = t->cells->d_y;                  // state plane 0
+  for (int s = 0; s < nsteps; ++s) {
+    const double t0 = t->t;
+    if (split == CEP_SPLIT_GODUNOV) {
+      if (int rc = tissue_react_launch(t, dt)) return rc;
+    } else {
+      if (int rc = tissue_react_launch(t, 0.5 * dt)) return rc;
+    }
+    int it = 0;
+    if (int rc = tissue_solve(t, dt, vm, cg_rtol, cg_maxit, &it)) return rc;
+    tissue_timed_collect(t, 0);
+    total += it;
+    if (split == CEP_SPLIT_STRANG) {
+      t->t = t0 + 0.5 * dt;
+      if (int rc = tissue_react_launch(t, 0.5 * dt)) return rc;
+      tissue_timed_collect(t, 0);
+    }
+    t->t = t0 + dt;
+  }
+  TCHK(t, hipStreamSynchronize(t->cells->stream));
+  if (cg_iters) *cg_iters = total;
+  return 0;
+}
+
+int cep_tissue_diffuse(cep_tissue *t, double dt, const double *v, double *x, double cg_rtol, int cg_maxit, int *iters)
+{
+  if (!t || !v || !x || !(dt > 0) || !(cg_rtol > 0) || cg_maxit < 1) return tfail(t, CEP_E_ARG, "diffuse arguments");
+  hipStream_t s = t->cells->stream;
+  double *xv = nullptr;
+  TCHK(t, hipMalloc(&xv, (size_t)t->nn * sizeof(double)));
+  TCHK(t, hipMemcpyAsync(xv, v, (size_t)t->nn * sizeof(double), hipMemcpyHostToDevice, s));
+  int rc = tissue_solve(t, dt, xv, cg_rtol, cg_maxit, iters);
+  tissue_timed_collect(t, 1);
+  if (!rc) {
+    if (hipMemcpyAsync(x, xv, (size_t)t->nn * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      rc = tfail(t, CEP_E_HIP, "diffuse copy back");
+  }
+  (void)hipFree(xv);
+  return rc;
+}
+
+int cep_tissue_get_vm(const cep_tissue *t, double *out)
+{
+  if (!t || !out) return CEP_E_ARG;
+  hipStream_t s = t->cells->stream;
+  if (hipMemcpyAsync(out, t->cells->d_y, (size_t)t->nn * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return CEP_E_HIP;
+  return 0;
+}
+
+int cep_tissue_get_states(const cep_tissue *t, double *out)
+{
+  if (!t) return CEP_E_ARG;
+  return cep_get_states(t->cells, out);
+}
+
+int cep_tissue_set_states(cep_tissue *t, const double *in)
+{
+  if (!t || !in) return CEP_E_ARG;
+  cep_batch *b = t->cells;
+  const size_t P = b->pitch;
+  std::vector<double> y(P * b->ns, 0.0);
+  for (int i = 0; i < t->nn; ++i)
+    for (int k = 0; k < b->ns; ++k) y[k * P + i] = in[(size_t)i * b->ns + k];
+  TCHK(t, hipMemcpyAsync(b->d_y, y.data(), y.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  TCHK(t, hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+double cep_tissue_time(const cep_tissue *t) { return t ? t->t : 0.0; }
+
+int cep_tissue_kernel_times(cep_tissue *t, double *avg_ms, long long *n)
+{
+  if (!t) return CEP_E_ARG;
+  for (int k = 0; k < 3; ++k) {
+    if (avg_ms) avg_ms[k] = t->tcount[k] ? t->tsum[k] / t->tcount[k] : 0.0;
+    if (n) n[k] = t->tcount[k];
+    t->tsum[k] = 0.0;
+    t->tcount[k] = 0;
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -12,8 +12,12 @@ workload = importlib.import_module("2ddrugsimulation_amd.workload")
 
 
 def _declared_symbols():
-    text = open(os.path.join(ROOT, "include", "cep_cellsim.h")).read()
-    return sorted(set(re.findall(r"\b(cep_[a-z_]+)\s*\(", text)))
+    names = set()
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if h.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", h)).read()
+            names |= set(re.findall(r"\b(cep_[a-z_0-9]+)\s*\(", text))
+    return sorted(names)
 
 
 def test_library_exports_every_declared_symbol():

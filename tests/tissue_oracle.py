@@ -1,0 +1,66 @@
+"""Test helpers: the tissue diffusion operator assembled on the host from the oracle's Q1
+element matrices (oracle/fem.c, restating Cfem::set_amat_bmat, fem.cpp:628-661) and solved
+with the oracle's Jacobi-PCG (cg_solve_csr).  Test infrastructure only."""
+import ctypes as C
+
+import numpy as np
+
+import oracle_lib as O
+
+# element node order (0,0), (hx,0), (hx,hy), (0,hy)
+OX = (0, 1, 1, 0)
+OY = (0, 0, 1, 1)
+
+
+def element(hx, hy, sxx, sxy, syy):
+    Me = np.zeros(16)
+    Ke = np.zeros(16)
+    O.lib().fem_q1_element(hx, hy, sxx, sxy, syy, O.ptr(Me), O.ptr(Ke))
+    return Me.reshape(4, 4), Ke.reshape(4, 4)
+
+
+def assemble(nx, ny, hx, hy, sxx, sxy, syy, dt):
+    """Dense A = M/dt + K and B = M/dt of the (nx+1)x(ny+1)-node grid (node iy*(nx+1)+ix)."""
+    nn = (nx + 1) * (ny + 1)
+    A = np.zeros((nn, nn))
+    Bm = np.zeros((nn, nn))
+    for ey in range(ny):
+        for ex in range(nx):
+            e = ey * nx + ex
+            Me, Ke = element(hx, hy, sxx[e], sxy[e], syy[e])
+            idx = [(ey + OY[j]) * (nx + 1) + ex + OX[j] for j in range(4)]
+            for j in range(4):
+                for k in range(4):
+                    A[idx[j], idx[k]] += Me[j, k] / dt + Ke[j, k]
+                    Bm[idx[j], idx[k]] += Me[j, k] / dt
+    return A, Bm
+
+
+def to_csr(A):
+    rows, cols = np.nonzero(A)
+    order = np.lexsort((cols, rows))
+    rows, cols = rows[order], cols[order]
+    rowptr = np.zeros(A.shape[0] + 1, dtype=np.int32)
+    np.add.at(rowptr, rows + 1, 1)
+    rowptr = np.cumsum(rowptr).astype(np.int32)
+    return rowptr, cols.astype(np.int32), np.ascontiguousarray(A[rows, cols])
+
+
+def cg(A, b, rtol, maxit=1000):
+    rowptr, col, val = to_csr(A)
+    x = np.zeros(len(b))
+    bb = np.ascontiguousarray(b, dtype=np.float64)
+    it = O.lib().cg_solve_csr(len(b), rowptr.ctypes.data_as(C.POINTER(C.c_int)),
+                              col.ctypes.data_as(C.POINTER(C.c_int)), O.ptr(val), O.ptr(bb), O.ptr(x),
+                              rtol, maxit, 1)
+    return x, it
+
+
+def random_tensors(rng, ne, dl=0.12, dt_=0.03):
+    """Fibre-rotated tensors D = R diag(dl, dt) R^T with random angles (positive definite)."""
+    th = rng.uniform(0, np.pi, ne)
+    c, s = np.cos(th), np.sin(th)
+    sxx = dl * c * c + dt_ * s * s
+    syy = dl * s * s + dt_ * c * c
+    sxy = (dl - dt_) * c * s
+    return sxx, sxy, syy
