@@ -10,6 +10,10 @@ one (FP32 gating/currents, FP64 state/Newton/LU); an FP64 run of a subsample, as
 job, checks APD90 and qNet and gives the plain per-job FP64 rate.  A short serving session
 (Poisson arrivals of mixed ORd-CiPA / TT06 jobs) gives the p50/p99 latency.
 
+`--workload tissue` instead times the 2-D monodomain step (512 x 512 Q1 elements, ORd-CiPA
+nodes, Strang splitting, Jacobi-PCG diffusion) in node-steps/s; a "step" is one time step of
+the whole tissue.
+
 Prints one JSON line.  --gpus N > 1 is not supported (single-GPU build).
 """
 from __future__ import annotations
@@ -71,16 +75,16 @@ def flops_of(stats: np.ndarray) -> float:
     return float(np.sum((nfe + nout) * F_RHS + nsetups * F_SETUP + nni * F_NEWTON + nst * F_STEP))
 
 
-def read_traffic():
-    """HBM bytes per launch of the BDF kernel from the committed rocprofv3 PMC summary
-    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), or None."""
+def read_traffic(path=TRAFFIC_FILE, keys=("hbm_bytes_per_beat_launch",)):
+    """HBM bytes per launch of a kernel from a committed rocprofv3 PMC summary (FETCH_SIZE +
+    WRITE_SIZE, scaled by the copy-kernel calibration): the sum of the named entries, or None."""
     try:
         kv = {}
-        for line in open(TRAFFIC_FILE):
+        for line in open(path):
             if "=" in line and not line.startswith("#"):
                 k, v = line.split("=", 1)
-                kv[k.strip()] = v.strip()
-        return float(kv["hbm_bytes_per_beat_launch"])
+                kv[k.strip()] = v.split("#")[0].strip()
+        return float(sum(float(kv[k]) for k in keys))
     except (OSError, KeyError, ValueError):
         return None
 
@@ -122,13 +126,132 @@ def serving_latency(njobs: int, rate: float):
                         "ORd-CiPA or TT06 (p=0.5), mixed precision"}
 
 
+# Tissue (monodomain) workload: 512 x 512 Q1 elements, 0.1 mm, one seeded drug at 2 x Cmax.
+TISSUE_NX, TISSUE_H, TISSUE_DT = 512, 0.1, 0.01
+# algorithmic HBM bytes per node and CG iteration (FP64): SpMV 9 coefficients + p + q = 88,
+# update x r z read+write with p q dinv read = 64, direction z p read + p write = 24
+B_SPMV, B_UPDATE, B_DIR = 88, 64, 24
+PEAK_HBM = 8.0e12
+TISSUE_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "tissue_monodomain_513x513_traffic.txt")
+
+
+def tissue_setup(nx: int, precision: int):
+    """The 2-D tissue of the monodomain workload: fibre angle linear 0 -> 60 deg across x,
+    sigma_l / sigma_t = 0.12 / 0.03 S/m as D = sigma / (chi Cm) with chi = 1400 /cm and
+    Cm = 1 uF/cm^2 (0.0857 / 0.0214 mm^2/ms), ORd-CiPA endo cells with one seeded drug at
+    2 x Cmax, 5 x 5 paced corner nodes."""
+    rng = np.random.default_rng(20251207)
+    herg, cmax = cep.workload.draw_drug(rng)
+    tis = cep.Tissue(nx, nx, TISSUE_H, TISSUE_H, precision)
+    tis.set_cells(celltype=0.0, conc=2.0 * cmax, herg=herg)
+    dl, dtr = 0.12 / 1.4, 0.03 / 1.4
+    th = np.deg2rad(60.0) * np.tile((np.arange(nx) + 0.5) / nx, nx)
+    c, s = np.cos(th), np.sin(th)
+    sxx, sxy, syy = dl * c * c + dtr * s * s, (dl - dtr) * c * s, dl * s * s + dtr * c * c
+    tis.set_conductivity(sxx, sxy, syy)
+    tis.set_stimulus([tis.node(i, j) for i in range(5) for j in range(5)], amp_scale=2.0, bcl=1000.0,
+                     start=0.0, duration=1.0)
+    return tis, (sxx, sxy, syy), (herg, 2.0 * cmax)
+
+
+def tissue_cpu_baseline(tis, tensors, drug, nx: int):
+    """The FP64 oracle on the host, one thread: one diffusion solve of the current state with
+    the oracle's Jacobi-PCG on the assembled CSR system, and the oracle's right-hand side on a
+    sample of the nodes' states (Strang: two reaction half-steps per step).  The reaction's
+    state updates are not timed, so this slightly favours the host."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    import tissue_oracle as T
+    t0 = time.perf_counter()
+    Acsr, Bcsr = T.assemble_csr(nx, nx, TISSUE_H, TISSUE_H, *tensors, TISSUE_DT)
+    v = tis.vm()
+    rp, col, val = Bcsr
+    b = np.add.reduceat(val * v[col], rp[:-1])
+    t1 = time.perf_counter()
+    _, it = T.cg_csr(Acsr, b, 1e-5)
+    t_cg = time.perf_counter() - t1
+    st = tis.states()
+    nn = st.shape[0]
+    ns = min(nn, 20000)
+    sel = np.linspace(0, nn - 1, ns).astype(int)
+    Cc, _ = O.ord_init(0.0, drug[1], drug[0])
+    Crow = np.ascontiguousarray(np.broadcast_to(Cc, (ns, len(Cc))))
+    S = np.ascontiguousarray(st[sel])
+    R = np.zeros_like(S)
+    t_rhs = O.lib().oracle_ord_rates_many(ns, O.ptr(Crow), O.ptr(S), O.ptr(R), tis.time) / ns
+    per_step = t_cg + 2.0 * nn * t_rhs
+    return {"value": nn / per_step, "unit": "node-steps/s", "cores": 1, "kind": "port",
+            "sample": f"one oracle Jacobi-PCG solve of the {nn}-node system ({it} iterations, "
+                      f"{t_cg:.2f} s) + oracle RHS on {ns} node states x 2 half-steps, scaled to "
+                      f"{nn} nodes; assembly {t1 - t0:.1f} s not counted"}
+
+
+def tissue_main(args):
+    K = args.steps if args.steps is not None else 2000
+    W = args.warmup if args.warmup is not None else 200
+    nx = args.tissue_nx
+    prec = cep.PREC_MIXED if args.precision == "mixed" else cep.PREC_FP64
+    tis, tensors, drug = tissue_setup(nx, prec)
+    nn = tis.nn
+    log(f"bench: tissue {nx}x{nx} elements ({nn} nodes), warmup {W} steps")
+    if W:
+        tis.step(W, TISSUE_DT, cep.SPLIT_STRANG)
+    tis.kernel_times()
+    log(f"bench: timing {K} steps")
+    t0 = time.perf_counter()
+    iters = tis.step(K, TISSUE_DT, cep.SPLIT_STRANG)
+    t1 = time.perf_counter()
+    avg, n = tis.kernel_times()
+    vm = tis.vm()
+    elapsed = t1 - t0
+    ipi = iters / K
+    spmv_bytes = nn * B_SPMV
+    ach = spmv_bytes / (avg[1] * 1e-3) if avg[1] > 0 else 0.0
+    cpu = None if args.no_cpu_baseline else tissue_cpu_baseline(tis, tensors, drug, nx)
+    tis.close()
+    out = {
+        "metric": "tissue node-steps/s (2-D monodomain, ORd-CiPA nodes, Q1 FEM + Jacobi-PCG)",
+        "value": nn * K / elapsed,
+        "unit": "node-steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": 1e3 * elapsed / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == cep.PREC_FP64 else "f32-rates/f64-state",
+        "data": "synthetic (seeded drug, fibre-rotated conductivity)",
+        "config": {"workload": f"{nx}x{nx} Q1 elements ({nn} nodes), dx 0.1 mm, sigma 0.12/0.03 S/m, "
+                               "fibre angle 0->60 deg across x, 1 drug at 2xCmax, 5x5 corner stimulus, "
+                               "dt 0.01 ms, Strang splitting, CG rtol 1e-5",
+                   "nodes": nn},
+        "cg_iters_per_step": ipi,
+        "vm_range_mv": [float(np.min(vm)), float(np.max(vm))],
+        "kernels_avg_ms": {"tissue_react": float(avg[0]), "cg_spmv": float(avg[1]), "cg_update": float(avg[2]),
+                           "launches": [int(x) for x in n]},
+        "roofline": {"bound": "hbm", "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                     "frac": ach / PEAK_HBM,
+                     "traffic": read_traffic(TISSUE_TRAFFIC_FILE, ("cg_spmv.hbm_read_bytes", "cg_spmv.hbm_write_bytes"))
+                     if nx == TISSUE_NX else None,
+                     "kernel": "cg_spmv",
+                     "bytes_per_launch": spmv_bytes,
+                     "update_gbs": nn * B_UPDATE / (avg[2] * 1e-3) / 1e9 if avg[2] > 0 else None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=("population", "tissue"), default="population")
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--tissue-nx", type=int, default=TISSUE_NX)
     ap.add_argument("--ncell", type=int, default=10000)
-    ap.add_argument("--precision", choices=("mixed", "fp64"), default="mixed")
+    ap.add_argument("--precision", choices=("mixed", "fp64"), default=None,
+                    help="population default: mixed; tissue default: fp64")
     ap.add_argument("--check-cells", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -139,8 +262,14 @@ def main():
         print(json.dumps({"error": "single-GPU build: run with --gpus 1 (cells shard trivially "
                           "across GPUs; see the design notes)"}))
         sys.exit(2)
+    if args.workload == "tissue":
+        args.precision = args.precision or "fp64"
+        tissue_main(args)
+        return
+    args.precision = args.precision or "mixed"
     prec = cep.PREC_MIXED if args.precision == "mixed" else cep.PREC_FP64
-    K, W = args.steps, args.warmup
+    K = args.steps if args.steps is not None else 2
+    W = args.warmup if args.warmup is not None else 1
     pop = make_population(args.ncell)
     n = len(pop["bcl"])
     nb_total = 200

@@ -162,3 +162,16 @@ double oracle_tt_run_many(const tt_cell_cfg_t *cfgs, int ncell, int nthreads, bm
 {
   return run_many(1, cfgs, ncell, nthreads, res, states_out);
 }
+
+/* Host timing of the reaction's right-hand sides: ord_rates for n cells (rows of C and S,
+   row strides ORD_NC and ORD_NS), one thread; returns the wall seconds. */
+double oracle_ord_rates_many(int n, const double *C, const double *S, double *R, double t)
+{
+  double A[ORD_NA];
+  struct timespec a, b;
+  clock_gettime(CLOCK_MONOTONIC, &a);
+  for (int i = 0; i < n; ++i)
+    ord_rates(t, C + (size_t)i * ORD_NC, R + (size_t)i * ORD_NS, S + (size_t)i * ORD_NS, A);
+  clock_gettime(CLOCK_MONOTONIC, &b);
+  return (b.tv_sec - a.tv_sec) + 1e-9 * (b.tv_nsec - a.tv_nsec);
+}

@@ -122,6 +122,8 @@ double oracle_ord_run_many(const ord_cell_cfg_t *cfgs, int ncell, int nthreads,
                            bm_result_t *res, double *states_out);
 double oracle_tt_run_many(const tt_cell_cfg_t *cfgs, int ncell, int nthreads,
                           bm_result_t *res, double *states_out);
+/* ord_rates over n cells (rows of C and S), one thread; returns wall seconds */
+double oracle_ord_rates_many(int n, const double *C, const double *S, double *R, double t);
 
 /* ---------------- tissue (fem.cpp / heart.cpp) */
 void fem_q1_element(double hx, double hy, double sxx, double sxy, double syy,

@@ -76,6 +76,8 @@ def lib():
         L.oracle_tt_run_many.argtypes = [C.POINTER(TtCellCfg), C.c_int, C.c_int,
                                          C.POINTER(BmResult), dp]
         L.oracle_tt_run_many.restype = C.c_double
+        L.oracle_ord_rates_many.argtypes = [C.c_int, dp, dp, dp, C.c_double]
+        L.oracle_ord_rates_many.restype = C.c_double
         L.bdf_create.argtypes = [C.c_int, BDF_RHS, C.c_void_p, C.POINTER(BdfOpts)]
         L.bdf_create.restype = C.c_void_p
         L.bdf_init.argtypes = [C.c_void_p, C.c_double, dp]

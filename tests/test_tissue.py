@@ -49,6 +49,20 @@ def test_oracle_cg_matches_dense_solve():
     assert 1 <= it < A.shape[0]
 
 
+def test_sparse_assembly_matches_dense():
+    rng = np.random.default_rng(6)
+    nx, ny = 6, 4
+    sxx, sxy, syy = T.random_tensors(rng, nx * ny)
+    A, Bm = T.assemble(nx, ny, 0.1, 0.12, sxx, sxy, syy, 0.01)
+    Acsr, Bcsr = T.assemble_csr(nx, ny, 0.1, 0.12, sxx, sxy, syy, 0.01)
+    for dense, (rp, col, val) in ((A, Acsr), (Bm, Bcsr)):
+        rebuilt = np.zeros_like(dense)
+        for i in range(dense.shape[0]):
+            rebuilt[i, col[rp[i]:rp[i + 1]]] = val[rp[i]:rp[i + 1]]
+        np.testing.assert_allclose(rebuilt, dense, rtol=1e-13, atol=1e-15)
+        assert rp[-1] == np.count_nonzero(dense)
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("nx,ny", [(13, 9), (40, 33)])
 def test_tissue_diffusion_matches_oracle(nx, ny):

@@ -64,3 +64,40 @@ def random_tensors(rng, ne, dl=0.12, dt_=0.03):
     syy = dl * s * s + dt_ * c * c
     sxy = (dl - dt_) * c * s
     return sxx, sxy, syy
+
+
+def assemble_csr(nx, ny, hx, hy, sxx, sxy, syy, dt):
+    """A = M/dt + K and B = M/dt as CSR (rowptr, col, val) triples, for grids too large for
+    the dense assembly.  The element stiffness is linear in the tensor, so K_e is formed from
+    the oracle's element matrices of the three unit tensors."""
+    import scipy.sparse as sp
+    Me, Kxx = element(hx, hy, 1.0, 0.0, 0.0)
+    _, Kxy = element(hx, hy, 0.0, 1.0, 0.0)
+    _, Kyy = element(hx, hy, 0.0, 0.0, 1.0)
+    ne = nx * ny
+    ex = np.tile(np.arange(nx), ny)
+    ey = np.repeat(np.arange(ny), nx)
+    idx = np.stack([(ey + OY[j]) * (nx + 1) + ex + OX[j] for j in range(4)], axis=1)   # ne x 4
+    rows = np.repeat(idx, 4, axis=1).ravel()
+    cols = np.tile(idx, (1, 4)).ravel()
+    sxx, sxy, syy = (np.broadcast_to(np.asarray(v, np.float64), (ne,)) for v in (sxx, sxy, syy))
+    Ke = (sxx[:, None] * Kxx.ravel()[None, :] + sxy[:, None] * Kxy.ravel()[None, :]
+          + syy[:, None] * Kyy.ravel()[None, :])
+    Mv = np.broadcast_to(Me.ravel()[None, :] / dt, (ne, 16))
+    nn = (nx + 1) * (ny + 1)
+    out = []
+    for vals in (Mv + Ke, Mv):
+        m = sp.coo_matrix((np.ascontiguousarray(vals).ravel(), (rows, cols)), shape=(nn, nn)).tocsr()
+        m.sort_indices()
+        out.append((m.indptr.astype(np.int32), m.indices.astype(np.int32), np.ascontiguousarray(m.data)))
+    return out[0], out[1]
+
+
+def cg_csr(csr, b, rtol, maxit=1000):
+    rowptr, col, val = csr
+    x = np.zeros(len(b))
+    bb = np.ascontiguousarray(b, dtype=np.float64)
+    it = O.lib().cg_solve_csr(len(b), rowptr.ctypes.data_as(C.POINTER(C.c_int)),
+                              col.ctypes.data_as(C.POINTER(C.c_int)), O.ptr(val), O.ptr(bb), O.ptr(x),
+                              rtol, maxit, 1)
+    return x, it
