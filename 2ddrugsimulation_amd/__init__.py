@@ -19,6 +19,9 @@ LIB_PATH = os.path.join(PKG_DIR, "libcepgpu.so")
 
 MODEL_ORD_CIPA = 1
 MODEL_TT06 = 2
+MODEL_ORD2011 = 3
+OPT_DUTTA = 1
+OPT_TIME_POINT = 2
 PREC_FP64 = 0
 PREC_MIXED = 1
 
@@ -26,10 +29,15 @@ BM_FIELDS = ("qnet", "inal_auc", "ical_auc", "dvmdt_repol", "dvmdt_max", "vm_pea
              "vm_dia", "apd90", "apd50", "ca_peak", "ca_valley", "ca_dia", "cad90", "cad50",
              "steepest_pace")
 NBM = len(BM_FIELDS)
+# ORd 2011 (DrugSimulation driver) record, same 16-slot width
+BM11_FIELDS = ("qnet_ap", "qnet4_ap", "inal_auc_ap", "ical_auc_ap", "qnet_cl", "qnet4_cl",
+               "inal_auc_cl", "ical_auc_cl", "vm_peak", "vm_valley", "dvmdt_repol", "t_peak",
+               "vm_repol30", "vm_repol50", "vm_repol90", "steepest_pace")
 STATUS_NAMES = {0: "ok", 1: "conv_fail", 2: "err_fail", 3: "nonfinite", 4: "max_steps"}
 
 EXPORTED = (
     "cep_batch_create", "cep_batch_destroy", "cep_last_error", "cep_batch_info", "cep_set_cells",
+    "cep_set_model_option",
     "cep_set_run_config", "cep_init_consts", "cep_get_constants", "cep_compute_rates",
     "cep_run_beats", "cep_run_beats_async", "cep_sync", "cep_get_states", "cep_set_states",
     "cep_get_biomarkers", "cep_get_status", "cep_get_stats", "cep_enable_vm_trace",
@@ -71,6 +79,7 @@ def library(path: str | None = None):
     L.cep_set_run_config.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int,
                                      C.c_int, C.c_int, C.c_double]
     L.cep_init_consts.argtypes = [C.c_void_p]
+    L.cep_set_model_option.argtypes = [C.c_void_p, C.c_int, C.c_double]
     L.cep_get_constants.argtypes = [C.c_void_p, C.c_int, dp]
     L.cep_compute_rates.argtypes = [C.c_void_p, C.c_double, dp, dp, dp]
     L.cep_run_beats.argtypes = [C.c_void_p, C.c_int]
@@ -176,8 +185,12 @@ class CellBatch:
     def set_run_config(self, dt=None, rtol=1e-6, atol=1e-8, maxord=5, nbeats_total=1, nwin=250,
                        stim_amp_scale=1.0):
         if dt is None:
-            dt = 0.1 if self.model == MODEL_ORD_CIPA else 0.02
+            dt = {MODEL_ORD_CIPA: 0.1, MODEL_TT06: 0.02, MODEL_ORD2011: 1.0}[self.model]
         self._chk(self.L.cep_set_run_config(self.h, dt, rtol, atol, maxord, nbeats_total, nwin, stim_amp_scale))
+
+    def set_model_option(self, option: int, value: float):
+        """ORd 2011: OPT_DUTTA (the reference's Is_Dutta), OPT_TIME_POINT (fixed-step window)."""
+        self._chk(self.L.cep_set_model_option(self.h, int(option), float(value)))
 
     def init_consts(self):
         self._chk(self.L.cep_init_consts(self.h))
@@ -234,7 +247,8 @@ class CellBatch:
     def biomarkers(self) -> dict:
         out = np.zeros((self.n, NBM))
         self._chk(self.L.cep_get_biomarkers(self.h, _p(out)))
-        return {k: out[:, i].copy() for i, k in enumerate(BM_FIELDS)}
+        names = BM11_FIELDS if self.model == MODEL_ORD2011 else BM_FIELDS
+        return {k: out[:, i].copy() for i, k in enumerate(names)}
 
     def biomarker_array(self) -> np.ndarray:
         out = np.zeros((self.n, NBM))

@@ -10,6 +10,7 @@
 #include "../../include/cep_cellsim.h"
 #include "../../include/cep_tissue.h"
 #include "bdf_ord.hpp"
+#include "ord11_model.hpp"
 #include "tt06_model.hpp"
 
 namespace cep {
@@ -550,6 +551,175 @@ __global__ void __launch_bounds__(256) tt_rl_kernel(TtBuffers B, RunCfg rc, int 
   bm.store(B.bmd, B.bmi, B.pitch, c);
 }
 
+// ---- ORd 2011: the DrugSimulation loop (drug_sim.cpp:306-420) -- adaptive Rush-Larsen steps
+// clipped to beat ends, per-beat features over the trailing nwin beats -- one cell per lane.
+// Persistent per-cell scalars (biomarker doubles area): [0, O11_SD_N) the loop's running
+// values, [O11_BEST, O11_BEST + 16) the selected beat's record; ints: pace, eligible AP,
+// steepest pace, initialised flag.
+enum : int {
+  O11_T = 0, O11_R30, O11_R50, O11_R90, O11_TPEAK, O11_QAP, O11_Q4AP, O11_INALAP, O11_ICALAP,
+  O11_QCL, O11_Q4CL, O11_INALCL, O11_ICALCL, O11_TMP_DVDT, O11_TMP_PEAK, O11_TMP_VALLEY, O11_SD_N,
+  O11_BEST = O11_SD_N
+};
+enum : int { O11_I_PACE = 0, O11_I_ELIG, O11_I_STEEP, O11_I_INIT };
+static_assert(O11_BEST + CEP_NBM <= BM_TOTAL_D, "ORd 2011 loop state fits the biomarker area");
+static_assert(O11_I_INIT < BM_I_N, "ORd 2011 loop ints fit the biomarker area");
+
+struct O11Buffers {
+  int ncell, pitch, nbeats_total, nwin;
+  double *y, *prm, *bmd;
+  int *bmi, *status;
+  long long *stats;
+  double time_point, max_dt;
+};
+
+template <typename R>
+__global__ void __launch_bounds__(64) ord11_rl_kernel(O11Buffers B, int beats_end, long long max_steps)
+{
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= B.ncell) return;
+  if (B.status[c] != ST_OK) return;
+  const size_t P_ = (size_t)B.pitch;
+  const PrmAcc P{B.prm + c, B.pitch};
+  double y[ord11::NS];
+#pragma unroll
+  for (int i = 0; i < ord11::NS; ++i) y[i] = B.y[i * P_ + c];
+  double *sd = B.bmd + c;
+  int *si = B.bmi + c;
+  double st[O11_SD_N];
+#pragma unroll
+  for (int k = 0; k < O11_SD_N; ++k) st[k] = sd[k * P_];
+  int pace = si[O11_I_PACE * P_], elig = si[O11_I_ELIG * P_], steep = si[O11_I_STEEP * P_];
+  double best_dvdt = sd[(O11_BEST + CEP_BM11_DVMDT_REPOL) * P_];
+  if (si[O11_I_INIT * P_] == 0) {
+    // cipa_result.init / temp_result.init with the starting Vm (drug_sim.cpp:166-167)
+    st[O11_TMP_DVDT] = -999.0; st[O11_TMP_PEAK] = -999.0; st[O11_TMP_VALLEY] = y[0];
+    for (int k = 0; k < CEP_NBM; ++k) sd[(O11_BEST + k) * P_] = 0.0;
+    sd[(O11_BEST + CEP_BM11_DVMDT_REPOL) * P_] = -999.0;
+    sd[(O11_BEST + CEP_BM11_VM_PEAK) * P_] = -999.0;
+    sd[(O11_BEST + CEP_BM11_VM_VALLEY) * P_] = y[0];
+    best_dvdt = -999.0;
+    si[O11_I_INIT * P_] = 1;
+  }
+  const double bcl = P(ord11::P_BCL);
+  const double tmax = beats_end * bcl;
+  const int first_win = B.nbeats_total - B.nwin;
+  double t = st[O11_T];
+  long long n = 0;
+  while (t < tmax && n < max_steps) {
+    double f[ord11::NS];
+    ord11::Gates g;
+    ord11::Currents cur;
+    ord11::rhs<R>(t, y, f, P, cur, g);
+    const double dts = ord11::time_step(t, B.time_point, B.max_dt, bcl, f[0]);
+    double dt;
+    if (floor((t + dts) / bcl) == floor(t / bcl)) {
+      dt = dts;
+    } else {
+      dt = (floor(t / bcl) + 1) * bcl - t;
+      if (elig && pace >= first_win) {
+        if (st[O11_TMP_DVDT] > best_dvdt) {
+          best_dvdt = st[O11_TMP_DVDT];
+          steep = pace;
+          double *bs = sd + O11_BEST * P_;
+          bs[CEP_BM11_QNET_AP * P_] = st[O11_QAP];
+          bs[CEP_BM11_QNET4_AP * P_] = st[O11_Q4AP];
+          bs[CEP_BM11_INAL_AUC_AP * P_] = st[O11_INALAP];
+          bs[CEP_BM11_ICAL_AUC_AP * P_] = st[O11_ICALAP];
+          bs[CEP_BM11_QNET_CL * P_] = st[O11_QCL];
+          bs[CEP_BM11_QNET4_CL * P_] = st[O11_Q4CL];
+          bs[CEP_BM11_INAL_AUC_CL * P_] = st[O11_INALCL];
+          bs[CEP_BM11_ICAL_AUC_CL * P_] = st[O11_ICALCL];
+          bs[CEP_BM11_VM_PEAK * P_] = st[O11_TMP_PEAK];
+          bs[CEP_BM11_VM_VALLEY * P_] = st[O11_TMP_VALLEY];
+          bs[CEP_BM11_DVMDT_REPOL * P_] = st[O11_TMP_DVDT];
+          bs[CEP_BM11_T_PEAK * P_] = st[O11_TPEAK];
+          bs[CEP_BM11_VM_REPOL30 * P_] = st[O11_R30];
+          bs[CEP_BM11_VM_REPOL50 * P_] = st[O11_R50];
+          bs[CEP_BM11_VM_REPOL90 * P_] = st[O11_R90];
+        }
+      }
+#pragma unroll
+      for (int k = O11_QAP; k <= O11_ICALCL; ++k) st[k] = 0.0;
+      st[O11_TPEAK] = 0.0;
+      st[O11_TMP_DVDT] = -999.0; st[O11_TMP_PEAK] = -999.0; st[O11_TMP_VALLEY] = y[0];
+      pace++;
+      elig = 0;
+    }
+    ord11::rl_step<R>(dt, y, f, g);
+    ++n;
+    if (pace >= first_win) {
+      const double t_beat = bcl * pace + ord11::stim_start;
+      if (t > t_beat + 2 && t < t_beat + 10 && fabs(cur.INa) < 1) {
+        if (y[0] > st[O11_TMP_PEAK]) {
+          st[O11_TMP_PEAK] = y[0];
+          if (y[0] > 0) {
+            const double amp = st[O11_TMP_PEAK] - st[O11_TMP_VALLEY];
+            st[O11_R30] = st[O11_TMP_PEAK] - 0.3 * amp;
+            st[O11_R50] = st[O11_TMP_PEAK] - 0.5 * amp;
+            st[O11_R90] = st[O11_TMP_PEAK] - 0.9 * amp;
+            elig = 1;
+            st[O11_TPEAK] = t;
+          } else {
+            elig = 0;
+          }
+        }
+      } else if (t > t_beat + 10 && elig) {
+        if (f[0] > st[O11_TMP_DVDT] && y[0] <= st[O11_R30] && y[0] >= st[O11_R90]) st[O11_TMP_DVDT] = f[0];
+      }
+      const double inet = cur.INaL + cur.ICaL + cur.Ito + cur.IKr + cur.IKs + cur.IK1;
+      const double inet4 = cur.INaL + cur.ICaL + cur.IKr + cur.INa;
+      if (elig && y[0] > st[O11_R90]) {
+        st[O11_QAP] += (inet * dt) / 1000.;
+        st[O11_Q4AP] += (inet4 * dt) / 1000.;
+        st[O11_INALAP] += cur.INaL * dt;
+        st[O11_ICALAP] += cur.ICaL * dt;
+      }
+      st[O11_QCL] += (inet * dt) / 1000.;
+      st[O11_Q4CL] += (inet4 * dt) / 1000.;
+      st[O11_INALCL] += cur.INaL * dt;
+      st[O11_ICALCL] += cur.ICaL * dt;
+    }
+    t += dt;
+  }
+  st[O11_T] = t;
+  bool fin = true;
+#pragma unroll
+  for (int i = 0; i < ord11::NS; ++i) { B.y[i * P_ + c] = y[i]; fin = fin && isfinite(y[i]); }
+  if (!fin) B.status[c] = ST_NONFINITE;
+  else if (t < tmax) B.status[c] = ST_MAX_STEPS;
+#pragma unroll
+  for (int k = 0; k < O11_SD_N; ++k) sd[k * P_] = st[k];
+  si[O11_I_PACE * P_] = pace; si[O11_I_ELIG * P_] = elig; si[O11_I_STEEP * P_] = steep;
+  B.stats[c] += n;              // steps (one right-hand side each)
+}
+
+// computeRates of ORd 2011 for a batch (parity path): AoS states in, AoS rates and the
+// ALGEBRAIC current slots out
+constexpr int ORD11_NALG = 198;
+template <typename R>
+__global__ void __launch_bounds__(64) ord11_rates_kernel(int n, int pitch, double t, const double *__restrict__ yin,
+                                                        double *__restrict__ fout, double *__restrict__ alg,
+                                                        const double *__restrict__ prm)
+{
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  double y[ord11::NS], f[ord11::NS];
+#pragma unroll
+  for (int i = 0; i < ord11::NS; ++i) y[i] = yin[(size_t)c * ord11::NS + i];
+  const PrmAcc P{prm + c, pitch};
+  ord11::Gates g;
+  ord11::Currents cur;
+  ord11::rhs<R>(t, y, f, P, cur, g);
+#pragma unroll
+  for (int i = 0; i < ord11::NS; ++i) fout[(size_t)c * ord11::NS + i] = f[i];
+  if (alg) {
+    double *a = alg + (size_t)c * ORD11_NALG;
+    a[12] = cur.Istim; a[62] = cur.INa; a[64] = cur.INaL; a[70] = cur.Ito; a[81] = cur.ICaL;
+    a[96] = cur.IKr; a[98] = cur.IKs; a[100] = cur.IK1;
+  }
+}
+
 // ============================================================================ host side
 
 namespace host {
@@ -722,6 +892,68 @@ static void tt_constants(const TtCellIn &in, double stim_scale, double *C)
   C[6] = p[TP_BCL]; C[8] = p[TP_STIM_AMP];
 }
 
+// ---- ORd 2011 (DrugSimulation/cellmodels/Ohara_Rudy_2011.cpp)
+struct O11CellIn {
+  double celltype, conc, bcl;
+  double ic50[14];
+};
+
+// initial STATES of ___initConsts (:494-679)
+static const double kO11Init[41] = {
+  -87, 0, 1e-4, 7, 7, 145, 145, 1.2, 1.2, 1e-4, 0, 1, 1, 1, 1, 1, 0, 1, 1, 0, 1, 1, 0, 1, 1,
+  0, 1, 1, 1, 1, 1, 1, 1, 0, 0, 0, 0, 0, 1, 0, 0 };
+
+// per-cell parameters: endocardial constants, the Dutta scaling of the reference (slots 96,
+// 98, 100, 81 of CONSTANTS = GNaL, Aff, tjca, H), then the Hill block of ___applyDrugEffect
+static void o11_params(const O11CellIn &in, bool dutta, double stim_scale, double *p)
+{
+  using namespace ord11;
+  auto block = [&](int k) {
+    const double ic = in.ic50[2 * k], h = in.ic50[2 * k + 1];
+    return (ic > 10E-14 && h > 10E-14) ? 1. / (1. + pow(in.conc / ic, h)) : 1.;
+  };
+  p[P_PCA] = PCa_b * block(0);
+  p[P_GK1] = GK1_b * block(1);
+  p[P_GKS] = GKs_b * block(2);
+  p[P_GNA] = GNa * block(3);
+  p[P_GNAL] = (dutta ? GNaL_b * 1.013 : GNaL_b) * block(4);
+  p[P_GTO] = Gto_b * block(5);
+  p[P_GKR] = GKr_b * block(6);
+  p[P_AFF] = dutta ? 0.6 * 1.870 : 0.6;
+  p[P_TJCA] = dutta ? 75.0 * 1.698 : 75.0;
+  p[P_H] = dutta ? 1e-7 * 1.007 : 1e-7;
+  p[P_BCL] = in.bcl;
+  p[P_STIM_AMP] = stim_amp * stim_scale;
+}
+
+// CONSTANTS[144] in the reference layout (enum E_CONSTANTS_T)
+static void o11_constants(const O11CellIn &in, bool dutta, double stim_scale, double *C)
+{
+  memset(C, 0, sizeof(double) * ord11::NC);
+  static const double prim[93] = {
+    0, 140, 1.8, 5.4, 8314, 310, 96485, 1, 2, 1, 0.01, 0.0011, -80, 0.5, 0.15, 0.05, 0.00068, 0.05,
+    0.0015, 0.05, 0.00238, 0.07, 0.0005, 0.047, 0.00087, 1.124, 0.0087, 10, 0.8, 1, 0.01833, 39.57,
+    9.871, 11.64, 34.77, 6.765, 8.552, 77.42, 5.955, 82.9, 6.086, 0.99, 75, 200, 0.0075, 0.02, 0.002,
+    1000, 0.0001, 0.046, 0.0034, 0.1908, 15, 5, 88.12, 12.5, 6e4, 6e4, 5e3, 1.5e6, 5e3, 0.5224, 0.167,
+    150e-6, 0.0008, 949.5, 182.4, 687.2, 39.4, 1899, 79300, 639, 40, 9.073, 27.78, -0.155, 0.5, 0.3582,
+    0.05, 9.8, 1.698e-7, 1e-7, 4.2, 1.698e-7, 224, 292, 30, 0.003, 3.75e-10, 2.5e-8, 0.0005, 0.0005, 4.75 };
+  memcpy(C, prim, sizeof(prim));
+  using namespace ord;
+  double p[ord11::NP];
+  o11_params(in, dutta, stim_scale, p);
+  C[12] = p[ord11::P_STIM_AMP];
+  C[93] = 0.05; C[94] = 1.0 - 0.99; C[95] = 3.0 * 200; C[96] = p[ord11::P_GNAL]; C[97] = p[ord11::P_GTO];
+  C[98] = p[ord11::P_AFF]; C[99] = p[ord11::P_PCA]; C[100] = p[ord11::P_TJCA]; C[101] = p[ord11::P_GKR];
+  C[102] = p[ord11::P_GKS]; C[103] = p[ord11::P_GK1]; C[104] = vcell; C[105] = 0.003; C[106] = a_rel;
+  C[107] = btp; C[108] = 1.0; C[109] = 1.0 - 0.6; C[110] = ord11::PCap; C[111] = ord11::PCaNa;
+  C[112] = ord11::PCaK; C[113] = Ageo; C[114] = a_relp; C[115] = ord11::PCaNap; C[116] = ord11::PCaKp;
+  C[117] = Acap; C[118] = vmyo; C[119] = vnsr; C[120] = vjsr; C[121] = vss; C[122] = h10; C[123] = h11;
+  C[124] = h12; C[125] = k1ncx; C[126] = kcaoff; C[127] = kcaoff; C[128] = 0.0008; C[129] = h10;
+  C[130] = h11; C[131] = h12; C[132] = k1ncx; C[133] = kcaoff; C[134] = kcaoff; C[135] = b1nak;
+  C[136] = a2nak; C[137] = a4nak; C[138] = 30; C[139] = ord11::stim_start; C[140] = in.bcl;
+  if (dutta) { C[81] = p[ord11::P_H]; C[64] = 0.0008 * 2.661; }
+}
+
 template <typename T>
 struct DevBuf {
   T *p = nullptr;
@@ -750,6 +982,10 @@ struct cep_batch {
   // host copies of inputs
   std::vector<host::OrdCellIn> ord_in;
   std::vector<host::TtCellIn> tt_in;
+  std::vector<host::O11CellIn> o11_in;
+  bool dutta = false;
+  double time_point = 25.0;
+  int beats_done = 0;
   // device memory
   double *d_y = nullptr, *d_prm = nullptr, *d_bmd = nullptr, *d_ct = nullptr, *d_cv = nullptr;
   double *d_zn = nullptr, *d_ewt = nullptr, *d_acor = nullptr, *d_ft = nullptr, *d_jd = nullptr;
@@ -819,7 +1055,7 @@ cep_batch *cep_batch_create(int model, int ncells, int precision)
 
 static cep_batch *batch_create(int model, int ncells, int precision, bool integrator)
 {
-  if ((model != CEP_MODEL_ORD_CIPA && model != CEP_MODEL_TT06) || ncells <= 0 ||
+  if ((model != CEP_MODEL_ORD_CIPA && model != CEP_MODEL_TT06 && model != CEP_MODEL_ORD2011) || ncells <= 0 ||
       (precision != CEP_PREC_FP64 && precision != CEP_PREC_MIXED))
     return nullptr;
   cep_batch *b = new cep_batch();
@@ -831,6 +1067,11 @@ static cep_batch *batch_create(int model, int ncells, int precision, bool integr
     b->ns = ord::NS; b->nc = ord::NC; b->nalg = ORD_NALG; b->np = ord::NP;
     b->ord_in.resize(ncells);
     for (auto &c : b->ord_in) { memset(&c, 0, sizeof(c)); c.bcl = 1000; for (double &g : c.gs) g = 1.0; c.herg[16] = 1.0; c.herg[17] = 1.0; }
+  } else if (model == CEP_MODEL_ORD2011) {
+    b->ns = ord11::NS; b->nc = ord11::NC; b->nalg = ORD11_NALG; b->np = ord11::NP;
+    b->dt = 1.0;        // max_time_step
+    b->o11_in.resize(ncells);
+    for (auto &c : b->o11_in) { memset(&c, 0, sizeof(c)); c.bcl = 1000; }
   } else {
     b->ns = tt::NS; b->nc = tt::NC; b->nalg = TT_NALG; b->np = tt::NP;
     b->dt = 0.02;
@@ -862,6 +1103,8 @@ static cep_batch *batch_create(int model, int ncells, int precision, bool integr
   } else if (model == CEP_MODEL_TT06) {
     rc |= dalloc(b, &b->d_tcur, P);
     rc |= dalloc(b, &b->d_iout, P);
+  } else if (model == CEP_MODEL_ORD2011) {
+    rc |= dalloc(b, &b->d_stats, P);
   }
   if (rc || ensure_markov_table()) {
     cep_batch_destroy(b);
@@ -911,6 +1154,15 @@ int cep_set_cells(cep_batch *b, int first, int count, const double *celltype, co
       if (herg) for (int j = 0; j < 19; ++j) c.herg[j] = herg[(size_t)k * herg_stride + j];
       if (gscale) for (int j = 0; j < 7; ++j) c.gs[j] = gscale[(size_t)k * gscale_stride + j];
       if (!(c.bcl > 0) || c.celltype < 0 || c.celltype > 2) return fail(b, CEP_E_ARG, "bad celltype/bcl");
+    } else if (b->model == CEP_MODEL_ORD2011) {
+      host::O11CellIn &c = b->o11_in[i];
+      if (gscale) return fail(b, CEP_E_ARG, "ORd 2011 takes no population scales");
+      if (herg && herg_stride < 14) return fail(b, CEP_E_ARG, "ORd 2011 IC50 row needs 14 values");
+      if (celltype) c.celltype = celltype[k];
+      if (conc) c.conc = conc[k];
+      if (bcl) c.bcl = bcl[k];
+      if (herg) for (int j = 0; j < 14; ++j) c.ic50[j] = herg[(size_t)k * herg_stride + j];
+      if (!(c.bcl > 0) || c.celltype < 0 || c.celltype > 2) return fail(b, CEP_E_ARG, "bad celltype/bcl");
     } else {
       host::TtCellIn &c = b->tt_in[i];
       if (celltype) c.celltype = celltype[k];
@@ -934,16 +1186,31 @@ int cep_set_run_config(cep_batch *b, double dt, double rtol, double atol, int ma
   return 0;
 }
 
+int cep_set_model_option(cep_batch *b, int option, double value)
+{
+  if (!b) return CEP_E_ARG;
+  if (b->model != CEP_MODEL_ORD2011) return fail(b, CEP_E_ARG, "no options for this model");
+  if (option == CEP_OPT_DUTTA) b->dutta = value != 0.0;
+  else if (option == CEP_OPT_TIME_POINT && value >= 0.0) b->time_point = value;
+  else return fail(b, CEP_E_ARG, "unknown option or value");
+  return 0;
+}
+
 static double max_bcl(const cep_batch *b)
 {
   double m = 0;
   if (b->model == CEP_MODEL_ORD_CIPA) for (auto &c : b->ord_in) m = c.bcl > m ? c.bcl : m;
+  else if (b->model == CEP_MODEL_ORD2011) for (auto &c : b->o11_in) m = c.bcl > m ? c.bcl : m;
   else for (auto &c : b->tt_in) m = c.bcl > m ? c.bcl : m;
   return m;
 }
 
 static int alloc_trace_buffers(cep_batch *b)
 {
+  if (b->model == CEP_MODEL_ORD2011) {   // features need no trace buffers
+    b->cap = 1;
+    return 0;
+  }
   // Ca trace of one beat sampled every 1 ms (dt_write), sized for the longest BCL
   const int print_freq = (int)((1. / b->dt) * 1.0);
   const int beat_steps = (int)(max_bcl(b) / b->dt);
@@ -973,6 +1240,9 @@ static int reset_integrators(cep_batch *b)
     HIPCHK(b, hipMemsetAsync(b->d_zn, 0, P * (ordbdf::QMAX + 1) * ordbdf::N * sizeof(double), b->stream));
     HIPCHK(b, hipMemsetAsync(b->d_sd, 0, P * ordbdf::SD_N * sizeof(double), b->stream));
     HIPCHK(b, hipMemsetAsync(b->d_stats, 0, P * 8 * sizeof(long long), b->stream));
+  } else if (b->model == CEP_MODEL_ORD2011) {
+    HIPCHK(b, hipMemsetAsync(b->d_stats, 0, P * sizeof(long long), b->stream));
+    b->beats_done = 0;
   } else {
     HIPCHK(b, hipMemsetAsync(b->d_tcur, 0, P * sizeof(double), b->stream));
     HIPCHK(b, hipMemsetAsync(b->d_iout, 0, P * sizeof(int), b->stream));
@@ -991,6 +1261,9 @@ int cep_init_consts(cep_batch *b)
       host::ord_params(b->ord_in[i], b->stim_scale, p);
       for (int s = 0; s < 49; ++s) y[s * P + i] = host::kOrdInitStates[s];
       y[43 * P + i] = b->ord_in[i].conc;
+    } else if (b->model == CEP_MODEL_ORD2011) {
+      host::o11_params(b->o11_in[i], b->dutta, b->stim_scale, p);
+      for (int s = 0; s < ord11::NS; ++s) y[s * P + i] = host::kO11Init[s];
     } else {
       host::tt_params(b->tt_in[i], b->stim_scale, p);
       const int ct = (int)b->tt_in[i].celltype;
@@ -1010,6 +1283,7 @@ int cep_get_constants(const cep_batch *b, int cell, double *out)
 {
   if (!b || cell < 0 || cell >= b->n || !out) return CEP_E_ARG;
   if (b->model == CEP_MODEL_ORD_CIPA) host::ord_constants(b->ord_in[cell], b->stim_scale, out);
+  else if (b->model == CEP_MODEL_ORD2011) host::o11_constants(b->o11_in[cell], b->dutta, b->stim_scale, out);
   else host::tt_constants(b->tt_in[cell], b->stim_scale, out);
   return 0;
 }
@@ -1034,6 +1308,11 @@ int cep_compute_rates(cep_batch *b, double t, const double *states, double *rate
       hipLaunchKernelGGL(ord_rates_kernel<double>, dim3(blocks), dim3(kRatesBlock), 0, b->stream, b->n, b->pitch, t, dy, df, da, b->d_prm);
     else
       hipLaunchKernelGGL(ord_rates_kernel<float>, dim3(blocks), dim3(kRatesBlock), 0, b->stream, b->n, b->pitch, t, dy, df, da, b->d_prm);
+  } else if (b->model == CEP_MODEL_ORD2011) {
+    if (b->prec == CEP_PREC_FP64)
+      hipLaunchKernelGGL(ord11_rates_kernel<double>, dim3(blocks), dim3(kRatesBlock), 0, b->stream, b->n, b->pitch, t, dy, df, da, b->d_prm);
+    else
+      hipLaunchKernelGGL(ord11_rates_kernel<float>, dim3(blocks), dim3(kRatesBlock), 0, b->stream, b->n, b->pitch, t, dy, df, da, b->d_prm);
   } else {
     if (b->prec == CEP_PREC_FP64)
       hipLaunchKernelGGL(tt_rates_kernel<double>, dim3(blocks), dim3(kRatesBlock), 0, b->stream, b->n, b->pitch, t, dy, df, da, b->d_prm);
@@ -1076,6 +1355,19 @@ int cep_run_beats_async(cep_batch *b, int nbeats)
       hipLaunchKernelGGL((ord_bdf_kernel<double, CPW>), dim3(blocks), dim3(64), 0, b->stream, B, rc, nbeats, max_passes);
     else
       hipLaunchKernelGGL((ord_bdf_kernel<float, CPW>), dim3(blocks), dim3(64), 0, b->stream, B, rc, nbeats, max_passes);
+  } else if (b->model == CEP_MODEL_ORD2011) {
+    O11Buffers B;
+    B.ncell = b->n; B.pitch = b->pitch; B.nbeats_total = b->nbeats_total; B.nwin = b->nwin;
+    B.y = b->d_y; B.prm = b->d_prm; B.bmd = b->d_bmd; B.bmi = b->d_bmi; B.status = b->d_status;
+    B.stats = b->d_stats; B.time_point = b->time_point; B.max_dt = b->dt;
+    // guard only: the smallest step the rule produces is 0.0005 ms
+    const long long max_steps = (long long)nbeats * (long long)(max_bcl(b) / 0.0005) + 1000;
+    const int blocks = (b->n + 63) / 64;
+    if (b->prec == CEP_PREC_FP64)
+      hipLaunchKernelGGL(ord11_rl_kernel<double>, dim3(blocks), dim3(64), 0, b->stream, B, b->beats_done + nbeats, max_steps);
+    else
+      hipLaunchKernelGGL(ord11_rl_kernel<float>, dim3(blocks), dim3(64), 0, b->stream, B, b->beats_done + nbeats, max_steps);
+    b->beats_done += nbeats;
   } else {
     TtBuffers B;
     B.ncell = b->n; B.pitch = b->pitch; B.cap = b->cap; B.tcap = b->tcap; B.y = b->d_y;
@@ -1118,8 +1410,13 @@ int cep_query(cep_batch *b)
 
 long long cep_batch_device_bytes(int model, int ncells, double max_bcl, double dt, int vm_trace)
 {
-  if ((model != CEP_MODEL_ORD_CIPA && model != CEP_MODEL_TT06) || ncells <= 0 || !(dt > 0) || !(max_bcl > 0))
+  if ((model != CEP_MODEL_ORD_CIPA && model != CEP_MODEL_TT06 && model != CEP_MODEL_ORD2011) || ncells <= 0 ||
+      !(dt > 0) || !(max_bcl > 0))
     return CEP_E_ARG;
+  if (model == CEP_MODEL_ORD2011) {
+    const long long P = (ncells + 63) / 64 * 64;
+    return (P * (ord11::NS + ord11::NP + BM_TOTAL_D + 1)) * 8 + P * (BM_I_N + 1) * 4 + 6 * 64;
+  }
   const long long P = (ncells + 63) / 64 * 64, pad = 64;
   const bool ord = model == CEP_MODEL_ORD_CIPA;
   const long long ns = ord ? ord::NS : tt::NS, np = ord ? ord::NP : tt::NP;
@@ -1188,6 +1485,11 @@ int cep_get_biomarkers(const cep_batch *b, double *out)
       hipStreamSynchronize(b->stream) != hipSuccess)
     return CEP_E_HIP;
   for (int i = 0; i < b->n; ++i) {
+    if (b->model == CEP_MODEL_ORD2011) {
+      for (int k = 0; k < CEP_BM11_STEEPEST_PACE; ++k) out[(size_t)i * CEP_NBM + k] = d[(size_t)(O11_BEST + k) * b->pitch + i];
+      out[(size_t)i * CEP_NBM + CEP_BM11_STEEPEST_PACE] = iv[(size_t)O11_I_STEEP * b->pitch + i];
+      continue;
+    }
     for (int k = 0; k < BM_RES_N; ++k) out[(size_t)i * CEP_NBM + k] = d[(size_t)(BM_D_N + BM_RES_N + k) * b->pitch + i];
     out[(size_t)i * CEP_NBM + CEP_BM_STEEPEST_PACE] = iv[(size_t)1 * b->pitch + i];
   }
@@ -1207,6 +1509,15 @@ int cep_get_stats(const cep_batch *b, long long *out)
 {
   if (!b || !out) return CEP_E_ARG;
   if (!b->d_stats) { memset(out, 0, sizeof(long long) * 8 * b->n); return 0; }
+  if (b->model == CEP_MODEL_ORD2011) {   // steps (one right-hand side each) in slots 0 and 1
+    std::vector<long long> tmp((size_t)b->pitch);
+    if (hipMemcpyAsync(tmp.data(), b->d_stats, tmp.size() * sizeof(long long), hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
+        hipStreamSynchronize(b->stream) != hipSuccess)
+      return CEP_E_HIP;
+    memset(out, 0, sizeof(long long) * 8 * b->n);
+    for (int i = 0; i < b->n; ++i) out[(size_t)i * 8] = out[(size_t)i * 8 + 1] = tmp[i];
+    return 0;
+  }
   std::vector<long long> tmp((size_t)b->pitch * 8);
   if (hipMemcpyAsync(tmp.data(), b->d_stats, tmp.size() * sizeof(long long), hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
       hipStreamSynchronize(b->stream) != hipSuccess)

@@ -29,6 +29,9 @@ extern "C" {
 
 #define CEP_MODEL_ORD_CIPA 1   /* O'Hara-Rudy CiPA v1 2017, 52 states (ohara_rudy_cipa_v1_2017.cpp) */
 #define CEP_MODEL_TT06 2       /* ten Tusscher 2006 IK1Ko endo/epi/M, 19 states */
+#define CEP_MODEL_ORD2011 3    /* O'Hara-Rudy 2011 of the DrugSimulation driver, 41 states
+                                  (DrugSimulation/cellmodels/Ohara_Rudy_2011.cpp), adaptive
+                                  Rush-Larsen (solveAnalytical + set_time_step, :991-1137) */
 
 #define CEP_PREC_FP64 0        /* all arithmetic in FP64 */
 #define CEP_PREC_MIXED 1       /* gating/currents in FP32; state, Newton, LU, error control in FP64 */
@@ -62,6 +65,27 @@ extern "C" {
 #define CEP_BM_STEEPEST_PACE 15
 #define CEP_NBM 16
 
+/* ORd 2011 record per cell: the DrugSimulation driver's features of the selected beat
+   (drug_sim.cpp:306-420; the beat with the largest dVm/dt in [repol90, repol30] among the
+   trailing nwin beats).  qnet*_ap accumulate while Vm > repol90, *_cl over the whole beat
+   (C/F); an EAD is reported by the reference when dvmdt_repol > 0 (:424). */
+#define CEP_BM11_QNET_AP 0
+#define CEP_BM11_QNET4_AP 1
+#define CEP_BM11_INAL_AUC_AP 2
+#define CEP_BM11_ICAL_AUC_AP 3
+#define CEP_BM11_QNET_CL 4
+#define CEP_BM11_QNET4_CL 5
+#define CEP_BM11_INAL_AUC_CL 6
+#define CEP_BM11_ICAL_AUC_CL 7
+#define CEP_BM11_VM_PEAK 8
+#define CEP_BM11_VM_VALLEY 9
+#define CEP_BM11_DVMDT_REPOL 10
+#define CEP_BM11_T_PEAK 11
+#define CEP_BM11_VM_REPOL30 12
+#define CEP_BM11_VM_REPOL50 13
+#define CEP_BM11_VM_REPOL90 14
+#define CEP_BM11_STEEPEST_PACE 15
+
 typedef struct cep_batch cep_batch;
 
 /* Create a batch of ncells cells of one model.  Replaces `new <model>()` per cell
@@ -79,13 +103,28 @@ int cep_batch_info(const cep_batch *b, int *model, int *ncells, int *nstates, in
    (CEPPatch_Population_Drugs/benchpatch.cpp:211-241 style CONSTANTS[g] *= scale).
    TT06: herg row = IC50[IKr, ICaL, INa], Hill[IKr, ICaL, INa]; gscale row = 10 scales of
    GNa, GK1, GKr, GKs, Gto, GCaL, GbNa, GbCa, GpCa, GpK (population_tn2006.cpp:91-100).
+   ORd 2011: herg row = the 14 IC50/Hill values of the IC50 CSV row (ICaL, IK1, IKs, INa, INaL,
+   Ito, hERG; initConsts(celltype, conc, ic50, is_dutta), Ohara_Rudy_2011.cpp:722-740); gscale
+   must be null; celltype is accepted but, as in the reference, the model's initialisation
+   resets it, so every cell has the endocardial constants.
    bcl: basic cycle length (ms) -> stimulus period.  Null gscale = all ones. */
 int cep_set_cells(cep_batch *b, int first, int count, const double *celltype, const double *conc,
                   const double *herg, int herg_stride, const double *gscale, int gscale_stride,
                   const double *bcl);
 
+/* Model options (no reference counterpart as a call; they are the driver parameters of
+   DrugSimulation/modules/drug_sim.cpp:74-81 and the Is_Dutta deck entry):
+   CEP_OPT_DUTTA (ORd 2011): nonzero applies ___applyDutta (Ohara_Rudy_2011.cpp:460-467) after
+   the constants, as the reference does (it scales the CONSTANTS slots 98, 96, 100, 81, 64);
+   CEP_OPT_TIME_POINT (ORd 2011): the fixed-0.005-ms window at the start of every beat (25 ms).
+   Call before cep_init_consts. */
+#define CEP_OPT_DUTTA 1
+#define CEP_OPT_TIME_POINT 2
+int cep_set_model_option(cep_batch *b, int option, double value);
+
 /* Integration / biomarker configuration shared by the batch.
-   dt: output step (ms; also CVODE max step for ORd, the Rush-Larsen step for TT06);
+   dt: output step (ms; also CVODE max step for ORd, the Rush-Larsen step for TT06, the
+   largest adaptive step max_time_step for ORd 2011, 1.0 in the reference);
    rtol/atol: BDF scalar tolerances (CVodeSStolerances); maxord: BDF max order (5);
    nbeats_total: beats of the whole job (biomarkers scan its trailing nwin beats);
    stim_amp_scale: multiplies the stimulus amplitude (drugs_herg.cpp:150). */

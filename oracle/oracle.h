@@ -47,10 +47,48 @@ void tt_init_consts(double *C, double *S, int celltype);
 void tt_rates(double t, const double *C, double *R, const double *S, double *A, int celltype);
 void tt_rush_larsen(double dt, const double *C, const double *R, double *S, const double *A);
 
-/* ---------------- ORd 2011 static-Hill (CEPPatch_CiPA_Sim/patches/Ohara_Rudy_2011.cpp) */
-#define ORD11_NS 44
-#define ORD11_NC 170
-#define ORD11_NA 200
+/* ---------------- ORd 2011 (DrugSimulation/cellmodels/Ohara_Rudy_2011.cpp) */
+#define ORD11_NS 41
+#define ORD11_NC 144
+#define ORD11_NA 198
+/* celltype is stored and then reset to 0 by the model's own initialisation (endo constants) */
+void ord11_init_consts(double *C, double *S, double celltype);
+void ord11_apply_dutta(double *C);
+/* ic50[14]: (IC50, Hill) of ICaL, IK1, IKs, INa, INaL, Ito, hERG (IC50 CSV column order) */
+void ord11_apply_drug(double *C, double conc, const double *ic50);
+void ord11_rates(double t, const double *C, double *R, const double *S, double *A);
+/* tissue_stim < 0: single-cell periodic stimulus; 0 / 1: the tissue stimulus off / on */
+void ord11_rates_ex(double t, const double *C, double *R, const double *S, double *A, int tissue_stim);
+void ord11_rush_larsen(double dt, const double *C, const double *R, double *S, const double *A);
+double ord11_time_step(double t, double time_point, double max_dt, const double *C, const double *R);
+
+/* per-beat features of the DrugSimulation driver (drug_sim.cpp:306-420) */
+#define ORD11_BM_QNET_AP 0
+#define ORD11_BM_QNET4_AP 1
+#define ORD11_BM_INAL_AUC_AP 2
+#define ORD11_BM_ICAL_AUC_AP 3
+#define ORD11_BM_QNET_CL 4
+#define ORD11_BM_QNET4_CL 5
+#define ORD11_BM_INAL_AUC_CL 6
+#define ORD11_BM_ICAL_AUC_CL 7
+#define ORD11_BM_VM_PEAK 8
+#define ORD11_BM_VM_VALLEY 9
+#define ORD11_BM_DVMDT_REPOL 10
+#define ORD11_BM_T_PEAK 11
+#define ORD11_BM_VM_REPOL30 12
+#define ORD11_BM_VM_REPOL50 13
+#define ORD11_BM_VM_REPOL90 14
+#define ORD11_BM_STEEPEST_PACE 15
+#define ORD11_NBM 16
+typedef struct {
+  double celltype, conc, bcl;
+  double ic50[14];
+  double time_point, max_dt;   /* 25 ms, 1.0 ms in the reference */
+  int dutta, nbeats, nwin;     /* nwin = last_drug_check_pace */
+} ord11_cell_cfg_t;
+long oracle_ord11_run(const ord11_cell_cfg_t *cfg, double *states_io, int use_states_in, double *res);
+double oracle_ord11_run_many(const ord11_cell_cfg_t *cfgs, int ncell, int nthreads, const double *states_in,
+                             double *res, double *states_out);
 
 /* ---------------- CVODE-style BDF */
 typedef void (*bdf_rhs_fn)(double t, const double *y, double *ydot, void *user);

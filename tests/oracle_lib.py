@@ -27,6 +27,12 @@ class BmResult(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class Ord11CellCfg(C.Structure):
+    _fields_ = [("celltype", C.c_double), ("conc", C.c_double), ("bcl", C.c_double),
+                ("ic50", C.c_double * 14), ("time_point", C.c_double), ("max_dt", C.c_double),
+                ("dutta", C.c_int), ("nbeats", C.c_int), ("nwin", C.c_int)]
+
+
 class BdfStats(C.Structure):
     _fields_ = [(n, C.c_long) for n in ("nst", "nfe", "nsetups", "nje", "nni", "ncfn", "netf")]
 
@@ -76,6 +82,18 @@ def lib():
         L.oracle_tt_run_many.argtypes = [C.POINTER(TtCellCfg), C.c_int, C.c_int,
                                          C.POINTER(BmResult), dp]
         L.oracle_tt_run_many.restype = C.c_double
+        L.ord11_init_consts.argtypes = [dp, dp, C.c_double]
+        L.ord11_apply_dutta.argtypes = [dp]
+        L.ord11_apply_drug.argtypes = [dp, C.c_double, dp]
+        L.ord11_rates.argtypes = [C.c_double, dp, dp, dp, dp]
+        L.ord11_rates_ex.argtypes = [C.c_double, dp, dp, dp, dp, C.c_int]
+        L.ord11_rush_larsen.argtypes = [C.c_double, dp, dp, dp, dp]
+        L.ord11_time_step.argtypes = [C.c_double, C.c_double, C.c_double, dp, dp]
+        L.ord11_time_step.restype = C.c_double
+        L.oracle_ord11_run.argtypes = [C.POINTER(Ord11CellCfg), dp, C.c_int, dp]
+        L.oracle_ord11_run.restype = C.c_long
+        L.oracle_ord11_run_many.argtypes = [C.POINTER(Ord11CellCfg), C.c_int, C.c_int, dp, dp, dp]
+        L.oracle_ord11_run_many.restype = C.c_double
         L.oracle_ord_rates_many.argtypes = [C.c_int, dp, dp, dp, C.c_double]
         L.oracle_ord_rates_many.restype = C.c_double
         L.bdf_create.argtypes = [C.c_int, BDF_RHS, C.c_void_p, C.POINTER(BdfOpts)]
@@ -182,3 +200,59 @@ def tt_run(cfg: TtCellCfg, states=None, trace=False):
     lib().oracle_tt_run(C.byref(cfg), ptr(S), 0 if states is None else 1, C.byref(res),
                         ptr(vm) if trace else None)
     return S, res.as_dict(), vm
+
+
+ORD11_NS, ORD11_NC, ORD11_NA, ORD11_NBM = 41, 144, 198, 16
+ORD11_BM_FIELDS = ("qnet_ap", "qnet4_ap", "inal_auc_ap", "ical_auc_ap", "qnet_cl", "qnet4_cl",
+                   "inal_auc_cl", "ical_auc_cl", "vm_peak", "vm_valley", "dvmdt_repol", "t_peak",
+                   "vm_repol30", "vm_repol50", "vm_repol90", "steepest_pace")
+
+
+def ord11_init(celltype=0.0, conc=0.0, ic50=None, dutta=False):
+    Cc = np.zeros(ORD11_NC); S = np.zeros(ORD11_NS)
+    lib().ord11_init_consts(ptr(Cc), ptr(S), float(celltype))
+    if dutta:
+        lib().ord11_apply_dutta(ptr(Cc))
+    if ic50 is not None:
+        a = np.ascontiguousarray(ic50, dtype=np.float64)
+        lib().ord11_apply_drug(ptr(Cc), float(conc), ptr(a))
+    return Cc, S
+
+
+def ord11_rates(t, Cc, S, tissue_stim=-1):
+    R = np.zeros(ORD11_NS); A = np.zeros(ORD11_NA)
+    S = np.ascontiguousarray(S, dtype=np.float64)
+    lib().ord11_rates_ex(float(t), ptr(Cc), ptr(R), ptr(S), ptr(A), int(tissue_stim))
+    return R, A
+
+
+def make_ord11_cfg(conc=0.0, ic50=None, bcl=1000.0, nbeats=1, nwin=1, dutta=False, celltype=0.0,
+                   time_point=25.0, max_dt=1.0):
+    c = Ord11CellCfg()
+    c.celltype = float(celltype); c.conc = float(conc); c.bcl = float(bcl)
+    if ic50 is not None:
+        for k in range(14):
+            c.ic50[k] = float(ic50[k])
+    c.time_point = time_point; c.max_dt = max_dt
+    c.dutta = int(bool(dutta)); c.nbeats = int(nbeats); c.nwin = int(nwin)
+    return c
+
+
+def ord11_run(cfg, states=None):
+    """One cell through the DrugSimulation loop: (final states, feature record, steps)."""
+    S = np.zeros(ORD11_NS) if states is None else np.ascontiguousarray(states, dtype=np.float64).copy()
+    res = np.zeros(ORD11_NBM)
+    n = lib().oracle_ord11_run(C.byref(cfg), ptr(S), int(states is not None), ptr(res))
+    return S, res, n
+
+
+def ord11_run_many(cfgs, states_in=None, nthreads=None):
+    """Cells of the DrugSimulation loop over host threads: (final states, records, wall s)."""
+    n = len(cfgs)
+    arr = (Ord11CellCfg * n)(*cfgs)
+    res = np.zeros((n, ORD11_NBM))
+    sout = np.zeros((n, ORD11_NS))
+    sin = None if states_in is None else np.ascontiguousarray(np.broadcast_to(states_in, (n, ORD11_NS)), dtype=np.float64)
+    nt = nthreads or min(16, os.cpu_count() or 1)
+    wall = lib().oracle_ord11_run_many(arr, n, nt, None if sin is None else ptr(sin), ptr(res), ptr(sout))
+    return sout, res, wall
