@@ -10,6 +10,8 @@ one (FP32 gating/currents, FP64 state/Newton/LU); an FP64 run of a subsample, as
 job, checks APD90 and qNet and gives the plain per-job FP64 rate.  A short serving session
 (Poisson arrivals of mixed ORd-CiPA / TT06 jobs) gives the p50/p99 latency.
 
+`--workload ord2011` times the reference's own ORd 2011 run (quinidine, 30 cells x 1000
+beats), whose wall time the reference published, and a batched population of that model.
 `--workload tissue` instead times the 2-D monodomain step (512 x 512 Q1 elements, ORd-CiPA
 nodes, Strang splitting, Jacobi-PCG diffusion) in node-steps/s; a "step" is one time step of
 the whole tissue.
@@ -242,10 +244,114 @@ def tissue_main(args):
     print(json.dumps(out), flush=True)
 
 
+# ORd 2011 workload with a published reference time: the DrugSimulation quinidine run (10 IC50
+# samples x 3 concentrations, 1000 paces at BCL 2000 ms from the shipped steady state, M cell,
+# no Dutta) took 1048.7 s on 10 MPI ranks = 28.6 cell-beats/s.
+ORD11_REF_CELL_BEATS_PER_S = 30 * 1000 / 1048.7
+ORD11_EAD_LOGGED = {(8, 6474.0), (4, 9711.0), (9, 9711.0), (7, 9711.0), (0, 9711.0), (8, 9711.0)}
+
+
+def ord11_main(args):
+    """The reference's own ORd 2011 run, on one GPU, plus a batched population of the same
+    model for throughput at scale."""
+    G = os.path.join(ROOT, "tests", "golden")
+    q = np.loadtxt(os.path.join(G, "quinidine_IC50_samples10.csv"), delimiter=",", skiprows=1)
+    s0 = np.loadtxt(os.path.join(G, "ord2011_output_orudy.dat"))
+    nbeats = args.steps if args.steps is not None else 1000
+    keys = [(i, c) for c in (3237.0, 6474.0, 9711.0) for i in range(len(q))]
+    n = len(keys)
+    prec = cep.PREC_FP64 if (args.precision or "fp64") == "fp64" else cep.PREC_MIXED
+
+    def batch(ncell, conc, ic50):
+        b = cep.CellBatch(cep.MODEL_ORD2011, ncell, prec)
+        b.set_cells(celltype=np.full(ncell, 2.0), conc=conc, herg=ic50, bcl=np.full(ncell, 2000.0))
+        b.set_run_config(dt=1.0, nbeats_total=nbeats, nwin=min(250, nbeats))
+        b.init_consts()
+        b.set_states(np.repeat(s0[None, :], ncell, axis=0))
+        return b
+
+    log(f"bench: ORd 2011 quinidine run, {n} cells x {nbeats} beats")
+    b = batch(n, np.array([c for _, c in keys]), np.vstack([q[i] for i, _ in keys]))
+    b.kernel_time()
+    t0 = time.perf_counter()
+    b.run_beats(nbeats)
+    t1 = time.perf_counter()
+    nl, kavg, ktot = b.kernel_time()
+    rec = b.biomarker_array()
+    steps = b.stats()[:, 0].astype(float)
+    bad = int(np.sum(b.status() != 0))
+    b.close()
+    ead = {k for k, r in zip(keys, rec) if r[cep.BM11_FIELDS.index("dvmdt_repol")] > 0}
+    wall = t1 - t0
+    value = n * nbeats / wall
+    # batched population of the same model: the quinidine samples tiled over conc 0..4 x 9711 nM
+    npop = args.ncell if args.ncell != 10000 else 100000
+    popbeats = 2
+    rng = np.random.default_rng(20251208)
+    conc = rng.uniform(0.0, 4 * 9711.0, npop)
+    ic50 = q[rng.integers(0, len(q), npop)]
+    log(f"bench: ORd 2011 population, {npop} cells x {popbeats} beats")
+    bp = batch(npop, conc, ic50)
+    bp.kernel_time()
+    t2 = time.perf_counter()
+    bp.run_beats(popbeats)
+    t3 = time.perf_counter()
+    pl, pavg, _ = bp.kernel_time()
+    psteps = float(bp.stats()[:, 0].sum())
+    bp.close()
+    # algorithmic work per step (oracle/opcount.cpp on the ORd 2011 restatement): RHS 1046
+    # flops + Rush-Larsen/Euler update 146 flops (103 exp/log/pow apart)
+    f_step = 1046 + 146
+    ach = f_step * psteps / (pavg * pl * 1e-3) if pavg > 0 else 0.0
+    peak = PEAK_FP64 if prec == cep.PREC_FP64 else PEAK_FP32
+    out = {
+        "metric": "ORd 2011 cell-beats/s (DrugSimulation quinidine run: 10 samples x 3 conc x 1000 beats)",
+        "value": value,
+        "unit": "cell-beats/s",
+        "n_gpus": 1,
+        "steps": nbeats,
+        "warmup": 0,
+        "ms_per_step": 1e3 * wall / nbeats,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": value / ORD11_REF_CELL_BEATS_PER_S if nbeats == 1000 else None,
+        "dtype": "f64" if prec == cep.PREC_FP64 else "f32-rates/f64-state",
+        "data": "reference fixtures (quinidine IC50 samples, steady state) + seeded population",
+        "config": {"workload": "ORd 2011 M cell, quinidine IC50_samples10 x {3237, 6474, 9711} nM, BCL 2000, "
+                               "adaptive Rush-Larsen 0.005-1 ms, features over the last 250 beats",
+                   "cells": n, "beats": nbeats},
+        "reference_wall_s": 1048.7,
+        "wall_s": wall,
+        "ead_cases": sorted([list(k) for k in ead]),
+        "ead_matches_reference_log": ead == ORD11_EAD_LOGGED if nbeats == 1000 else None,
+        "failed_cells": bad,
+        "steps_per_cell_beat": float(steps.mean() / nbeats),
+        "population": {"cells": npop, "beats": popbeats, "cell_beats_per_s": npop * popbeats / (t3 - t2),
+                       "kernel_avg_ms": pavg},
+        "roofline": {"bound": "valu", "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+                     "frac": ach / peak, "traffic": None, "kernel": "ord11_rl_kernel (population)",
+                     "flops_per_step": f_step},
+        "cpu_baseline": None if args.no_cpu_baseline else ord11_cpu_baseline(q, s0),
+    }
+    print(json.dumps(out), flush=True)
+
+
+def ord11_cpu_baseline(q, s0):
+    """The FP64 oracle on the host: the same quinidine cells, 20 beats each, 16 threads."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    cfgs = [O.make_ord11_cfg(conc=c, ic50=q[i], bcl=2000.0, nbeats=20, nwin=20, celltype=2.0)
+            for c in (3237.0, 6474.0, 9711.0) for i in range(len(q))]
+    threads = min(16, os.cpu_count() or 1)
+    _, _, wall = O.ord11_run_many(cfgs, s0, nthreads=threads)
+    return {"value": len(cfgs) * 20 / wall, "unit": "cell-beats/s", "cores": threads, "kind": "port",
+            "sample": f"{len(cfgs)} cells x 20 beats of the same run, FP64 oracle, {threads} threads, {wall:.1f} s"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("population", "tissue"), default="population")
+    ap.add_argument("--workload", choices=("population", "tissue", "ord2011"), default="population")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--tissue-nx", type=int, default=TISSUE_NX)
@@ -262,6 +368,9 @@ def main():
         print(json.dumps({"error": "single-GPU build: run with --gpus 1 (cells shard trivially "
                           "across GPUs; see the design notes)"}))
         sys.exit(2)
+    if args.workload == "ord2011":
+        ord11_main(args)
+        return
     if args.workload == "tissue":
         args.precision = args.precision or "fp64"
         tissue_main(args)

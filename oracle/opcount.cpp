@@ -34,6 +34,8 @@ static CD log(CD a) { T.logs++; return CD(std::log(a.v)); }
 static CD pow(CD a, CD b) { T.pows++; return CD(std::pow(a.v, b.v)); }
 static CD floor(CD a) { T.other++; return CD(std::floor(a.v)); }
 static CD sqrt(CD a) { T.other++; return CD(std::sqrt(a.v)); }
+static CD fabs(CD a) { return CD(std::fabs(a.v)); }
+static bool operator!=(CD a, CD b) { return a.v != b.v; }
 
 // the C prototypes of oracle.h are skipped: the model sources are re-typed here
 #define CEP_ORACLE_H
@@ -43,12 +45,21 @@ static CD sqrt(CD a) { T.other++; return CD(std::sqrt(a.v)); }
 #define TT_NS 19
 #define TT_NC 53
 #define TT_NA 70
+#define ORD11_NS 41
+#define ORD11_NC 144
+#define ORD11_NA 198
+#define ORD11_NBM 16
+#include <pthread.h>
+#include <time.h>
+typedef struct ord11_cell_cfg_t ord11_cell_cfg_t;
 #define double CD
 #define memset(p, v, n) for (size_t k_ = 0; k_ < (n) / sizeof(CD); ++k_) (p)[k_] = CD(0)
 #define memcpy(d, s, n) for (size_t k_ = 0; k_ < (n) / sizeof(CD); ++k_) (d)[k_] = (s)[k_]
 extern "C++" {
 #include "ord_cipa.c"
 #include "tt06.c"
+#define ORD11_ONLY_MODEL
+#include "ord2011.c"
 }
 #undef double
 
@@ -67,6 +78,16 @@ int main()
   T = Tally();
   tt_rush_larsen(CD(0.02), tc, tr, ts, ta);
   const Tally rl = T;
+  CD oc[ORD11_NC], os[ORD11_NS], orr[ORD11_NS], oa[ORD11_NA];
+  ord11_init_consts(oc, os, CD(0.0));
+  T = Tally();
+  ord11_rates(CD(100.0), oc, orr, os, oa);
+  const Tally o11 = T;
+  T = Tally();
+  ord11_rush_larsen(CD(0.1), oc, orr, os, oa);
+  const Tally o11rl = T;
+  printf("ord11_rhs flops %ld exp %ld log %ld pow %ld other %ld\n", o11.flops, o11.exps, o11.logs, o11.pows, o11.other);
+  printf("ord11_rl_update flops %ld exp %ld log %ld pow %ld other %ld\n", o11rl.flops, o11rl.exps, o11rl.logs, o11rl.pows, o11rl.other);
   printf("ord_rhs flops %ld exp %ld log %ld pow %ld other %ld\n", o.flops, o.exps, o.logs, o.pows, o.other);
   printf("tt_rhs flops %ld exp %ld log %ld pow %ld other %ld\n", t.flops, t.exps, t.logs, t.pows, t.other);
   printf("tt_rl_update flops %ld exp %ld log %ld pow %ld other %ld\n", rl.flops, rl.exps, rl.logs, rl.pows, rl.other);

@@ -22,7 +22,9 @@
  * the Rush-Larsen/rates consistency, the step rule, and the currents this model shares with
  * the ORd-CiPA oracle (itself pinned by the compiled ORd-CiPA reference).
  */
+#ifndef _POSIX_C_SOURCE
 #define _POSIX_C_SOURCE 200809L
+#endif
 #include <math.h>
 #include <pthread.h>
 #include <string.h>
@@ -516,6 +518,7 @@ double ord11_time_step(double t, double time_point, double max_dt, const double 
   return ts;
 }
 
+#ifndef ORD11_ONLY_MODEL
 /* One cell through the DrugSimulation ORd 2011 driver loop (drug_sim.cpp:306-420): adaptive
    Rush-Larsen, steps clipped to beat ends, and per-beat features over the last nwin beats.
    Output record res[ORD11_NBM] (see oracle.h).  Returns the number of steps taken. */
@@ -648,3 +651,4 @@ double oracle_ord11_run_many(const ord11_cell_cfg_t *cfgs, int ncell, int nthrea
   clock_gettime(CLOCK_MONOTONIC, &b);
   return (b.tv_sec - a.tv_sec) + 1e-9 * (b.tv_nsec - a.tv_nsec);
 }
+#endif
