@@ -113,6 +113,17 @@ double bdf_tn(const bdf_mem_t *m);
 double bdf_h(const bdf_mem_t *m);
 int bdf_q(const bdf_mem_t *m);
 
+/* ---------------- Shampine-Gordon Adams PECE (libs/ode.cpp) */
+typedef void (*adams_rhs_fn)(double t, const double *y, double *yp, void *user);
+typedef struct adams_mem adams_mem_t;
+adams_mem_t *adams_create(int n, adams_rhs_fn f, void *user);
+void adams_free(adams_mem_t *m);
+/* one call of the reference's ode(): returns and stores iflag (2 = reached tout) */
+int adams_ode(adams_mem_t *m, double *y, double *t, double tout, double relerr, double abserr, int *iflag);
+void adams_counts(const adams_mem_t *m, long *nsteps, long *nfe, int *order);
+/* the reference's Adams loop on a test problem (mode 0 linear, 1 Van der Pol, 2 ORd 2011) */
+long oracle_adams_run(int mode, const double *in, double *out, long out_cap, long *nsteps, long *nfe);
+
 /* ---------------- biomarkers (CEPCell_NEO/modules/cipa_drug_bench.cpp:210-381) */
 typedef struct {
   double qnet, inal_auc, ical_auc, dvmdt_repol, dvmdt_max, vm_peak, vm_valley, vm_dia,
