@@ -22,6 +22,9 @@ MODEL_TT06 = 2
 MODEL_ORD2011 = 3
 OPT_DUTTA = 1
 OPT_TIME_POINT = 2
+OPT_INTEGRATOR = 3
+INTEGRATOR_RUSH_LARSEN = 0
+INTEGRATOR_ADAMS = 1
 PREC_FP64 = 0
 PREC_MIXED = 1
 
@@ -37,7 +40,7 @@ STATUS_NAMES = {0: "ok", 1: "conv_fail", 2: "err_fail", 3: "nonfinite", 4: "max_
 
 EXPORTED = (
     "cep_batch_create", "cep_batch_destroy", "cep_last_error", "cep_batch_info", "cep_set_cells",
-    "cep_set_model_option",
+    "cep_set_model_option", "cep_run_outputs",
     "cep_set_run_config", "cep_init_consts", "cep_get_constants", "cep_compute_rates",
     "cep_run_beats", "cep_run_beats_async", "cep_sync", "cep_get_states", "cep_set_states",
     "cep_get_biomarkers", "cep_get_status", "cep_get_stats", "cep_enable_vm_trace",
@@ -80,6 +83,7 @@ def library(path: str | None = None):
                                      C.c_int, C.c_int, C.c_double]
     L.cep_init_consts.argtypes = [C.c_void_p]
     L.cep_set_model_option.argtypes = [C.c_void_p, C.c_int, C.c_double]
+    L.cep_run_outputs.argtypes = [C.c_void_p, C.c_int]
     L.cep_get_constants.argtypes = [C.c_void_p, C.c_int, dp]
     L.cep_compute_rates.argtypes = [C.c_void_p, C.c_double, dp, dp, dp]
     L.cep_run_beats.argtypes = [C.c_void_p, C.c_int]
@@ -220,6 +224,10 @@ class CellBatch:
     # ---- integration
     def run_beats(self, nbeats: int):
         self._chk(self.L.cep_run_beats(self.h, nbeats))
+
+    def run_outputs(self, nout: int):
+        """Adams integrator: advance every cell by nout output points (blocking)."""
+        self._chk(self.L.cep_run_outputs(self.h, nout))
 
     def run_beats_async(self, nbeats: int):
         self._chk(self.L.cep_run_beats_async(self.h, nbeats))

@@ -9,6 +9,7 @@
 
 #include "../../include/cep_cellsim.h"
 #include "../../include/cep_tissue.h"
+#include "adams.hpp"
 #include "bdf_ord.hpp"
 #include "ord11_model.hpp"
 #include "tt06_model.hpp"
@@ -694,6 +695,55 @@ __global__ void __launch_bounds__(64) ord11_rl_kernel(O11Buffers B, int beats_en
   B.stats[c] += n;              // steps (one right-hand side each)
 }
 
+// ---- ORd 2011 with the Adams PECE solver of the hackathon driver
+// (CEPCell_HACKATHON/modules/cipa_drug_shago_bench.cpp:119-146): nout[c] further output points
+// of dt each, every one an ode() call continuing the integration; optional Vm trace of the
+// last trace_n outputs of the launch.
+template <typename R>
+__global__ void __launch_bounds__(64) ord11_adams_kernel(O11Buffers B, adams::Buffers A, int nbeats, int nout_fixed,
+                                                        double dt, double relerr, double abserr,
+                                                        double *vtrace, int tcap)
+{
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= B.ncell) return;
+  if (B.status[c] != ST_OK) return;
+  const size_t P_ = (size_t)B.pitch;
+  const PrmAcc P{B.prm + c, B.pitch};
+  auto f = [&](double t, const double *y, double *yp) {
+    ord11::Gates g;
+    ord11::Currents cur;
+    ord11::rhs<R>(t, y, yp, P, cur, g);
+  };
+  adams::Lane<ord11::NS, decltype(f)> L{A, (size_t)c, P_, f};
+  L.load();
+  double y[ord11::NS];
+#pragma unroll
+  for (int i = 0; i < ord11::NS; ++i) y[i] = B.y[i * P_ + c];
+  double t = A.sc[adams::S_T * P_ + c];
+  int iflag = A.si[adams::I_IFLAG * P_ + c];
+  // outputs of this launch: nbeats beats of the cell's own BCL, or a fixed count
+  const int nout = nout_fixed > 0 ? nout_fixed : nbeats * (int)llround(P(ord11::P_BCL) / dt);
+  const int trace_from = nout - tcap;
+  int st = ST_OK;
+  for (int o = 0; o < nout; ++o) {
+    const double tout = t + dt;
+    iflag = L.de(y, t, tout, relerr, abserr, iflag == 0 ? 1 : iflag);
+    if (iflag != 2) {
+      st = (iflag == 4 || iflag == 5 || iflag == -4 || iflag == -5) ? ST_MAX_STEPS : ST_ERR_FAIL;
+      break;
+    }
+    if (vtrace && o >= trace_from) vtrace[(size_t)(o - trace_from) * P_ + c] = y[0];
+  }
+  bool fin = true;
+#pragma unroll
+  for (int i = 0; i < ord11::NS; ++i) { B.y[i * P_ + c] = y[i]; fin = fin && isfinite(y[i]); }
+  if (st == ST_OK && !fin) st = ST_NONFINITE;
+  if (st != ST_OK) B.status[c] = st;
+  A.sc[adams::S_T * P_ + c] = t;
+  A.si[adams::I_IFLAG * P_ + c] = iflag;
+  L.store();
+}
+
 // computeRates of ORd 2011 for a batch (parity path): AoS states in, AoS rates and the
 // ALGEBRAIC current slots out
 constexpr int ORD11_NALG = 198;
@@ -986,6 +1036,9 @@ struct cep_batch {
   bool dutta = false;
   double time_point = 25.0;
   int beats_done = 0;
+  int integrator = CEP_INTEGRATOR_RUSH_LARSEN;
+  double *d_aphi = nullptr, *d_avec = nullptr, *d_acoef = nullptr, *d_asc = nullptr;
+  int *d_asi = nullptr;
   // device memory
   double *d_y = nullptr, *d_prm = nullptr, *d_bmd = nullptr, *d_ct = nullptr, *d_cv = nullptr;
   double *d_zn = nullptr, *d_ewt = nullptr, *d_acor = nullptr, *d_ft = nullptr, *d_jd = nullptr;
@@ -1104,7 +1157,7 @@ static cep_batch *batch_create(int model, int ncells, int precision, bool integr
     rc |= dalloc(b, &b->d_tcur, P);
     rc |= dalloc(b, &b->d_iout, P);
   } else if (model == CEP_MODEL_ORD2011) {
-    rc |= dalloc(b, &b->d_stats, P);
+    rc |= dalloc(b, &b->d_stats, 2 * P);    // steps, right-hand sides
   }
   if (rc || ensure_markov_table()) {
     cep_batch_destroy(b);
@@ -1192,6 +1245,8 @@ int cep_set_model_option(cep_batch *b, int option, double value)
   if (b->model != CEP_MODEL_ORD2011) return fail(b, CEP_E_ARG, "no options for this model");
   if (option == CEP_OPT_DUTTA) b->dutta = value != 0.0;
   else if (option == CEP_OPT_TIME_POINT && value >= 0.0) b->time_point = value;
+  else if (option == CEP_OPT_INTEGRATOR && (value == CEP_INTEGRATOR_RUSH_LARSEN || value == CEP_INTEGRATOR_ADAMS))
+    b->integrator = (int)value;
   else return fail(b, CEP_E_ARG, "unknown option or value");
   return 0;
 }
@@ -1207,8 +1262,13 @@ static double max_bcl(const cep_batch *b)
 
 static int alloc_trace_buffers(cep_batch *b)
 {
-  if (b->model == CEP_MODEL_ORD2011) {   // features need no trace buffers
+  if (b->model == CEP_MODEL_ORD2011) {   // no Ca trace; the Adams path may keep a Vm trace
     b->cap = 1;
+    const int beat_steps = (int)llround(max_bcl(b) / b->dt);
+    if (b->vtrace_on && b->integrator == CEP_INTEGRATOR_ADAMS && beat_steps > b->tcap) {
+      if (dalloc(b, &b->d_vtrace, (size_t)b->pitch * beat_steps)) return CEP_E_HIP;
+      b->tcap = beat_steps;
+    }
     return 0;
   }
   // Ca trace of one beat sampled every 1 ms (dt_write), sized for the longest BCL
@@ -1241,8 +1301,23 @@ static int reset_integrators(cep_batch *b)
     HIPCHK(b, hipMemsetAsync(b->d_sd, 0, P * ordbdf::SD_N * sizeof(double), b->stream));
     HIPCHK(b, hipMemsetAsync(b->d_stats, 0, P * 8 * sizeof(long long), b->stream));
   } else if (b->model == CEP_MODEL_ORD2011) {
-    HIPCHK(b, hipMemsetAsync(b->d_stats, 0, P * sizeof(long long), b->stream));
+    HIPCHK(b, hipMemsetAsync(b->d_stats, 0, 2 * P * sizeof(long long), b->stream));
     b->beats_done = 0;
+    if (b->integrator == CEP_INTEGRATOR_ADAMS) {
+      if (!b->d_aphi) {
+        int rc = dalloc(b, &b->d_aphi, P * adams::COLS * ord11::NS);
+        rc |= dalloc(b, &b->d_avec, P * 5 * ord11::NS);
+        rc |= dalloc(b, &b->d_acoef, P * adams::C_N);
+        rc |= dalloc(b, &b->d_asc, P * adams::S_N);
+        rc |= dalloc(b, &b->d_asi, P * adams::I_N);
+        if (rc) return rc;
+      }
+      HIPCHK(b, hipMemsetAsync(b->d_aphi, 0, P * adams::COLS * ord11::NS * sizeof(double), b->stream));
+      HIPCHK(b, hipMemsetAsync(b->d_avec, 0, P * 5 * ord11::NS * sizeof(double), b->stream));
+      HIPCHK(b, hipMemsetAsync(b->d_acoef, 0, P * adams::C_N * sizeof(double), b->stream));
+      HIPCHK(b, hipMemsetAsync(b->d_asc, 0, P * adams::S_N * sizeof(double), b->stream));
+      HIPCHK(b, hipMemsetAsync(b->d_asi, 0, P * adams::I_N * sizeof(int), b->stream));
+    }
   } else {
     HIPCHK(b, hipMemsetAsync(b->d_tcur, 0, P * sizeof(double), b->stream));
     HIPCHK(b, hipMemsetAsync(b->d_iout, 0, P * sizeof(int), b->stream));
@@ -1329,6 +1404,40 @@ int cep_compute_rates(cep_batch *b, double t, const double *states, double *rate
   return 0;
 }
 
+static int launch_ord11_adams(cep_batch *b, int nbeats, int nout_fixed)
+{
+  O11Buffers B;
+  B.ncell = b->n; B.pitch = b->pitch; B.nbeats_total = b->nbeats_total; B.nwin = b->nwin;
+  B.y = b->d_y; B.prm = b->d_prm; B.bmd = b->d_bmd; B.bmi = b->d_bmi; B.status = b->d_status;
+  B.stats = b->d_stats; B.time_point = b->time_point; B.max_dt = b->dt;
+  adams::Buffers A;
+  A.ncell = b->n; A.pitch = b->pitch; A.phi = b->d_aphi; A.vec = b->d_avec; A.coef = b->d_acoef;
+  A.sc = b->d_asc; A.si = b->d_asi; A.stats = b->d_stats;
+  double *vt = b->vtrace_on ? b->d_vtrace : nullptr;
+  const int blocks = (b->n + 63) / 64;
+  if (b->prec == CEP_PREC_FP64)
+    hipLaunchKernelGGL(ord11_adams_kernel<double>, dim3(blocks), dim3(64), 0, b->stream, B, A, nbeats, nout_fixed,
+                       b->dt, b->rtol, b->atol, vt, vt ? b->tcap : 0);
+  else
+    hipLaunchKernelGGL(ord11_adams_kernel<float>, dim3(blocks), dim3(64), 0, b->stream, B, A, nbeats, nout_fixed,
+                       b->dt, b->rtol, b->atol, vt, vt ? b->tcap : 0);
+  return 0;
+}
+
+int cep_run_outputs(cep_batch *b, int nout)
+{
+  if (!b || nout < 1) return fail(b, CEP_E_ARG, "nout");
+  if (b->model != CEP_MODEL_ORD2011 || b->integrator != CEP_INTEGRATOR_ADAMS)
+    return fail(b, CEP_E_STATE, "output-step runs are the Adams integrator's");
+  if (b->cap == 0) return fail(b, CEP_E_STATE, "cep_init_consts not called");
+  HIPCHK(b, hipEventRecord(b->ev0, b->stream));
+  launch_ord11_adams(b, 0, nout);
+  HIPCHK(b, hipGetLastError());
+  HIPCHK(b, hipEventRecord(b->ev1, b->stream));
+  b->pending = true;
+  return cep_sync(b);
+}
+
 int cep_run_beats_async(cep_batch *b, int nbeats)
 {
   if (!b || nbeats < 1) return fail(b, CEP_E_ARG, "nbeats");
@@ -1355,6 +1464,9 @@ int cep_run_beats_async(cep_batch *b, int nbeats)
       hipLaunchKernelGGL((ord_bdf_kernel<double, CPW>), dim3(blocks), dim3(64), 0, b->stream, B, rc, nbeats, max_passes);
     else
       hipLaunchKernelGGL((ord_bdf_kernel<float, CPW>), dim3(blocks), dim3(64), 0, b->stream, B, rc, nbeats, max_passes);
+  } else if (b->model == CEP_MODEL_ORD2011 && b->integrator == CEP_INTEGRATOR_ADAMS) {
+    launch_ord11_adams(b, nbeats, 0);
+    b->beats_done += nbeats;
   } else if (b->model == CEP_MODEL_ORD2011) {
     O11Buffers B;
     B.ncell = b->n; B.pitch = b->pitch; B.nbeats_total = b->nbeats_total; B.nwin = b->nwin;
@@ -1510,12 +1622,16 @@ int cep_get_stats(const cep_batch *b, long long *out)
   if (!b || !out) return CEP_E_ARG;
   if (!b->d_stats) { memset(out, 0, sizeof(long long) * 8 * b->n); return 0; }
   if (b->model == CEP_MODEL_ORD2011) {   // steps (one right-hand side each) in slots 0 and 1
-    std::vector<long long> tmp((size_t)b->pitch);
+    std::vector<long long> tmp((size_t)b->pitch * 2);
     if (hipMemcpyAsync(tmp.data(), b->d_stats, tmp.size() * sizeof(long long), hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
         hipStreamSynchronize(b->stream) != hipSuccess)
       return CEP_E_HIP;
     memset(out, 0, sizeof(long long) * 8 * b->n);
-    for (int i = 0; i < b->n; ++i) out[(size_t)i * 8] = out[(size_t)i * 8 + 1] = tmp[i];
+    const bool adams_path = b->integrator == CEP_INTEGRATOR_ADAMS;
+    for (int i = 0; i < b->n; ++i) {
+      out[(size_t)i * 8] = tmp[i];
+      out[(size_t)i * 8 + 1] = adams_path ? tmp[b->pitch + i] : tmp[i];
+    }
     return 0;
   }
   std::vector<long long> tmp((size_t)b->pitch * 8);

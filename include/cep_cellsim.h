@@ -120,6 +120,13 @@ int cep_set_cells(cep_batch *b, int first, int count, const double *celltype, co
    Call before cep_init_consts. */
 #define CEP_OPT_DUTTA 1
 #define CEP_OPT_TIME_POINT 2
+/* CEP_OPT_INTEGRATOR (ORd 2011): CEP_INTEGRATOR_RUSH_LARSEN (the DrugSimulation loop, default)
+   or CEP_INTEGRATOR_ADAMS: the Shampine-Gordon Adams PECE solver of libs/ode.cpp driven as
+   CEPCell_HACKATHON/modules/cipa_drug_shago_bench.cpp:119-146 does (ode() to every output point
+   dt, relerr = rtol, abserr = atol of cep_set_run_config; no features, optional Vm trace). */
+#define CEP_OPT_INTEGRATOR 3
+#define CEP_INTEGRATOR_RUSH_LARSEN 0
+#define CEP_INTEGRATOR_ADAMS 1
 int cep_set_model_option(cep_batch *b, int option, double value);
 
 /* Integration / biomarker configuration shared by the batch.
@@ -146,6 +153,9 @@ int cep_compute_rates(cep_batch *b, double t, const double *states, double *rate
    updated on the device).  Blocks until done.  The _async form returns after the launch on
    the batch stream; cep_sync waits for it. */
 int cep_run_beats(cep_batch *b, int nbeats);
+/* Adams integrator only: advance every cell by nout output points (the reference loop's
+   ode() calls, CEPCell_HACKATHON/modules/cipa_drug_shago_bench.cpp:132-140).  Blocks. */
+int cep_run_outputs(cep_batch *b, int nout);
 int cep_run_beats_async(cep_batch *b, int nbeats);
 int cep_sync(cep_batch *b);
 /* Non-blocking completion test of the last cep_run_beats_async: 1 while it runs, 0 when
