@@ -49,6 +49,8 @@ EXPORTED = (
     "cep_tissue_set_conductivity", "cep_tissue_set_stimulus", "cep_tissue_step", "cep_tissue_diffuse",
     "cep_tissue_get_vm", "cep_tissue_get_states", "cep_tissue_set_states", "cep_tissue_time",
     "cep_tissue_kernel_times",
+    "cep_read_ic50_csv", "cep_read_states_file", "cep_write_states_file", "cep_param_defaults",
+    "cep_read_input_deck", "cep_param_concentrations", "cep_write_cipa_features", "cep_write_ord11_qni",
 )
 SPLIT_GODUNOV = 0
 SPLIT_STRANG = 1

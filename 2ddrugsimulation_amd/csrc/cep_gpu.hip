@@ -1673,3 +1673,5 @@ int cep_kernel_time(cep_batch *b, double *avg_ms, double *total_ms)
 }  // extern "C"
 
 #include "tissue.hpp"
+
+#include "io.hpp"
