@@ -45,7 +45,7 @@ EXPORTED = (
     "cep_run_beats", "cep_run_beats_async", "cep_sync", "cep_get_states", "cep_set_states",
     "cep_get_biomarkers", "cep_get_status", "cep_get_stats", "cep_enable_vm_trace",
     "cep_get_vm_trace", "cep_kernel_time", "cep_query", "cep_batch_device_bytes",
-    "cep_tissue_create", "cep_tissue_destroy", "cep_tissue_last_error", "cep_tissue_set_cells",
+    "cep_tissue_create", "cep_tissue_create_model", "cep_tissue_destroy", "cep_tissue_last_error", "cep_tissue_set_cells",
     "cep_tissue_set_conductivity", "cep_tissue_set_stimulus", "cep_tissue_step", "cep_tissue_diffuse",
     "cep_tissue_get_vm", "cep_tissue_get_states", "cep_tissue_set_states", "cep_tissue_time",
     "cep_tissue_kernel_times",
@@ -104,6 +104,8 @@ def library(path: str | None = None):
     L.cep_batch_device_bytes.restype = C.c_longlong
     L.cep_tissue_create.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int]
     L.cep_tissue_create.restype = C.c_void_p
+    L.cep_tissue_create_model.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int]
+    L.cep_tissue_create_model.restype = C.c_void_p
     L.cep_tissue_destroy.argtypes = [C.c_void_p]
     L.cep_tissue_destroy.restype = None
     L.cep_tissue_last_error.argtypes = [C.c_void_p]
@@ -286,14 +288,17 @@ class Tissue:
     hx x hy mm): the reference's Cheart reaction step + Cfem diffusion step (heart.cpp,
     fem.cpp) on the device."""
 
-    def __init__(self, nx: int, ny: int, hx: float, hy: float, precision: int = PREC_FP64):
+    def __init__(self, nx: int, ny: int, hx: float, hy: float, precision: int = PREC_FP64,
+                 model: int = MODEL_ORD_CIPA):
         self.L = library()
-        h = self.L.cep_tissue_create(nx, ny, hx, hy, precision)
+        h = self.L.cep_tissue_create_model(nx, ny, hx, hy, model, precision)
         if not h:
             raise CepError(f"cep_tissue_create({nx}, {ny}, {hx}, {hy}) failed")
         self.h = h
         self.nx, self.ny = nx, ny
         self.nn = (nx + 1) * (ny + 1)
+        self.model = model
+        self.nstates = 41 if model == MODEL_ORD2011 else 52
 
     def _chk(self, rc):
         if rc != 0:
@@ -320,9 +325,10 @@ class Tissue:
         n = self.nn
         ct = np.ascontiguousarray(np.broadcast_to(np.asarray(celltype, dtype=np.float64), (n,)))
         cc = np.ascontiguousarray(np.broadcast_to(np.asarray(conc, dtype=np.float64), (n,)))
-        hg = None if herg is None else np.ascontiguousarray(np.broadcast_to(np.asarray(herg, np.float64), (n, 19)))
+        w = 14 if self.model == MODEL_ORD2011 else 19
+        hg = None if herg is None else np.ascontiguousarray(np.broadcast_to(np.asarray(herg, np.float64), (n, w)))
         gs = None if gscale is None else np.ascontiguousarray(np.broadcast_to(np.asarray(gscale, np.float64), (n, 7)))
-        self._chk(self.L.cep_tissue_set_cells(self.h, _p(ct), _p(cc), _p(hg), 19, _p(gs), 7))
+        self._chk(self.L.cep_tissue_set_cells(self.h, _p(ct), _p(cc), _p(hg), w, _p(gs), 7))
 
     def set_conductivity(self, sxx, sxy, syy):
         ne = self.nx * self.ny
@@ -352,7 +358,7 @@ class Tissue:
         return out
 
     def states(self) -> np.ndarray:
-        out = np.empty((self.nn, 52))
+        out = np.empty((self.nn, self.nstates))
         self._chk(self.L.cep_tissue_get_states(self.h, _p(out)))
         return out
 

@@ -235,10 +235,32 @@ __global__ void __launch_bounds__(64) tissue_react(int n, int pitch, double t, d
   }
 }
 
+// reaction step of every ORd 2011 node, the reference tissue driver's own scheme
+// (Cheart::cellular_reaction, heart.cpp:341-368): computeRates with the TISSUE stimulus
+// (isS1 ? amp : 0; unpaced nodes carry amp = 0) then solveAnalytical(h).
+template <typename R>
+__global__ void __launch_bounds__(64) tissue_react11(int n, int pitch, double t, double h, int stim_on,
+                                                     double *__restrict__ y, const double *__restrict__ prm)
+{
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  double s[ord11::NS], f[ord11::NS];
+#pragma unroll
+  for (int i = 0; i < ord11::NS; ++i) s[i] = y[(size_t)i * pitch + c];
+  const PrmAcc P{prm + c, pitch};
+  ord11::Gates g;
+  ord11::Currents cur;
+  ord11::rhs<R>(t, s, f, P, cur, g, stim_on);
+  ord11::rl_step<R>(h, s, f, g);
+#pragma unroll
+  for (int i = 0; i < ord11::NS; ++i) y[(size_t)i * pitch + c] = s[i];
+}
+
 }  // namespace tis
 }  // namespace cep
 
 struct cep_tissue {
+  int model = CEP_MODEL_ORD_CIPA;
   cep_batch *cells = nullptr;
   int nx = 0, ny = 0, nxn = 0, nyn = 0, nn = 0;
   double hx = 0, hy = 0, t = 0, dt_built = -1;
@@ -402,6 +424,21 @@ static int tissue_react_launch(cep_tissue *t, double h)
   cep_batch *b = t->cells;
   const int blocks = (t->nn + 63) / 64;
   tissue_timed_begin(t, 0);
+  if (t->model == CEP_MODEL_ORD2011) {
+    // paced nodes are stimulated while the beat time lies in [start, start + duration]
+    // (heart.cpp:287-297); unpaced nodes carry a zero amplitude
+    const double tb = t->t - floor(t->t / t->stim_bcl) * t->stim_bcl;
+    const int on = (tb >= t->stim_start && tb <= t->stim_start + t->stim_dur) ? 1 : 0;
+    if (b->prec == CEP_PREC_FP64)
+      hipLaunchKernelGGL(cep::tis::tissue_react11<double>, dim3(blocks), dim3(64), 0, b->stream, t->nn, b->pitch, t->t,
+                         h, on, b->d_y, b->d_prm);
+    else
+      hipLaunchKernelGGL(cep::tis::tissue_react11<float>, dim3(blocks), dim3(64), 0, b->stream, t->nn, b->pitch, t->t,
+                         h, on, b->d_y, b->d_prm);
+    tissue_timed_end(t, 0);
+    TCHK(t, hipGetLastError());
+    return 0;
+  }
   if (b->prec == CEP_PREC_FP64)
     hipLaunchKernelGGL(cep::tis::tissue_react<double>, dim3(blocks), dim3(64), 0, b->stream, t->nn, b->pitch, t->t, h,
                        b->d_y, b->d_prm);
@@ -423,6 +460,12 @@ static int tissue_upload_params(cep_tissue *t)
   for (int k : t->stim_nodes) paced[k] = 1;
   double p[64];
   for (int i = 0; i < t->nn; ++i) {
+    if (t->model == CEP_MODEL_ORD2011) {
+      cep::host::o11_params(b->o11_in[i], b->dutta, paced[i] ? t->stim_scale : 0.0, p);
+      p[cep::ord11::P_BCL] = t->stim_bcl;
+      for (int k = 0; k < b->np; ++k) prm[k * P + i] = p[k];
+      continue;
+    }
     cep::host::ord_params(b->ord_in[i], paced[i] ? t->stim_scale : 0.0, p);
     p[cep::ord::OP_BCL] = t->stim_bcl;
     p[cep::ord::OP_STIM_START] = t->stim_start;
@@ -438,12 +481,19 @@ extern "C" {
 
 cep_tissue *cep_tissue_create(int nx, int ny, double hx, double hy, int precision)
 {
+  return cep_tissue_create_model(nx, ny, hx, hy, CEP_MODEL_ORD_CIPA, precision);
+}
+
+cep_tissue *cep_tissue_create_model(int nx, int ny, double hx, double hy, int model, int precision)
+{
   if (nx < 1 || ny < 1 || !(hx > 0) || !(hy > 0)) return nullptr;
+  if (model != CEP_MODEL_ORD_CIPA && model != CEP_MODEL_ORD2011) return nullptr;
   const long long nn = (long long)(nx + 1) * (ny + 1);
   if (nn > 50000000LL) return nullptr;
   cep_tissue *t = new cep_tissue();
   t->nx = nx; t->ny = ny; t->nxn = nx + 1; t->nyn = ny + 1; t->nn = (int)nn; t->hx = hx; t->hy = hy;
-  t->cells = batch_create(CEP_MODEL_ORD_CIPA, t->nn, precision, false);   // no BDF workspace
+  t->model = model;
+  t->cells = batch_create(model, t->nn, precision, false);   // no BDF workspace
   if (!t->cells) { delete t; return nullptr; }
   t->sxx.assign((size_t)nx * ny, 0.0);
   t->sxy.assign((size_t)nx * ny, 0.0);
@@ -497,6 +547,10 @@ int cep_tissue_set_cells(cep_tissue *t, const double *celltype, const double *co
   const size_t P = b->pitch;
   std::vector<double> y(P * b->ns, 0.0);
   for (int i = 0; i < t->nn; ++i) {
+    if (t->model == CEP_MODEL_ORD2011) {
+      for (int k = 0; k < cep::ord11::NS; ++k) y[k * P + i] = cep::host::kO11Init[k];
+      continue;
+    }
     for (int k = 0; k < 49; ++k) y[k * P + i] = cep::host::kOrdInitStates[k];
     y[43 * P + i] = b->ord_in[i].conc;
   }

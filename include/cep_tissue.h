@@ -11,7 +11,8 @@
  * The mesh here is a structured nx x ny grid of hx x hy elements (nodes (nx+1) x (ny+1),
  * node (ix, iy) has index iy*(nx+1) + ix) with a per-element diffusion tensor, so the
  * global matrix is a 9-point stencil per node stored as nine coefficient planes.
- * Each node carries an ORd-CiPA cell (the population batch's cell parameters and states).
+ * Each node carries an ORd-CiPA cell (the population batch's cell parameters and states), or an
+ * ORd 2011 cell (cep_tissue_create_model).
  */
 #ifndef CEP_TISSUE_H
 #define CEP_TISSUE_H
@@ -27,6 +28,12 @@ typedef struct cep_tissue cep_tissue;
 /* Grid of nx x ny elements of size hx x hy (mm); every node an ORd-CiPA cell with the
    given precision of its ionic model (CEP_PREC_*).  Diffusion is always FP64. */
 cep_tissue *cep_tissue_create(int nx, int ny, double hx, double hy, int precision);
+/* The same with the node model chosen: CEP_MODEL_ORD_CIPA, or CEP_MODEL_ORD2011, the model the
+   reference's tissue driver uses (heart.h:10-14) with its own reaction scheme: computeRates
+   with the TISSUE stimulus (isS1 ? amp : 0) and solveAnalytical(dt) (heart.cpp:341-368);
+   paced nodes are stimulated while the beat time lies in [start, start + duration]
+   (heart.cpp:287-297).  ORd 2011 cells take 14-value IC50 rows and no population scales. */
+cep_tissue *cep_tissue_create_model(int nx, int ny, double hx, double hy, int model, int precision);
 void cep_tissue_destroy(cep_tissue *t);
 const char *cep_tissue_last_error(const cep_tissue *t);
 
@@ -57,8 +64,8 @@ int cep_tissue_diffuse(cep_tissue *t, double dt, const double *v, double *x, dou
                        int *iters);
 
 int cep_tissue_get_vm(const cep_tissue *t, double *out);              /* nnodes */
-int cep_tissue_get_states(const cep_tissue *t, double *out);          /* nnodes x 52 */
-int cep_tissue_set_states(cep_tissue *t, const double *in);           /* nnodes x 52 */
+int cep_tissue_get_states(const cep_tissue *t, double *out);          /* nnodes x nstates (52 / 41) */
+int cep_tissue_set_states(cep_tissue *t, const double *in);           /* nnodes x nstates (52 / 41) */
 /* simulated time (ms) */
 double cep_tissue_time(const cep_tissue *t);
 /* Average kernel time (ms) per call of the reaction kernel, the SpMV and the CG update

@@ -161,3 +161,29 @@ def test_tissue_wave_propagates_from_the_paced_corner():
         assert np.isfinite(tis.vm()).all()
     assert not np.isnan(act).any(), act
     assert np.all(np.diff(act) > 0), act
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec,rtol,atol", [(0, 1e-11, 1e-13), (1, 1e-5, 1e-6)])
+def test_tissue_ord2011_reaction_is_the_reference_scheme(prec, rtol, atol):
+    """ORd 2011 nodes: one step with zero conductivity is computeRates with the tissue
+    stimulus (paced nodes inside the window) followed by solveAnalytical, node by node."""
+    nx, ny, dt = 4, 3, 0.01
+    ic50 = np.array([5000, 1.0, 3000, 1.0, 20000, 1.0, 800, 1.0, 10000, 1.0, 4000, 1.0, 900, 0.8])
+    with cep.Tissue(nx, ny, 0.1, 0.1, prec, model=cep.MODEL_ORD2011) as tis:
+        tis.set_cells(celltype=0.0, conc=300.0, herg=ic50)
+        tis.set_conductivity(0.0, 0.0, 0.0)
+        paced = [0, 1, 5]
+        tis.set_stimulus(paced, amp_scale=1.0, bcl=1000.0, start=0.0, duration=1.0)
+        y0 = tis.states()
+        y0[:, 0] = np.linspace(-88, 40, tis.nn)
+        tis.set_states(y0)
+        tis.step(1, dt, cg_rtol=1e-14)
+        y1 = tis.states()
+    Cc, _ = O.ord11_init(0.0, 300.0, ic50)
+    Cc[140] = 1000.0
+    for i in range(tis.nn):
+        R, A = O.ord11_rates(0.0, Cc, y0[i], tissue_stim=1 if i in paced else 0)
+        ref = y0[i].copy()
+        O.lib().ord11_rush_larsen(dt, O.ptr(Cc), O.ptr(R), O.ptr(ref), O.ptr(A))
+        np.testing.assert_allclose(y1[i], ref, rtol=rtol, atol=atol)
