@@ -236,6 +236,10 @@ struct Lane {
   // load/store interleaving would serialise every load on memory latency.
   static constexpr int CH = 13;
   static_assert(N % CH == 0, "whole chunks");
+  // sweeps over the whole Nordsieck array hold six rows per component: smaller chunks,
+  // whose loads still issue together, keep them in registers
+  static constexpr int CHZ = 4;
+  static_assert(N % CHZ == 0, "whole chunks");
 
   // CVODE's weights from zn[0]: 1 / (rtol |y| + atol)
   CEP_D void set_ewt()
@@ -309,14 +313,15 @@ struct Lane {
       for (int r = 0; r <= QMAX; ++r) M[r][col] = zz[r];
     }
 #pragma unroll
-    for (int c0 = 0; c0 < N; c0 += CH) {
-      double z[CH][QMAX + 1];
+    for (int c0 = 0; c0 < N; c0 += CHZ) {
+      double z[CHZ][QMAX + 1];
 #pragma unroll
-      for (int k = 0; k < CH; ++k)
+      for (int k = 0; k < CHZ; ++k)
 #pragma unroll
         for (int j = 0; j <= QMAX; ++j) z[k][j] = ZN(j, c0 + k);
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-      for (int k = 0; k < CH; ++k) {
+      for (int k = 0; k < CHZ; ++k) {
         double w[QMAX + 1];
 #pragma unroll
         for (int r = 0; r <= QMAX; ++r) {
@@ -330,7 +335,7 @@ struct Lane {
         y[c0 + k] = w[0];
       }
 #pragma unroll
-      for (int k = 0; k < CH; ++k)
+      for (int k = 0; k < CHZ; ++k)
 #pragma unroll
         for (int j = 0; j <= QMAX; ++j) ZN(j, c0 + k) = z[k][j];
     }
@@ -543,68 +548,186 @@ struct Lane {
     return ok;
   }
 
-  // solve M x = b in place (b in registers), then CVODE's 2/(1+gamrat) scaling
-  CEP_D void solve(double *b) const
+  // K consecutive elements of a blocked vector, loaded together: the scheduling barrier
+  // keeps the compiler from sinking each load to its first use, which would put one memory
+  // round trip per element on the critical path.
+  template <int K>
+  CEP_D void fetch(double (&r)[K], Rsrc rs, int k0) const
+  {
+#pragma unroll
+    for (int k = 0; k < K; ++k) r[k] = blk(rs, k0 + k);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  // the Jacobian entries J(row, col) of the solve's coupling terms for one row, in the order
+  // the solve uses them (simple gates, nca, O); returns their number
+  CEP_D int fetch_couplings(double (&r)[34], int row, bool with_core) const
+  {
+    int n = 0;
+    if (with_core) {
+#pragma unroll
+      for (int a = 0; a < 12; ++a) r[n++] = J(row, kCore[a]);
+    }
+#pragma unroll
+    for (int k = 0; k < NG; ++k)
+      if (couples(row, kSimpleGates[k])) r[n++] = J(row, kSimpleGates[k]);
+    if (couples(row, 33)) r[n++] = J(row, 33);
+    if (couples(row, 38)) r[n++] = J(row, 38);
+    __builtin_amdgcn_sched_barrier(0);
+    return n;
+  }
+
+  // solve M x = b in place (b: a lane vector, here the LDS staging vector), then CVODE's
+  // 2/(1+gamrat) scaling.  Factors and Jacobian entries are fetched in batches.
+  template <typename BV>
+  CEP_D void solve(const BV &b) const
   {
     const double g = m.gammap;
-    double beta[NG];
+    double beta[NG], beta_n;
+    {
+      double rg[NG], nc[2];
+      fetch(rg, V.fac, F_RG);
+      fetch(nc, V.fac, F_NCA);
 #pragma unroll
-    for (int k = 0; k < NG; ++k) beta[k] = FAC(F_RG + k) * b[kSimpleGates[k]];
-    const double beta_n = FAC(F_NCA + 0) * b[33] - FAC(F_NCA + 1) * beta[20];
+      for (int k = 0; k < NG; ++k) beta[k] = rg[k] * b[kSimpleGates[k]];
+      beta_n = nc[0] * b[33] - nc[1] * beta[20];
+    }
     double bm[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) bm[i] = b[34 + i];
+    {
+      // Markov LU: rows 1-8 of L, then rows 8..0 of U (with the diagonal)
+      double lo[36];
+      int n = 0;
 #pragma unroll
-    for (int i = 0; i < 9; ++i)
+      for (int i = 1; i < 9; ++i)
 #pragma unroll
-      for (int j = 0; j < i; ++j) bm[i] -= FAC(F_MLU + i * 9 + j) * bm[j];
+        for (int j = 0; j < i; ++j) lo[n++] = FAC(F_MLU + i * 9 + j);
+      __builtin_amdgcn_sched_barrier(0);
+      n = 0;
 #pragma unroll
-    for (int i = 8; i >= 0; --i) {
+      for (int i = 1; i < 9; ++i)
 #pragma unroll
-      for (int j = i + 1; j < 9; ++j) bm[i] -= FAC(F_MLU + i * 9 + j) * bm[j];
-      bm[i] /= FAC(F_MLU + i * 9 + i);
+        for (int j = 0; j < i; ++j) bm[i] -= lo[n++] * bm[j];
+    }
+    {
+      double up[45];
+      int n = 0;
+#pragma unroll
+      for (int i = 8; i >= 0; --i)
+#pragma unroll
+        for (int j = i; j < 9; ++j) up[n++] = FAC(F_MLU + i * 9 + j);
+      __builtin_amdgcn_sched_barrier(0);
+      n = 0;
+#pragma unroll
+      for (int i = 8; i >= 0; --i) {
+        const double d = up[n++];
+#pragma unroll
+        for (int j = i + 1; j < 9; ++j) bm[i] -= up[n++] * bm[j];
+        bm[i] /= d;
+      }
     }
     double bk[12];
 #pragma unroll
     for (int a = 0; a < 12; ++a) {
       const int row = kCore[a];
       double s = b[row];
+      double jr[34];
+      const int nj = fetch_couplings(jr, row, false);
+      (void)nj;
+      int n = 0;
 #pragma unroll
       for (int k = 0; k < NG; ++k)
-        if (couples(row, kSimpleGates[k])) s += g * J(row, kSimpleGates[k]) * beta[k];
-      if (couples(row, 33)) s += g * J(row, 33) * beta_n;
-      if (couples(row, 38)) s += g * J(row, 38) * bm[4];
+        if (couples(row, kSimpleGates[k])) s += g * jr[n++] * beta[k];
+      if (couples(row, 33)) s += g * jr[n++] * beta_n;
+      if (couples(row, 38)) s += g * jr[n++] * bm[4];
       bk[a] = s;
     }
+    {
+      // core LU: rows 1-11 of L in two batches, then U rows 11..0 in two
+      double lo[38];
+      int n = 0;
 #pragma unroll
-    for (int i = 0; i < 12; ++i)
+      for (int i = 1; i < 8; ++i)
 #pragma unroll
-      for (int j = 0; j < i; ++j) bk[i] -= FAC(F_SLU + i * 12 + j) * bk[j];
+        for (int j = 0; j < i; ++j) lo[n++] = FAC(F_SLU + i * 12 + j);
+      __builtin_amdgcn_sched_barrier(0);
+      n = 0;
 #pragma unroll
-    for (int i = 11; i >= 0; --i) {
+      for (int i = 1; i < 8; ++i)
 #pragma unroll
-      for (int j = i + 1; j < 12; ++j) bk[i] -= FAC(F_SLU + i * 12 + j) * bk[j];
-      bk[i] /= FAC(F_SLU + i * 12 + i);
+        for (int j = 0; j < i; ++j) bk[i] -= lo[n++] * bk[j];
+      n = 0;
+#pragma unroll
+      for (int i = 8; i < 12; ++i)
+#pragma unroll
+        for (int j = 0; j < i; ++j) lo[n++] = FAC(F_SLU + i * 12 + j);
+      __builtin_amdgcn_sched_barrier(0);
+      n = 0;
+#pragma unroll
+      for (int i = 8; i < 12; ++i)
+#pragma unroll
+        for (int j = 0; j < i; ++j) bk[i] -= lo[n++] * bk[j];
+    }
+    {
+      double up[42];
+      int n = 0;
+#pragma unroll
+      for (int i = 11; i >= 4; --i)
+#pragma unroll
+        for (int j = i; j < 12; ++j) up[n++] = FAC(F_SLU + i * 12 + j);
+      __builtin_amdgcn_sched_barrier(0);
+      n = 0;
+#pragma unroll
+      for (int i = 11; i >= 4; --i) {
+        const double d = up[n++];
+#pragma unroll
+        for (int j = i + 1; j < 12; ++j) bk[i] -= up[n++] * bk[j];
+        bk[i] /= d;
+      }
+      n = 0;
+#pragma unroll
+      for (int i = 3; i >= 0; --i)
+#pragma unroll
+        for (int j = i; j < 12; ++j) up[n++] = FAC(F_SLU + i * 12 + j);
+      __builtin_amdgcn_sched_barrier(0);
+      n = 0;
+#pragma unroll
+      for (int i = 3; i >= 0; --i) {
+        const double d = up[n++];
+#pragma unroll
+        for (int j = i + 1; j < 12; ++j) bk[i] -= up[n++] * bk[j];
+        bk[i] /= d;
+      }
     }
     const double xV = bk[CORE_V], xC = bk[CORE_CASS];
 #pragma unroll
     for (int a = 0; a < 12; ++a) b[kCore[a]] = bk[a];
+    {
+      double cg[NG], nc[2], w[9];
+      fetch(cg, V.fac, F_CG);
+      fetch(nc, V.fac, F_NCA + 2);
+      fetch(w, V.fac, F_W);
 #pragma unroll
-    for (int k = 0; k < NG; ++k) b[kSimpleGates[k]] = beta[k] - FAC(F_CG + k) * xV;
-    b[33] = beta_n - FAC(F_NCA + 2) * xV - FAC(F_NCA + 3) * xC;
+      for (int k = 0; k < NG; ++k) b[kSimpleGates[k]] = beta[k] - cg[k] * xV;
+      b[33] = beta_n - nc[0] * xV - nc[1] * xC;
 #pragma unroll
-    for (int i = 0; i < 9; ++i) b[34 + i] = bm[i] - FAC(F_W + i) * xV;
+      for (int i = 0; i < 9; ++i) b[34 + i] = bm[i] - w[i] * xV;
+    }
     // accumulators: x = b + gamma * J(acc, :) x
 #pragma unroll
     for (int r = 49; r <= 51; ++r) {
+      double jr[34];
+      const int nj = fetch_couplings(jr, r, true);
+      (void)nj;
+      int n = 0;
       double s = 0.0;
 #pragma unroll
-      for (int a = 0; a < 12; ++a) s += J(r, kCore[a]) * b[kCore[a]];
+      for (int a = 0; a < 12; ++a) s += jr[n++] * b[kCore[a]];
 #pragma unroll
       for (int k = 0; k < NG; ++k)
-        if (couples(r, kSimpleGates[k])) s += J(r, kSimpleGates[k]) * b[kSimpleGates[k]];
-      if (couples(r, 33)) s += J(r, 33) * b[33];
-      if (couples(r, 38)) s += J(r, 38) * b[38];
+        if (couples(r, kSimpleGates[k])) s += jr[n++] * b[kSimpleGates[k]];
+      if (couples(r, 33)) s += jr[n++] * b[33];
+      if (couples(r, 38)) s += jr[n++] * b[38];
       b[r] += g * s;
     }
     if (m.gamrat != 1.0) {
@@ -614,21 +737,28 @@ struct Lane {
     }
   }
 
-  // One modified-Newton iteration with f(tn, y_iterate) in FT; writes the next iterate
-  // into `y`.  Returns NLS_OK, NLS_ITER (evaluate f(y) and call again), NLS_TRY_AGAIN or
-  // NLS_CONV_FAIL (CVODE's cvNewtonIteration).
+  // One modified-Newton iteration.  `y` (the lane's staging vector) holds f(tn, y_iterate)
+  // on entry; the Newton right-hand side and the solve run in place there, and on exit it
+  // holds the next iterate.  Returns NLS_OK, NLS_ITER (evaluate f(y) and call again),
+  // NLS_TRY_AGAIN or NLS_CONV_FAIL (CVODE's cvNewtonIteration).
   template <typename YV>
   CEP_D int newton(const YV &y)
   {
     const int mm = m.mnewt;
     const double gm = m.gamma, r1 = m.rl1;
-    double b[N];
 #pragma unroll
-    for (int i = 0; i < N; ++i) {
-      const double a = mm > 0 ? ACOR(i) : 0.0;
-      b[i] = gm * FT(i) - (r1 * ZN(1, i) + a);
+    for (int c0 = 0; c0 < N; c0 += CH) {
+      double z1[CH], a[CH];
+      fetch(z1, V.zn, N + c0);
+      if (mm > 0) fetch(a, V.acor, c0);
+      else {
+#pragma unroll
+        for (int k = 0; k < CH; ++k) a[k] = 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < CH; ++k) y[c0 + k] = gm * y[c0 + k] - (r1 * z1[k] + a[k]);
     }
-    solve(b);
+    solve(y);
     m.nni++;
     double s2 = 0.0, sa = 0.0;
 #pragma unroll
@@ -640,12 +770,14 @@ struct Lane {
         z[k] = ZN(0, c0 + k);
         w[k] = EWT(c0 + k);
       }
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int k = 0; k < CH; ++k) {
         const int i = c0 + k;
-        a[k] += b[i];
+        const double bi = y[i];
+        a[k] += bi;
         y[i] = z[k] + a[k];
-        const double t = b[i] * w[k], u = a[k] * w[k];
+        const double t = bi * w[k], u = a[k] * w[k];
         s2 += t * t;
         sa += u * u;
       }
@@ -694,17 +826,18 @@ struct Lane {
     const double rt = B.rtol, at_ = B.atol;
     double s2 = 0.0, s3 = 0.0;
 #pragma unroll
-    for (int c0 = 0; c0 < N; c0 += CH) {
-      double z[CH][QMAX + 1], a[CH], w[CH];
+    for (int c0 = 0; c0 < N; c0 += CHZ) {
+      double z[CHZ][QMAX + 1], a[CHZ], w[CHZ];
 #pragma unroll
-      for (int k = 0; k < CH; ++k) {
+      for (int k = 0; k < CHZ; ++k) {
 #pragma unroll
         for (int j = 0; j <= QMAX; ++j) z[k][j] = ZN(j, c0 + k);
         a[k] = ACOR(c0 + k);
         w[k] = EWT(c0 + k);
       }
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-      for (int k = 0; k < CH; ++k) {
+      for (int k = 0; k < CHZ; ++k) {
         const double zqm = z[k][QMAX];
         double zq = 0.0;
 #pragma unroll
@@ -718,7 +851,7 @@ struct Lane {
         w[k] = 1.0 / (rt * fabs(z[k][0]) + at_);
       }
 #pragma unroll
-      for (int k = 0; k < CH; ++k) {
+      for (int k = 0; k < CHZ; ++k) {
 #pragma unroll
         for (int j = 0; j <= QMAX; ++j) ZN(j, c0 + k) = z[k][j];
         EWT(c0 + k) = w[k];
@@ -796,14 +929,15 @@ struct Lane {
     const double s = (t - m.tn) / m.h;
     const bool q5 = m.q == QMAX;     // below order 5 row QMAX holds the saved correction
 #pragma unroll
-    for (int c0 = 0; c0 < N; c0 += CH) {
-      double z[CH][QMAX + 1];
+    for (int c0 = 0; c0 < N; c0 += CHZ) {
+      double z[CHZ][QMAX + 1];
 #pragma unroll
-      for (int k = 0; k < CH; ++k)
+      for (int k = 0; k < CHZ; ++k)
 #pragma unroll
         for (int j = 0; j <= QMAX; ++j) z[k][j] = ZN(j, c0 + k);
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-      for (int k = 0; k < CH; ++k) {
+      for (int k = 0; k < CHZ; ++k) {
         double v = q5 ? z[k][QMAX] : 0.0;
 #pragma unroll
         for (int j = QMAX - 1; j >= 0; --j) v = z[k][j] + s * v;   // rows above q are zero

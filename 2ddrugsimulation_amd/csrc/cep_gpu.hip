@@ -206,6 +206,10 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
       if (pass >= max_passes && ph != PH_DONE) { m.status = ST_MAX_STEPS; ph = PH_DONE; }
       if (ph == PH_WAIT_JAC && jq_head >= m.jend) {
         m.nje++;
+        // the staging vector served Jacobian columns meanwhile: f(tn, zn[0]) back from FT
+        double fz[N];
+        for (int i = 0; i < N; ++i) fz[i] = L.FT(i);
+        for (int i = 0; i < N; ++i) LV[i] = fz[i];
         ph = PH_FACTOR;
       }
       if (ph == PH_FACTOR) {
@@ -447,15 +451,12 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
             ph = PH_WAIT_JAC;
           } else {
             m.jcur = 0;
-            for (int i = 0; i < N; ++i) L.FT(i) = f[i];
-            ph = PH_FACTOR;
+            ph = PH_FACTOR;                    // f stays in the staging vector for NEWTON
           }
         } else {
-          for (int i = 0; i < N; ++i) L.FT(i) = f[i];
           ph = PH_NEWTON;
         }
       } else if (ph == PH_ITER) {
-        for (int i = 0; i < N; ++i) L.FT(i) = f[i];
         ph = PH_NEWTON;
       } else if (ph == PH_RESTART_F) {
         for (int i = 0; i < N; ++i) L.ZN(1, i) = m.h * f[i];
