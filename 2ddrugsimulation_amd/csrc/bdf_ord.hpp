@@ -83,6 +83,18 @@ CEP_HD constexpr bool couples(int row, int g)
   }
 }
 
+// entries fetch_couplings gathers for one row: 12 core columns (optional), the coupled simple
+// gates, nca and O
+CEP_HD constexpr int ncouplings(int row, bool with_core)
+{
+  int n = with_core ? 12 : 0;
+  for (int k = 0; k < NG; ++k) n += couples(row, kSimpleGates[k]) ? 1 : 0;
+  return n + (couples(row, 33) ? 1 : 0) + (couples(row, 38) ? 1 : 0);
+}
+constexpr int NCPL_MAX = 40;
+static_assert(ncouplings(0, true) == NCPL_MAX && ncouplings(49, true) <= NCPL_MAX,
+              "coupling batch size");
+
 struct Buffers {
   int ncell, pitch;
   double *y;        // [52] state at the last output point (CellML STATES)
@@ -448,37 +460,38 @@ struct Lane {
   }
 
   // ---------------- linear algebra: structured factorisation of M = I - gamma*J.
-  // Each section loads its Jacobian entries, computes in registers and stores its factors.
+  // Each section fetches its Jacobian entries in batches (fetch / fetch_couplings below),
+  // computes in registers and stores its factors.
   CEP_D bool factor()
   {
     const double g = m.gamma;
-    // simple gates: x_g = r_g b_g - c_g x_V
-    double rg[NG], cg[NG];
-#pragma unroll
-    for (int k = 0; k < NG; ++k) {
-      const int s = kSimpleGates[k];
-      const double r = 1.0 / (1.0 - g * J(s, s));
-      rg[k] = r;
-      cg[k] = (-g * J(s, 0)) * r;
-    }
-    // nca: row {cass, jca, nca}
-    const double rn = 1.0 / (1.0 - g * J(33, 33));
-    const double e_nj = rn * (-g * J(33, 30));
-    const double cnV = -e_nj * cg[20];           // jca is simple gate #20
-    const double cnC = rn * (-g * J(33, 2));
-#pragma unroll
-    for (int k = 0; k < NG; ++k) { FAC(F_RG + k) = rg[k]; FAC(F_CG + k) = cg[k]; }
-    FAC(F_NCA + 0) = rn;
-    FAC(F_NCA + 1) = e_nj;
-    FAC(F_NCA + 2) = cnV;
-    FAC(F_NCA + 3) = cnC;
     // Markov 9x9 LU (no pivoting: column diagonally dominant M-matrix), w = M^-1 (-g J_mV)
     double A[9][9], w[9];
+    {
+      double jm[45], jw[9];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) {
+      for (int i = 0; i < 5; ++i)
 #pragma unroll
-      for (int j = 0; j < 9; ++j) A[i][j] = (i == j ? 1.0 : 0.0) - g * J(34 + i, 34 + j);
-      w[i] = -g * J(34 + i, 0);
+        for (int j = 0; j < 9; ++j) jm[i * 9 + j] = J(34 + i, 34 + j);
+#pragma unroll
+      for (int i = 0; i < 9; ++i) jw[i] = J(34 + i, 0);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int i = 0; i < 5; ++i) {
+#pragma unroll
+        for (int j = 0; j < 9; ++j) A[i][j] = (i == j ? 1.0 : 0.0) - g * jm[i * 9 + j];
+      }
+#pragma unroll
+      for (int i = 0; i < 9; ++i) w[i] = -g * jw[i];
+#pragma unroll
+      for (int i = 5; i < 9; ++i)
+#pragma unroll
+        for (int j = 0; j < 9; ++j) jm[(i - 5) * 9 + j] = J(34 + i, 34 + j);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int i = 5; i < 9; ++i)
+#pragma unroll
+        for (int j = 0; j < 9; ++j) A[i][j] = (i == j ? 1.0 : 0.0) - g * jm[(i - 5) * 9 + j];
     }
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
@@ -501,32 +514,64 @@ struct Lane {
       w[i] /= A[i][i];
     }
     const double w4 = w[4];
-    // core Schur complement (12x12, V last), its loads before the Markov stores
-    double S[12][12];
-#pragma unroll
-    for (int a = 0; a < 12; ++a)
-#pragma unroll
-      for (int b = 0; b < 12; ++b) S[a][b] = (a == b ? 1.0 : 0.0) - g * J(kCore[a], kCore[b]);
-#pragma unroll
-    for (int a = 0; a < 12; ++a) {
-      const int row = kCore[a];
-      double sv = 0.0;
-#pragma unroll
-      for (int k = 0; k < NG; ++k)
-        if (couples(row, kSimpleGates[k])) sv += (-g * J(row, kSimpleGates[k])) * cg[k];
-      if (couples(row, 33)) {
-        const double mn = -g * J(row, 33);
-        sv += mn * cnV;
-        S[a][CORE_CASS] -= mn * cnC;
-      }
-      if (couples(row, 38)) sv += (-g * J(row, 38)) * w4;
-      S[a][CORE_V] -= sv;
-    }
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
 #pragma unroll
       for (int j = 0; j < 9; ++j) FAC(F_MLU + i * 9 + j) = A[i][j];
       FAC(F_W + i) = w[i];
+    }
+    // simple gates: x_g = r_g b_g - c_g x_V; nca: row {cass, jca, nca}
+    double cg[NG], cnV, cnC;
+    {
+      double jd[NG], jv[NG], jn[3];
+#pragma unroll
+      for (int k = 0; k < NG; ++k) {
+        jd[k] = J(kSimpleGates[k], kSimpleGates[k]);
+        jv[k] = J(kSimpleGates[k], 0);
+      }
+      jn[0] = J(33, 33);
+      jn[1] = J(33, 30);
+      jn[2] = J(33, 2);
+      __builtin_amdgcn_sched_barrier(0);
+      double rg[NG];
+#pragma unroll
+      for (int k = 0; k < NG; ++k) {
+        const double r = 1.0 / (1.0 - g * jd[k]);
+        rg[k] = r;
+        cg[k] = (-g * jv[k]) * r;
+      }
+      const double rn = 1.0 / (1.0 - g * jn[0]);
+      const double e_nj = rn * (-g * jn[1]);
+      cnV = -e_nj * cg[20];                      // jca is simple gate #20
+      cnC = rn * (-g * jn[2]);
+#pragma unroll
+      for (int k = 0; k < NG; ++k) { FAC(F_RG + k) = rg[k]; FAC(F_CG + k) = cg[k]; }
+      FAC(F_NCA + 0) = rn;
+      FAC(F_NCA + 1) = e_nj;
+      FAC(F_NCA + 2) = cnV;
+      FAC(F_NCA + 3) = cnC;
+    }
+    // core Schur complement (12x12, V last), one row's entries per batch
+    double S[12][12];
+#pragma unroll
+    for (int a = 0; a < 12; ++a) {
+      const int row = kCore[a];
+      double jr[NCPL_MAX];
+      fetch_couplings(jr, row, true);
+      int n = 0;
+#pragma unroll
+      for (int b = 0; b < 12; ++b) S[a][b] = (a == b ? 1.0 : 0.0) - g * jr[n++];
+      double sv = 0.0;
+#pragma unroll
+      for (int k = 0; k < NG; ++k)
+        if (couples(row, kSimpleGates[k])) sv += (-g * jr[n++]) * cg[k];
+      if (couples(row, 33)) {
+        const double mn = -g * jr[n++];
+        sv += mn * cnV;
+        S[a][CORE_CASS] -= mn * cnC;
+      }
+      if (couples(row, 38)) sv += (-g * jr[n++]) * w4;
+      S[a][CORE_V] -= sv;
     }
 #pragma unroll
     for (int k = 0; k < 12; ++k) {
@@ -560,7 +605,7 @@ struct Lane {
   }
   // the Jacobian entries J(row, col) of the solve's coupling terms for one row, in the order
   // the solve uses them (simple gates, nca, O); returns their number
-  CEP_D int fetch_couplings(double (&r)[34], int row, bool with_core) const
+  CEP_D int fetch_couplings(double (&r)[NCPL_MAX], int row, bool with_core) const
   {
     int n = 0;
     if (with_core) {
@@ -631,7 +676,7 @@ struct Lane {
     for (int a = 0; a < 12; ++a) {
       const int row = kCore[a];
       double s = b[row];
-      double jr[34];
+      double jr[NCPL_MAX];
       const int nj = fetch_couplings(jr, row, false);
       (void)nj;
       int n = 0;
@@ -716,7 +761,7 @@ struct Lane {
     // accumulators: x = b + gamma * J(acc, :) x
 #pragma unroll
     for (int r = 49; r <= 51; ++r) {
-      double jr[34];
+      double jr[NCPL_MAX];
       const int nj = fetch_couplings(jr, r, true);
       (void)nj;
       int n = 0;
