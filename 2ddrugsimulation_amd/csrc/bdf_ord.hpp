@@ -194,6 +194,16 @@ struct Lane {
   CEP_D BElem JD(int slot, int row) const { return blk(V.jd, slot * N + row); }
   CEP_D double J(int row, int col) const { return JD(jslot(col), row); }
   CEP_D BElem FAC(int k) const { return blk(V.fac, k); }
+  // the same elements for an index that differs between lanes (a Jacobian column slot, a
+  // perturbed component): the index goes into the per-lane offset.  As a scalar offset a
+  // divergent index would make the compiler loop over its distinct values.
+  CEP_D BElem blk_lane(Rsrc r, unsigned k, int kc) const
+  {
+    return BElem{r, (ln & 63u) * 8u + k * (CPB * 8u), kc * CPB * 8};
+  }
+  CEP_D BElem ZN_lane(int j, unsigned i) const { return blk_lane(V.zn, i, j * N); }
+  CEP_D BElem EWT_lane(unsigned i) const { return blk_lane(V.ewt, i, 0); }
+  CEP_D BElem JD_lane(unsigned slot, int row) const { return blk_lane(V.jd, slot * N, row); }
   CEP_D double &Y(int i) const { return B.y[i * p + c]; }
 
   // ---------------- persistence between launches

@@ -388,9 +388,9 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
         const int co = cell0 + jown;
         Lane<R> Lo{B, s_m[jown], V, (unsigned)jown, (unsigned)co, pp};
         const int k = jstate(jcol);
-        const double ysave = Lo.ZN(0, k);
+        const double ysave = Lo.ZN_lane(0, (unsigned)k);
         double inc = srur * fabs(ysave);
-        const double alt = s_m[jown].mininc / Lo.EWT(k);
+        const double alt = s_m[jown].mininc / Lo.EWT_lane((unsigned)k);
         if (alt > inc) inc = alt;
         // no runtime index into the register array: select the perturbed component
         double z[N];
@@ -487,7 +487,7 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
       double ft[N], fv[N];
       for (int i = 0; i < N; ++i) ft[i] = Lo.FT(i);
       for (int i = 0; i < N; ++i) fv[i] = f[i];
-      for (int i = 0; i < N; ++i) Lo.JD(jcol, i) = iinv * fv[i] - iinv * ft[i];
+      for (int i = 0; i < N; ++i) Lo.JD_lane((unsigned)jcol, i) = iinv * fv[i] - iinv * ft[i];
     }
     // integrators that need a fresh Jacobian queue their 48 columns
     const unsigned long long pubm = __ballot(publish);
