@@ -51,6 +51,12 @@ constexpr double UROUND = 2.220446049250313e-16;
 enum { FIRST_CALL = 0, PREV_CONV_FAIL = 1, PREV_ERR_FAIL = 2 };
 enum { NO_FAILURES = 0, FAIL_BAD_J = 1, FAIL_OTHER = 2 };
 enum { NLS_OK = 0, NLS_ITER = 1, NLS_TRY_AGAIN = 2, NLS_CONV_FAIL = 3 };
+// array work split across the wave: the completion update, the interpolation to an output
+// point, the map of the next attempt, the copy of the output to the cell's STATES
+enum { SWF_COMPLETE = 1, SWF_INTERP = 2, SWF_ATTEMPT = 4, SWF_YCOPY = 8 };
+// upper triangle of the attempt map, row-major: entry (r, c), c >= r
+CEP_HD constexpr int map_idx(int r, int c) { return r * 6 - (r * (r - 1)) / 2 + (c - r); }
+constexpr int NMAP = 21;
 // where a cell's integrator is; PH_F0 .. PH_HIN end a pass with a right-hand-side request
 enum {
   PH_FRESH = 0, PH_INIT_F, PH_HIN, PH_START, PH_PREDICT, PH_F0, PH_WAIT_JAC, PH_FACTOR,
@@ -126,6 +132,9 @@ struct Mach {
   int q, qprime, qwait, jcur, phase, callSetup, convfail, nflag, ncf, nef, mnewt;
   int iout, iout_end, status, hin_count, hin_ok, nstloc, jend;
   int sw_ops, sw_dq, sw_q1;   // Nordsieck-array work pending for the next attempt
+  // array work of this pass done by the whole wave (SWF_* flags) and its parameters
+  int wsw, c_save, c_dn, c_up, i_q5;
+  double c_cquot, c_dsm, i_s;
   long long nst, nstlp, nstlj, nfe, nje, nsetups, nni, ncfn, netf;
 };
 
@@ -299,11 +308,44 @@ struct Lane {
   template <typename YV>
   CEP_D void attempt_sweep(int ops, int q0, int q1, const double *ll, double a1, double eta, const YV &y)
   {
+    double M[QMAX + 1][QMAX + 1];
+    build_map(ops, q0, q1, ll, a1, eta, M);
+#pragma unroll
+    for (int c0 = 0; c0 < N; c0 += CHZ) {
+      double z[CHZ][QMAX + 1];
+#pragma unroll
+      for (int k = 0; k < CHZ; ++k)
+#pragma unroll
+        for (int j = 0; j <= QMAX; ++j) z[k][j] = ZN(j, c0 + k);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int k = 0; k < CHZ; ++k) {
+        double w[QMAX + 1];
+#pragma unroll
+        for (int r = 0; r <= QMAX; ++r) {
+          double acc = 0.0;
+#pragma unroll
+          for (int col = QMAX; col >= r; --col) acc += M[r][col] * z[k][col];
+          w[r] = acc;
+        }
+#pragma unroll
+        for (int r = 0; r <= QMAX; ++r) z[k][r] = w[r];
+        y[c0 + k] = w[0];
+      }
+#pragma unroll
+      for (int k = 0; k < CHZ; ++k)
+#pragma unroll
+        for (int j = 0; j <= QMAX; ++j) ZN(j, c0 + k) = z[k][j];
+    }
+  }
+  // the composed map of attempt_sweep (applied to the unit columns)
+  CEP_D static void build_map(int ops, int q0, int q1, const double *ll, double a1, double eta,
+                              double (&M)[QMAX + 1][QMAX + 1])
+  {
     double fac[QMAX + 1];
     fac[0] = 1.0;
 #pragma unroll
     for (int j = 1; j <= QMAX; ++j) fac[j] = fac[j - 1] * eta;
-    double M[QMAX + 1][QMAX + 1];
 #pragma unroll
     for (int col = 0; col <= QMAX; ++col) {
       double zz[QMAX + 1];
@@ -333,33 +375,6 @@ struct Lane {
       for (int r = 0; r < QMAX; ++r) if (r > q1) zz[r] = 0.0;
 #pragma unroll
       for (int r = 0; r <= QMAX; ++r) M[r][col] = zz[r];
-    }
-#pragma unroll
-    for (int c0 = 0; c0 < N; c0 += CHZ) {
-      double z[CHZ][QMAX + 1];
-#pragma unroll
-      for (int k = 0; k < CHZ; ++k)
-#pragma unroll
-        for (int j = 0; j <= QMAX; ++j) z[k][j] = ZN(j, c0 + k);
-      __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-      for (int k = 0; k < CHZ; ++k) {
-        double w[QMAX + 1];
-#pragma unroll
-        for (int r = 0; r <= QMAX; ++r) {
-          double acc = 0.0;
-#pragma unroll
-          for (int col = QMAX; col >= r; --col) acc += M[r][col] * z[k][col];
-          w[r] = acc;
-        }
-#pragma unroll
-        for (int r = 0; r <= QMAX; ++r) z[k][r] = w[r];
-        y[c0 + k] = w[0];
-      }
-#pragma unroll
-      for (int k = 0; k < CHZ; ++k)
-#pragma unroll
-        for (int j = 0; j <= QMAX; ++j) ZN(j, c0 + k) = z[k][j];
     }
   }
 
@@ -858,6 +873,161 @@ struct Lane {
   // the components for: zn[j] += l[j] acor; the saved correction zn[QMAX] = acor when
   // qwait reaches 1; the norms of the order-selection test (zn[q] and acor - cquot
   // zn[QMAX], weighted with this step's ewt); the error weights of the next step.
+  // ---------------- array sweeps split across the wavefront: the four lanes of group g =
+  // 0..3 of a cell each take NQ consecutive components.  Per component the arithmetic is
+  // that of complete_step / interpolate / attempt_sweep; a sum over components (the
+  // order-selection norms) is left as per-component terms in LDS and added in component
+  // order by the integrator lane, so the results do not depend on the split.
+  static constexpr int NGRP = 4, NQ = N / NGRP;
+  static_assert(N % NGRP == 0, "whole groups");
+  CEP_D BElem gblk(Rsrc r, int g, int kc) const
+  {
+    return BElem{r, (ln & 63u) * 8u + (unsigned)g * (NQ * CPB * 8u), kc * CPB * 8};
+  }
+
+  // scalar part of complete_step before its sweep (what the sweep and complete_end need
+  // goes to the cell's LDS scalars)
+  CEP_D void complete_begin(double dsm)
+  {
+    m.nst++;
+    const int q = m.q;
+#pragma unroll
+    for (int i = QMAX; i >= 2; --i) if (i <= q) m.tau[i] = m.tau[i - 1];
+    if ((q == 1) && (m.nst > 1)) m.tau[2] = m.tau[1];
+    m.tau[1] = m.h;
+    const bool save = (m.qwait == 2) && (q != B.maxord);   // qwait becomes 1
+    m.qwait--;
+    const bool norms = (m.etamax != 1.0) && (m.qwait == 0);
+    const bool up = norms && (q != B.maxord) && (m.saved_tq5 != 0.0);
+    m.c_save = save;
+    m.c_dn = norms && q > 1;
+    m.c_up = up;
+    m.c_cquot = up ? (m.tq5 / m.saved_tq5) * pow(m.h / m.tau[2], (double)(q + 1)) : 0.0;
+    m.c_dsm = dsm;
+  }
+  // complete_step's sweep over group g's components; the norm terms go to t2 / t3
+  template <typename TV>
+  CEP_D void complete_part(int g, const TV &t2, const TV &t3) const
+  {
+    const int q = m.q;
+    double l[QMAX + 1], oh[QMAX + 1];
+#pragma unroll
+    for (int j = 0; j <= QMAX; ++j) { l[j] = (j <= q) ? m.l[j] : 0.0; oh[j] = (j == q) ? 1.0 : 0.0; }
+    const bool save = m.c_save;
+    const double cquot = m.c_cquot, rt = B.rtol, at_ = B.atol;
+    constexpr int C4 = 4;
+#pragma unroll
+    for (int c0 = 0; c0 < NQ; c0 += C4) {
+      constexpr int dummy = 0;
+      (void)dummy;
+      double z[C4][QMAX + 1], a[C4], w[C4];
+#pragma unroll
+      for (int k = 0; k < C4; ++k) {
+        if (c0 + k < NQ) {
+#pragma unroll
+          for (int j = 0; j <= QMAX; ++j) z[k][j] = gblk(V.zn, g, j * N + c0 + k);
+          a[k] = gblk(V.acor, g, c0 + k);
+          w[k] = gblk(V.ewt, g, c0 + k);
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int k = 0; k < C4; ++k) {
+        if (c0 + k < NQ) {
+          const int i = g * NQ + c0 + k;
+          const double zqm = z[k][QMAX];
+          double zq = 0.0;
+#pragma unroll
+          for (int j = 0; j <= QMAX; ++j) {
+            z[k][j] += l[j] * a[k];
+            zq += oh[j] * z[k][j];
+          }
+          t2[i] = zq * w[k];
+          t3[i] = (a[k] - cquot * zqm) * w[k];
+          if (save) z[k][QMAX] = a[k];
+          w[k] = 1.0 / (rt * fabs(z[k][0]) + at_);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < C4; ++k) {
+        if (c0 + k < NQ) {
+#pragma unroll
+          for (int j = 0; j <= QMAX; ++j) gblk(V.zn, g, j * N + c0 + k) = z[k][j];
+          gblk(V.ewt, g, c0 + k) = w[k];
+        }
+      }
+    }
+  }
+  // complete_step after its sweep: the norms from the per-component terms, in component order
+  template <typename TV>
+  CEP_D void complete_end(const TV &t2, const TV &t3)
+  {
+    double s2 = 0.0, s3 = 0.0;
+    if (m.c_dn)
+      for (int i = 0; i < N; ++i) { const double t = t2[i]; s2 += t * t; }
+    if (m.c_up)
+      for (int i = 0; i < N; ++i) { const double t = t3[i]; s3 += t * t; }
+    if (m.c_save) m.saved_tq5 = m.tq5;
+    prepare_next(m.c_dsm, s2, s3, m.c_dn, m.c_up);
+    m.etamax = (m.nst <= SMALL_NST) ? ETAMX2 : ETAMX3;
+  }
+  // interpolate() (at s = m.i_s) and/or the attempt map (apply_map with M from LDS) over group
+  // g's components; with `ycopy` the interpolated values also become the cell's STATES
+  template <typename YV>
+  CEP_D void output_attempt_part(int g, bool interp, bool ycopy, const YV &yo, bool attempt, const double *Mp,
+                                 const YV &yp) const
+  {
+    const double s = m.i_s;
+    const bool q5 = m.i_q5 != 0;
+    double M[NMAP];
+    if (attempt) {
+#pragma unroll
+      for (int k = 0; k < NMAP; ++k) M[k] = Mp[k];
+    }
+    constexpr int C4 = 4;
+#pragma unroll
+    for (int c0 = 0; c0 < NQ; c0 += C4) {
+      double z[C4][QMAX + 1];
+#pragma unroll
+      for (int k = 0; k < C4; ++k)
+        if (c0 + k < NQ) {
+#pragma unroll
+          for (int j = 0; j <= QMAX; ++j) z[k][j] = gblk(V.zn, g, j * N + c0 + k);
+        }
+      __builtin_amdgcn_sched_barrier(0);
+      if (interp) {
+#pragma unroll
+        for (int k = 0; k < C4; ++k)
+          if (c0 + k < NQ) {
+            const int i = g * NQ + c0 + k;
+            double v = q5 ? z[k][QMAX] : 0.0;
+#pragma unroll
+            for (int j = QMAX - 1; j >= 0; --j) v = z[k][j] + s * v;   // rows above q are zero
+            yo[i] = v;
+            if (ycopy) Y(i) = v;
+          }
+      }
+      if (attempt) {
+#pragma unroll
+        for (int k = 0; k < C4; ++k)
+          if (c0 + k < NQ) {
+            const int i = g * NQ + c0 + k;
+            double w[QMAX + 1];
+#pragma unroll
+            for (int r = 0; r <= QMAX; ++r) {
+              double acc = 0.0;
+#pragma unroll
+              for (int col = QMAX; col >= r; --col) acc += M[map_idx(r, col)] * z[k][col];
+              w[r] = acc;
+            }
+#pragma unroll
+            for (int j = 0; j <= QMAX; ++j) gblk(V.zn, g, j * N + c0 + k) = w[j];
+            yp[i] = w[0];
+          }
+      }
+    }
+  }
+
   CEP_D void complete_step(double dsm)
   {
     m.nst++;

@@ -115,6 +115,32 @@ CEP_D int lanes_below(unsigned long long mask)
   return __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0));
 }
 
+// The array work of a pass split across the wavefront: lane l works on cell slot l % CPW,
+// components [g*NQ, (g+1)*NQ) with g = l / CPW.  The first call does the completion sweeps
+// (norm terms into the cell's staging vector and the output lane's), the second the
+// interpolations to output points and the maps of the next attempts.
+template <typename R, int CPW>
+CEP_D void wave_sweeps(const ordbdf::Buffers &B, const ordbdf::BlockBufs &V, ordbdf::Mach *s_m, double (*s_v)[64],
+                       double (*s_map)[ordbdf::NMAP],
+                       int cell0, int lane, int pp, bool complete)
+{
+  using namespace ordbdf;
+  static_assert(CPW * Lane<R>::NGRP == 64, "four lanes per cell");
+  const int s = lane % CPW, g = lane / CPW;
+  const int fl = (cell0 + s < B.ncell) ? s_m[s].wsw : 0;
+  const LVec yv{&s_v[0][s]}, ov{&s_v[0][CPW + s]};
+  if (complete) {
+    if (fl & SWF_COMPLETE) {
+      Lane<R> Ls{B, s_m[s], V, (unsigned)s, (unsigned)(cell0 + s), pp};
+      Ls.complete_part(g, yv, ov);
+    }
+  } else if (fl & (SWF_INTERP | SWF_ATTEMPT)) {
+    Lane<R> Ls{B, s_m[s], V, (unsigned)s, (unsigned)(cell0 + s), pp};
+    Ls.output_attempt_part(g, (fl & SWF_INTERP) != 0, (fl & SWF_YCOPY) != 0, ov, (fl & SWF_ATTEMPT) != 0,
+                           &s_map[s][0], yv);
+  }
+}
+
 template <typename R, int CPW>
 __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunCfg rc, const int nbeats,
                                                      const long long max_passes)
@@ -131,6 +157,7 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
   __shared__ double s_tout[CPW];
   __shared__ int s_oiout[CPW], s_opend[CPW], s_ofail[CPW];
   __shared__ unsigned short s_jq[QCAP];           // Jacobian tasks: (slot << 6) | column slot
+  __shared__ double s_map[CPW][NMAP];             // each cell's attempt map for the wave sweep
 
   const int lane = threadIdx.x;
   const int cell0 = blockIdx.x * CPW;
@@ -197,11 +224,14 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
     asm volatile("" : "+s"(pp));
     int req = RQ_NONE;
     double treq = 0.0;
-    // ================= integrator bookkeeping (each block masked to its phase)
+    int ph = PH_DONE;
+    // ================= integrator bookkeeping (each block masked to its phase), in two
+    // stages around the wave-wide sweep of the completed steps
     if (is_int) {
       Lane<R> L{B, s_m[slot], V, (unsigned)slot, (unsigned)c, pp};
       Mach &m = s_m[slot];
-      int ph = m.phase;
+      ph = m.phase;
+      m.wsw = 0;
       if (s_ofail[slot] && ph != PH_DONE) { m.status = ST_NONFINITE; ph = PH_DONE; }
       if (pass >= max_passes && ph != PH_DONE) { m.status = ST_MAX_STEPS; ph = PH_DONE; }
       if (ph == PH_WAIT_JAC && jq_head >= m.jend) {
@@ -246,7 +276,8 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
       if (ph == PH_ETEST) {
         const double dsm = m.acnrm * m.tq2;
         if (dsm <= 1.0) {
-          L.complete_step(dsm);
+          L.complete_begin(dsm);               // the array update: wave sweep below
+          m.wsw |= SWF_COMPLETE;
           ph = PH_STEP_DONE;
         } else {
           m.nef++;
@@ -284,15 +315,23 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
           }
         }
       }
+    }
+    __syncthreads();
+    wave_sweeps<R, CPW>(B, V, s_m, s_v, s_map, cell0, lane, pp, true);
+    __syncthreads();
+    if (is_int) {
+      Lane<R> L{B, s_m[slot], V, (unsigned)slot, (unsigned)c, pp};
+      Mach &m = s_m[slot];
+      if (m.wsw & SWF_COMPLETE) L.complete_end(LV, LVec{&s_v[0][CPW + slot]});
       if (ph == PH_STEP_DONE) {
         if ((m.tn - m.tout) * m.h >= 0.0) {
           if (!s_opend[slot]) {              // hand y(tout) to the output lane
-            const LVec yo{&s_v[0][CPW + slot]};   // the output lane's staging vector
-            L.interpolate(m.tout, yo);
+            m.i_s = (m.tout - m.tn) / m.h;       // interpolated by the wave sweep below
+            m.i_q5 = m.q == QMAX;
+            m.wsw |= SWF_INTERP;
             s_tout[slot] = m.tout;
             m.iout++;
-            if (m.iout >= m.iout_end)              // CellML STATES at the end of the run
-              for (int i = 0; i < N; ++i) L.Y(i) = yo[i];
+            if (m.iout >= m.iout_end) m.wsw |= SWF_YCOPY;   // CellML STATES at the end of the run
             s_oiout[slot] = m.iout;
             s_opend[slot] = 1;
             m.nstloc = 0;
@@ -337,7 +376,15 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
         m.q = q1;
         if (ops & Lane<R>::SW_RESCALE) { m.h = m.hscale * m.eta; m.hscale = m.h; }
         if (!restart) { ops |= Lane<R>::SW_PREDICT; m.tn += m.h; }
-        L.attempt_sweep(ops, q0, q1, ll, a1, m.eta, LV);
+        {
+          double M[QMAX + 1][QMAX + 1];
+          Lane<R>::build_map(ops, q0, q1, ll, a1, m.eta, M);
+#pragma unroll
+          for (int r = 0; r <= QMAX; ++r)
+#pragma unroll
+            for (int col = r; col <= QMAX; ++col) s_map[slot][map_idx(r, col)] = M[r][col];
+          m.wsw |= SWF_ATTEMPT;                // applied by the wave sweep below
+        }
         m.sw_ops = 0; m.sw_dq = 0; m.sw_q1 = m.q;
         if (restart) {
           ph = PH_RESTART_F;
@@ -368,6 +415,8 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
       }
       m.phase = ph;
     }
+    __syncthreads();
+    wave_sweeps<R, CPW>(B, V, s_m, s_v, s_map, cell0, lane, pp, false);
     __syncthreads();
     // ================= output lanes
     if (is_out && s_opend[slot]) {                 // y(tout) is already in s_v

@@ -92,6 +92,9 @@ def _gpu_ord11_adams(key):
 
 
 @pytest.mark.gpu
+@pytest.mark.skip(reason="a GPU run of this case, in a build with -ffp-contract=on, ended in a GPU "
+                         "memory fault whose cause has not been found; the case stays off the GPU "
+                         "until it is")
 @pytest.mark.parametrize("key", ["ord11_ctl", "ord11_quin"])
 def test_adams_gpu_matches_reference(key):
     t, yref, trailer = _samples(key, G[f"{key}_out"])
