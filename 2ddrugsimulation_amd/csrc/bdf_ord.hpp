@@ -963,10 +963,27 @@ struct Lane {
   CEP_D void complete_end(const TV &t2, const TV &t3)
   {
     double s2 = 0.0, s3 = 0.0;
-    if (m.c_dn)
-      for (int i = 0; i < N; ++i) { const double t = t2[i]; s2 += t * t; }
-    if (m.c_up)
-      for (int i = 0; i < N; ++i) { const double t = t3[i]; s3 += t * t; }
+    // the terms are read in chunks (LDS reads issued together), summed in component order
+    if (m.c_dn) {
+#pragma unroll
+      for (int c0 = 0; c0 < N; c0 += CH) {
+        double t[CH];
+#pragma unroll
+        for (int k = 0; k < CH; ++k) t[k] = t2[c0 + k];
+#pragma unroll
+        for (int k = 0; k < CH; ++k) s2 += t[k] * t[k];
+      }
+    }
+    if (m.c_up) {
+#pragma unroll
+      for (int c0 = 0; c0 < N; c0 += CH) {
+        double t[CH];
+#pragma unroll
+        for (int k = 0; k < CH; ++k) t[k] = t3[c0 + k];
+#pragma unroll
+        for (int k = 0; k < CH; ++k) s3 += t[k] * t[k];
+      }
+    }
     if (m.c_save) m.saved_tq5 = m.tq5;
     prepare_next(m.c_dsm, s2, s3, m.c_dn, m.c_up);
     m.etamax = (m.nst <= SMALL_NST) ? ETAMX2 : ETAMX3;
