@@ -23,6 +23,7 @@ MODEL_ORD2011 = 3
 OPT_DUTTA = 1
 OPT_TIME_POINT = 2
 OPT_INTEGRATOR = 3
+OPT_TIME_SERIES = 4
 INTEGRATOR_RUSH_LARSEN = 0
 INTEGRATOR_ADAMS = 1
 PREC_FP64 = 0
@@ -44,13 +45,15 @@ EXPORTED = (
     "cep_set_run_config", "cep_init_consts", "cep_get_constants", "cep_compute_rates",
     "cep_run_beats", "cep_run_beats_async", "cep_sync", "cep_get_states", "cep_set_states",
     "cep_get_biomarkers", "cep_get_status", "cep_get_stats", "cep_enable_vm_trace",
-    "cep_get_vm_trace", "cep_kernel_time", "cep_query", "cep_batch_device_bytes",
+    "cep_get_vm_trace", "cep_get_time_series", "cep_get_dvmdt_repol_history",
+    "cep_kernel_time", "cep_query", "cep_batch_device_bytes",
     "cep_tissue_create", "cep_tissue_create_model", "cep_tissue_destroy", "cep_tissue_last_error", "cep_tissue_set_cells",
     "cep_tissue_set_conductivity", "cep_tissue_set_stimulus", "cep_tissue_step", "cep_tissue_diffuse",
     "cep_tissue_get_vm", "cep_tissue_get_states", "cep_tissue_set_states", "cep_tissue_time",
     "cep_tissue_kernel_times",
     "cep_read_ic50_csv", "cep_read_states_file", "cep_write_states_file", "cep_param_defaults",
     "cep_read_input_deck", "cep_param_concentrations", "cep_write_cipa_features", "cep_write_ord11_qni",
+    "cep_write_time_series",
 )
 SPLIT_GODUNOV = 0
 SPLIT_STRANG = 1
@@ -98,6 +101,10 @@ def library(path: str | None = None):
     L.cep_get_stats.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
     L.cep_enable_vm_trace.argtypes = [C.c_void_p, C.c_int]
     L.cep_get_vm_trace.argtypes = [C.c_void_p, dp, C.POINTER(C.c_int)]
+    L.cep_get_time_series.argtypes = [C.c_void_p, C.c_int, C.c_int, dp, dp, dp, dp, C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int)]
+    L.cep_get_dvmdt_repol_history.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int), dp,
+                                              C.POINTER(C.c_int)]
     L.cep_kernel_time.argtypes = [C.c_void_p, dp, dp]
     L.cep_query.argtypes = [C.c_void_p]
     L.cep_batch_device_bytes.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int]
@@ -197,7 +204,8 @@ class CellBatch:
         self._chk(self.L.cep_set_run_config(self.h, dt, rtol, atol, maxord, nbeats_total, nwin, stim_amp_scale))
 
     def set_model_option(self, option: int, value: float):
-        """ORd 2011: OPT_DUTTA (the reference's Is_Dutta), OPT_TIME_POINT (fixed-step window)."""
+        """ORd 2011: OPT_DUTTA (the reference's Is_Dutta), OPT_TIME_POINT (fixed-step window),
+        OPT_INTEGRATOR; ORd-CiPA: OPT_TIME_SERIES (record the selected beat's time series)."""
         self._chk(self.L.cep_set_model_option(self.h, int(option), float(value)))
 
     def init_consts(self):
@@ -255,6 +263,29 @@ class CellBatch:
         out = np.zeros((self.n, ns.value))
         self._chk(self.L.cep_get_vm_trace(self.h, _p(out), C.byref(ns)))
         return out
+
+    def time_series(self, cell: int) -> dict:
+        """The selected (steepest) beat's samples of one cell: t, vm, cai (n), ires (n x 6:
+        INaL ICaL Ito IKr IKs IK1), and the selected pace (OPT_TIME_SERIES must be on)."""
+        n, pace = C.c_int(), C.c_int()
+        self._chk(self.L.cep_get_time_series(self.h, int(cell), 0, None, None, None, None, C.byref(n),
+                                             C.byref(pace)))
+        k = n.value
+        t, vm, ca, ires = np.zeros(k), np.zeros(k), np.zeros(k), np.zeros((k, 6))
+        self._chk(self.L.cep_get_time_series(self.h, int(cell), k, _p(t), _p(vm), _p(ca), _p(ires), C.byref(n),
+                                             C.byref(pace)))
+        return {"t": t, "vm": vm, "cai": ca, "ires": ires, "pace": pace.value}
+
+    def dvmdt_repol_history(self, cell: int):
+        """(paces, dvmdt_repol) of every eligible pace of the window (OPT_TIME_SERIES on)."""
+        n = C.c_int()
+        self._chk(self.L.cep_get_dvmdt_repol_history(self.h, int(cell), 0, None, None, C.byref(n)))
+        paces = np.zeros(n.value, dtype=np.int32)
+        vals = np.zeros(n.value)
+        self._chk(self.L.cep_get_dvmdt_repol_history(self.h, int(cell), n.value,
+                                                     paces.ctypes.data_as(C.POINTER(C.c_int)), _p(vals),
+                                                     C.byref(n)))
+        return paces, vals
 
     def biomarkers(self) -> dict:
         out = np.zeros((self.n, NBM))

@@ -116,6 +116,7 @@ struct Buffers {
   int *bmi;         // [BM_I_N]
   double *ct, *cv;  // Ca trace [cap]
   double *vtrace;   // optional Vm trace of the last beat [tcap], or null
+  TsView ts;        // optional time series of the selected beat (ts.t == null: off)
   int tcap;
   long long *stats; // [8] nst nfe nje nsetups nni ncfn netf nout
   int *status;

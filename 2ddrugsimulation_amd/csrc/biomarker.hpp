@@ -22,8 +22,21 @@ struct BmResult {   // one beat's (or the selected beat's) biomarkers
 constexpr int BM_RES_N = 15;
 // persistent scalar state (besides the two results): doubles, then ints
 constexpr int BM_D_N = 14;
-constexpr int BM_I_N = 5;
+constexpr int BM_I_N = 8;
 constexpr int BM_TOTAL_D = BM_D_N + 2 * BM_RES_N;   // doubles stored per cell
+
+// Optional time series of the selected (steepest) beat, for the reference's vmcheck / ca_i /
+// ires / max_dvdt_repol files (CEPCell_NEO/modules/cipa_drug_bench.cpp:168-199,230-235,262,
+// 338-350).  Samples (t, Vm, Ca_i, INaL, ICaL, Ito, IKr, IKs, IK1) at the Ca-trace points go
+// to one of two slots per cell; when a beat becomes the selected one its slot is kept and
+// the next beat writes the other, so no trace is copied.  Pointers are this cell's (offset
+// by the cell index, strided by the pitch); t == nullptr: off.
+struct TsView {
+  double *t, *vm, *ca, *ires;   // [2][cap], [2][cap], [2][cap], [2][6][cap]
+  int *rp;                      // [nwin] paces of the dvmdt_repol history
+  double *rv;                   // [nwin] their dvmdt_repol
+  int cap, nwin;
+};
 
 CEP_D void bm_res_init(BmResult &r, double v, double ca)
 {
@@ -39,7 +52,8 @@ struct Biomarker {
   // persistent state
   double qnet, inal_auc, ical_auc, t_depol, repol30, repol50, repol90, t_ca_peak, ca_amp50,
       ca_amp90, cad50_prev, cad50_curr, cad90_prev, cad90_curr;
-  int pace, steepest_pace, eligible, nca, pad;
+  int pace, steepest_pace, eligible, nca;
+  int cur_slot, best_slot, best_n, nrep;   // time series (TsView): slot in use, selected slot
   BmResult temp, best;
 
   CEP_D void begin(double v0, double ca0)
@@ -47,7 +61,8 @@ struct Biomarker {
     bm_res_init(best, v0, ca0);
     bm_res_init(temp, v0, ca0);
     qnet = inal_auc = ical_auc = 0.;
-    pace = 0; steepest_pace = 0; eligible = 0; nca = 0; pad = 0;
+    pace = 0; steepest_pace = 0; eligible = 0; nca = 0;
+    cur_slot = 0; best_slot = 0; best_n = 0; nrep = 0;
     t_depol = (bcl * pace) + stim_start;
     t_ca_peak = 0.; ca_amp50 = ca_amp90 = 0.;
     cad50_prev = cad50_curr = cad90_prev = cad90_curr = 0.;
@@ -55,8 +70,10 @@ struct Biomarker {
   }
 
   // ct/cv: this cell's Ca trace (time, value), strided by `pitch`
+  // ires: INaL, ICaL, Ito, IKr, IKs, IK1 at this point (read only when ts.t is set)
   CEP_D void sample(int iout, double t, double v, double ca, double dvdt, double qsum, double inal,
-                    double ical, double *ct, double *cv, int pitch)
+                    double ical, double *ct, double *cv, int pitch, const TsView &ts = TsView{},
+                    const double *ires = nullptr)
   {
     qnet += qsum * dt;
     inal_auc += inal * dt;
@@ -64,6 +81,13 @@ struct Biomarker {
     if (pace >= win_start && iout % print_freq == 0 && nca < cap) {
       ct[(size_t)nca * pitch] = t;
       cv[(size_t)nca * pitch] = ca;
+      if (ts.t && nca < ts.cap) {
+        const size_t o = ((size_t)cur_slot * ts.cap + nca) * pitch;
+        ts.t[o] = t;
+        ts.vm[o] = v;
+        ts.ca[o] = ca;
+        for (int j = 0; j < 6; ++j) ts.ires[((size_t)(cur_slot * 6 + j) * ts.cap + nca) * pitch] = ires[j];
+      }
       nca++;
     }
     if (iout > 0 && iout % beat_steps == 0) {
@@ -85,16 +109,27 @@ struct Biomarker {
         temp.ical_auc = ical_auc;
         temp.vm_dia = v;
         temp.ca_dia = ca;
+        if (ts.rp && nrep < ts.nwin) {      // the reference's max_dvdt_repol line for this pace
+          ts.rp[(size_t)nrep * pitch] = pace;
+          ts.rv[(size_t)nrep * pitch] = temp.dvmdt_repol;
+          nrep++;
+        }
         if (temp.dvmdt_repol > best.dvmdt_repol) {
           steepest_pace = pace;
           best = temp;
+          best_slot = cur_slot;
+          best_n = nca;
         }
       }
+      const int fin_slot = cur_slot, fin_n = nca;
       pace++;
       if (pace == win_start + 1) {
         best = temp;
         steepest_pace = pace - 1;
+        best_slot = fin_slot;
+        best_n = fin_n;
       }
+      if (best_slot == fin_slot) cur_slot = 1 - fin_slot;   // keep the selected beat's samples
       qnet = inal_auc = ical_auc = 0.;
       if (pace >= win_start) {
         bm_res_init(temp, v, ca);
@@ -143,6 +178,8 @@ struct Biomarker {
     for (int k = 0; k < BM_RES_N; ++k) br[k] = p[(size_t)(BM_D_N + BM_RES_N + k) * pitch];
     const int *q = iv + c;
     pace = q[0]; steepest_pace = q[(size_t)pitch]; eligible = q[2 * (size_t)pitch]; nca = q[3 * (size_t)pitch];
+    cur_slot = q[4 * (size_t)pitch]; best_slot = q[5 * (size_t)pitch]; best_n = q[6 * (size_t)pitch];
+    nrep = q[7 * (size_t)pitch];
   }
   CEP_D void store(double *d, int *iv, int pitch, int c) const
   {
@@ -157,6 +194,8 @@ struct Biomarker {
     for (int k = 0; k < BM_RES_N; ++k) p[(size_t)(BM_D_N + BM_RES_N + k) * pitch] = br[k];
     int *q = iv + c;
     q[0] = pace; q[(size_t)pitch] = steepest_pace; q[2 * (size_t)pitch] = eligible; q[3 * (size_t)pitch] = nca;
+    q[4 * (size_t)pitch] = cur_slot; q[5 * (size_t)pitch] = best_slot; q[6 * (size_t)pitch] = best_n;
+    q[7 * (size_t)pitch] = nrep;
   }
 };
 

@@ -524,9 +524,12 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
       for (int i = 0; i < N; ++i) fin &= (bool)isfinite(f[i]);   // no short-circuit branches
       if (!fin) s_ofail[slot] = 1;
       const int iout = s_oiout[slot];
+      TsView tv = B.ts;
+      if (tv.t) { tv.t += c; tv.vm += c; tv.ca += c; tv.ires += c; tv.rp += c; tv.rv += c; }
+      const double ires[6] = { cur.INaL, cur.ICaL, cur.Ito, cur.IKr, cur.IKs, cur.IK1 };
       s_bm[slot].sample(iout, treq, v0, ca0, f[0],
                         cur.INaL + cur.ICaL + cur.Ito + cur.IKr + cur.IKs + cur.IK1, cur.INaL, cur.ICaL,
-                        B.ct + c, B.cv + c, pp);
+                        B.ct + c, B.cv + c, pp, tv, ires);
       if (B.vtrace && iout > trace_start && iout - trace_start <= B.tcap)
         B.vtrace[(iout - trace_start - 1) * pp + c] = v0;
       s_opend[slot] = 0;
@@ -1079,6 +1082,11 @@ struct cep_batch {
   int maxord = 5, nbeats_total = 1, nwin = 1;
   bool vtrace_on = false;
   int tcap = 0, cap = 0;
+  // time series of the selected beat (ORd-CiPA): [t, vm, ca][2][ts_cap], ires[6][2][ts_cap]
+  bool ts_on = false;
+  int ts_cap = 0, ts_nwin = 0;
+  double *d_ts = nullptr, *d_tsv = nullptr;
+  int *d_tsp = nullptr;
   // host copies of inputs
   std::vector<host::OrdCellIn> ord_in;
   std::vector<host::TtCellIn> tt_in;
@@ -1292,6 +1300,11 @@ int cep_set_run_config(cep_batch *b, double dt, double rtol, double atol, int ma
 int cep_set_model_option(cep_batch *b, int option, double value)
 {
   if (!b) return CEP_E_ARG;
+  if (option == CEP_OPT_TIME_SERIES) {
+    if (b->model != CEP_MODEL_ORD_CIPA) return fail(b, CEP_E_ARG, "time series are recorded for ORd-CiPA");
+    b->ts_on = value != 0.0;
+    return 0;
+  }
   if (b->model != CEP_MODEL_ORD2011) return fail(b, CEP_E_ARG, "no options for this model");
   if (option == CEP_OPT_DUTTA) b->dutta = value != 0.0;
   else if (option == CEP_OPT_TIME_POINT && value >= 0.0) b->time_point = value;
@@ -1334,6 +1347,15 @@ static int alloc_trace_buffers(cep_batch *b)
   if (b->vtrace_on && beat_steps > b->tcap) {
     if (dalloc(b, &b->d_vtrace, (size_t)b->pitch * beat_steps)) return CEP_E_HIP;
     b->tcap = beat_steps;
+  }
+  if (b->ts_on && (cap > b->ts_cap || b->nwin > b->ts_nwin)) {
+    const int nw = b->nwin > 1 ? b->nwin : 1;
+    int rc = dalloc(b, &b->d_ts, (size_t)b->pitch * 2 * cap * 9);
+    rc |= dalloc(b, &b->d_tsv, (size_t)b->pitch * nw);
+    rc |= dalloc(b, &b->d_tsp, (size_t)b->pitch * nw);
+    if (rc) return rc;
+    b->ts_cap = cap;
+    b->ts_nwin = nw;
   }
   return 0;
 }
@@ -1505,6 +1527,12 @@ int cep_run_beats_async(cep_batch *b, int nbeats)
     B.stats = b->d_stats; B.status = b->d_status; B.dt = b->dt; B.rtol = b->rtol; B.atol = b->atol;
     B.uround_fd = b->prec == CEP_PREC_FP64 ? ordbdf::UROUND : 1.1920928955078125e-07;
     B.maxord = b->maxord; B.max_steps = 500; B.cap = b->cap;
+    B.ts = TsView{};
+    if (b->ts_on && b->d_ts) {
+      const size_t sl = (size_t)b->pitch * 2 * b->ts_cap;   // one field, both slots
+      B.ts.t = b->d_ts; B.ts.vm = b->d_ts + sl; B.ts.ca = b->d_ts + 2 * sl; B.ts.ires = b->d_ts + 3 * sl;
+      B.ts.rp = b->d_tsp; B.ts.rv = b->d_tsv; B.ts.cap = b->ts_cap; B.ts.nwin = b->ts_nwin;
+    }
     // safety net only: a step takes one pass when Newton converges at once, a few more on
     // failures; cells still running after this many passes are flagged CEP_ST_MAX_STEPS
     const long long max_passes = 64LL * nbeats * (long long)(max_bcl(b) / b->dt) + 100000;
@@ -1707,6 +1735,60 @@ int cep_get_vm_trace(const cep_batch *b, double *out, int *nsamp)
   *nsamp = b->tcap;
   if (!out) return 0;   // size query
   return get_soa(b, b->d_vtrace, b->tcap, out);
+}
+
+// one cell's column of a SoA block: rows [0, n) of width 1 at row pitch b->pitch
+static int get_column(const cep_batch *b, const void *d, size_t elem, int cell, int n, void *out)
+{
+  if (n <= 0) return 0;
+  if (hipMemcpy2DAsync(out, elem, (const char *)d + (size_t)cell * elem, (size_t)b->pitch * elem, elem, n,
+                       hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
+      hipStreamSynchronize(b->stream) != hipSuccess)
+    return CEP_E_HIP;
+  return 0;
+}
+
+int cep_get_time_series(const cep_batch *b, int cell, int max_n, double *t, double *vm, double *cai, double *ires,
+                        int *n, int *pace)
+{
+  if (!b || !n || cell < 0 || cell >= b->n || max_n < 0) return CEP_E_ARG;
+  cep_batch *bm = const_cast<cep_batch *>(b);
+  if (!b->ts_on || !b->d_ts) return fail(bm, CEP_E_STATE, "time series not enabled (CEP_OPT_TIME_SERIES)");
+  int iv[BM_I_N];
+  if (int rc = get_column(b, b->d_bmi, sizeof(int), cell, BM_I_N, iv)) return rc;
+  const int slot = iv[5], cnt = iv[6] < b->ts_cap ? iv[6] : b->ts_cap;
+  *n = cnt;
+  if (pace) *pace = iv[1];
+  if (!t && !vm && !cai && !ires) return 0;   // size query
+  const int m = cnt < max_n ? cnt : max_n;
+  const size_t sl = (size_t)b->pitch * 2 * b->ts_cap, off = (size_t)slot * b->ts_cap * b->pitch;
+  if (t) if (int rc = get_column(b, b->d_ts + off, 8, cell, m, t)) return rc;
+  if (vm) if (int rc = get_column(b, b->d_ts + sl + off, 8, cell, m, vm)) return rc;
+  if (cai) if (int rc = get_column(b, b->d_ts + 2 * sl + off, 8, cell, m, cai)) return rc;
+  if (ires) {
+    std::vector<double> col((size_t)m);
+    for (int j = 0; j < 6; ++j) {
+      const size_t oj = 3 * sl + ((size_t)(slot * 6 + j) * b->ts_cap) * b->pitch;
+      if (int rc = get_column(b, b->d_ts + oj, 8, cell, m, col.data())) return rc;
+      for (int k = 0; k < m; ++k) ires[(size_t)k * 6 + j] = col[k];
+    }
+  }
+  return 0;
+}
+
+int cep_get_dvmdt_repol_history(const cep_batch *b, int cell, int max_n, int *paces, double *values, int *n)
+{
+  if (!b || !n || cell < 0 || cell >= b->n || max_n < 0) return CEP_E_ARG;
+  cep_batch *bm = const_cast<cep_batch *>(b);
+  if (!b->ts_on || !b->d_ts) return fail(bm, CEP_E_STATE, "time series not enabled (CEP_OPT_TIME_SERIES)");
+  int iv[BM_I_N];
+  if (int rc = get_column(b, b->d_bmi, sizeof(int), cell, BM_I_N, iv)) return rc;
+  const int cnt = iv[7] < b->ts_nwin ? iv[7] : b->ts_nwin;
+  *n = cnt;
+  const int m = cnt < max_n ? cnt : max_n;
+  if (paces) if (int rc = get_column(b, b->d_tsp, sizeof(int), cell, m, paces)) return rc;
+  if (values) if (int rc = get_column(b, b->d_tsv, 8, cell, m, values)) return rc;
+  return 0;
 }
 
 int cep_kernel_time(cep_batch *b, double *avg_ms, double *total_ms)

@@ -189,4 +189,35 @@ int cep_write_ord11_qni(const char *prefix, const int *sample_ids, const double 
   return 0;
 }
 
+int cep_write_time_series(const char *prefix, int sample_id, const double *t, const double *vm, const double *cai,
+                          const double *ires, int n, const int *rep_pace, const double *rep_val, int nrep,
+                          int write_header)
+{
+  if (!prefix || n < 0 || nrep < 0 || (n > 0 && (!t || !vm || !cai || !ires)) || (nrep > 0 && (!rep_pace || !rep_val)))
+    return CEP_E_ARG;
+  const std::string base(prefix), id = std::to_string(sample_id);
+  FILE *fv = fopen((base + "_vmcheck_smp" + id + ".plt").c_str(), "w");
+  FILE *fc = fopen((base + "_ca_i_smp" + id + ".plt").c_str(), "w");
+  FILE *fi = fopen((base + "_ires_smp" + id + ".plt").c_str(), "w");
+  FILE *fr = fopen((base + "_max_dvdt_repol_smp" + id + ".plt").c_str(), "w");
+  if (!fv || !fc || !fi || !fr) {
+    for (FILE *f : {fv, fc, fi, fr}) if (f) fclose(f);
+    return CEP_E_ARG;
+  }
+  if (write_header) {
+    fprintf(fv, "%s %s\n", "TIME", "Vm");
+    fprintf(fc, "%s %s\n", "TIME", "cai");
+    fprintf(fi, "%s %s %s %s %s %s %s\n", "TIME", "INaL", "ICaL", "Ito", "IKr", "IKs", "IK1");
+  }
+  for (int k = 0; k < nrep; ++k) fprintf(fr, "%d %lf\n", rep_pace[k], rep_val[k]);
+  for (int k = 0; k < n; ++k) fprintf(fc, "%lf %lf\n", t[k], cai[k]);
+  for (int k = 0; k < n; ++k) fprintf(fv, "%lf %lf\n", t[k], vm[k]);
+  for (int k = 0; k < n; ++k) {
+    const double *r = ires + (size_t)k * 6;
+    fprintf(fi, "%lf %lf %lf %lf %lf %lf %lf\n", t[k], r[0], r[1], r[2], r[3], r[4], r[5]);
+  }
+  for (FILE *f : {fv, fc, fi, fr}) fclose(f);
+  return 0;
+}
+
 }  // extern "C"

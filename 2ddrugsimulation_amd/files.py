@@ -40,6 +40,9 @@ def _lib():
                                               C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]
         L.cep_write_ord11_qni.argtypes = [C.c_char_p, ip, C.POINTER(C.c_double), C.c_int,
                                           C.POINTER(C.c_double), C.c_int]
+        L.cep_write_time_series.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                            C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int, ip,
+                                            C.POINTER(C.c_double), C.c_int, C.c_int]
         L._io_typed = True
     return L
 
@@ -100,3 +103,22 @@ def write_ord11_qni(prefix: str, sample_ids, bm11, ctl=None, header: bool = True
     c = None if ctl is None else np.ascontiguousarray(ctl, dtype=np.float64).reshape(-1, NBM)
     _chk(_lib().cep_write_ord11_qni(prefix.encode(), idp, _p(b), len(ids), _p(c), int(header)),
          "writing ORd 2011 qNet file")
+
+
+def write_time_series(prefix: str, sample_id: int, ts: dict, history=None, header: bool = True) -> None:
+    """The reference's Is_Print_Graph files of one sample: <prefix>_vmcheck_smp<id>.plt,
+    _ca_i_smp, _ires_smp, _max_dvdt_repol_smp.  ts: CellBatch.time_series(cell); history:
+    CellBatch.dvmdt_repol_history(cell) or None."""
+    t = np.ascontiguousarray(ts["t"], dtype=np.float64)
+    vm = np.ascontiguousarray(ts["vm"], dtype=np.float64)
+    ca = np.ascontiguousarray(ts["cai"], dtype=np.float64)
+    ires = np.ascontiguousarray(ts["ires"], dtype=np.float64).reshape(-1, 6)
+    if history is None:
+        rp, rpp, rv, nrep = None, None, None, 0
+    else:
+        rp, rpp = _ints(history[0])
+        rv = np.ascontiguousarray(history[1], dtype=np.float64)
+        nrep = len(rp)
+    _chk(_lib().cep_write_time_series(prefix.encode(), int(sample_id), _p(t), _p(vm), _p(ca), _p(ires), len(t),
+                                      rpp, _p(rv), nrep, int(header)),
+         "writing time series")

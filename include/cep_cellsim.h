@@ -127,6 +127,11 @@ int cep_set_cells(cep_batch *b, int first, int count, const double *celltype, co
 #define CEP_OPT_INTEGRATOR 3
 #define CEP_INTEGRATOR_RUSH_LARSEN 0
 #define CEP_INTEGRATOR_ADAMS 1
+/* CEP_OPT_TIME_SERIES (ORd-CiPA): nonzero records the time series of the selected (steepest)
+   beat and the per-pace dvmdt_repol history that the reference driver writes with
+   Is_Print_Graph (CEPCell_NEO/modules/cipa_drug_bench.cpp:168-199,230-235,262,338-350):
+   t, Vm, Ca_i and INaL ICaL Ito IKr IKs IK1 at every dt_write (1 ms) point of the beat. */
+#define CEP_OPT_TIME_SERIES 4
 int cep_set_model_option(cep_batch *b, int option, double value);
 
 /* Integration / biomarker configuration shared by the batch.
@@ -180,6 +185,15 @@ int cep_get_stats(const cep_batch *b, long long *out);
 /* Vm at every output point of the most recent beat (enable before running). */
 int cep_enable_vm_trace(cep_batch *b, int enable);
 int cep_get_vm_trace(const cep_batch *b, double *out, int *nsamp);  /* n x nsamp */
+/* The selected beat's time series of one cell (CEP_OPT_TIME_SERIES): *n samples (the first
+   max_n are copied; ires is max_n x 6: INaL ICaL Ito IKr IKs IK1), *pace = the selected pace
+   (may be null).  All output arrays null: size query.  The reference's vm_data / cai_data /
+   ires_data of cipa_result (cipa_drug_bench.cpp:230-235,264-278). */
+int cep_get_time_series(const cep_batch *b, int cell, int max_n, double *t, double *vm, double *cai,
+                        double *ires, int *n, int *pace);
+/* dvmdt_repol of every eligible pace of the window, in pace order (the reference's
+   max_dvdt_repol file, cipa_drug_bench.cpp:262): *n entries, the first max_n copied. */
+int cep_get_dvmdt_repol_history(const cep_batch *b, int cell, int max_n, int *paces, double *values, int *n);
 
 /* average duration (ms) of the integration kernel over the launches since the last
    call, timed with HIP events on the batch stream; returns launches counted */

@@ -56,6 +56,18 @@ int cep_write_cipa_features(const char *prefix, const int *sample_ids, const dou
 int cep_write_ord11_qni(const char *prefix, const int *sample_ids, const double *bm11, int n,
                         const double *ctl, int write_header);
 
+/* Write one sample's time series in the reference's Is_Print_Graph files (opened "w"):
+   <prefix>_vmcheck_smp<id>.plt    "TIME Vm"
+   <prefix>_ca_i_smp<id>.plt       "TIME cai"
+   <prefix>_ires_smp<id>.plt       "TIME INaL ICaL Ito IKr IKs IK1"
+   <prefix>_max_dvdt_repol_smp<id>.plt  "<pace> <dvmdt_repol>" per line, no header
+   (CEPCell_NEO/modules/cipa_drug_bench.cpp:168-177,189-193,262,338-350; "%lf" fields; the
+   headers only when write_header != 0, as the reference writes them for group 0).
+   ires: n x 6; rep_pace / rep_val: nrep entries (may be null with nrep = 0). */
+int cep_write_time_series(const char *prefix, int sample_id, const double *t, const double *vm, const double *cai,
+                          const double *ires, int n, const int *rep_pace, const double *rep_val, int nrep,
+                          int write_header);
+
 #ifdef __cplusplus
 }
 #endif

@@ -100,3 +100,21 @@ def test_ord11_qni_file(tmp_path):
     lines = open(prefix + "_qni.plt").read().splitlines()
     assert lines[0] == "Sample_ID,Qnet_AP,Qnet4_AP,Qinward_CL,Qnet_CL,Qnet4_CL,Qinward_CL"
     assert lines[1] == "3,0.040000,0.002000,1.250000,0.050000,-0.090000,1.250000"   # (2 + 0.5) / 2
+
+
+def test_time_series_files(tmp_path):
+    """The Is_Print_Graph files of cipa_drug_bench.cpp:189-193,262,338-350: headers, one
+    "%lf %lf" line per sample (Vm, Ca_i), "%lf" x 7 for the currents, "%d %lf" per pace."""
+    ts = {"t": np.array([1.0, 2.0]), "vm": np.array([-86.5, 20.25]), "cai": np.array([1e-4, 2.5e-4]),
+          "ires": np.array([[0.1, -0.2, 0.3, 0.4, 0.5, -0.6], [1, 2, 3, 4, 5, 6]], dtype=float)}
+    prefix = str(tmp_path / "drug_3237.00")
+    files.write_time_series(prefix, 7, ts, history=([250, 251], [-0.75, -0.5]))
+    vm = open(prefix + "_vmcheck_smp7.plt").read().splitlines()
+    ca = open(prefix + "_ca_i_smp7.plt").read().splitlines()
+    ir = open(prefix + "_ires_smp7.plt").read().splitlines()
+    rp = open(prefix + "_max_dvdt_repol_smp7.plt").read().splitlines()
+    assert vm == ["TIME Vm", "1.000000 -86.500000", "2.000000 20.250000"]
+    assert ca == ["TIME cai", "1.000000 0.000100", "2.000000 0.000250"]
+    assert ir[0] == "TIME INaL ICaL Ito IKr IKs IK1"
+    assert ir[1] == "1.000000 0.100000 -0.200000 0.300000 0.400000 0.500000 -0.600000"
+    assert rp == ["250 -0.750000", "251 -0.500000"]

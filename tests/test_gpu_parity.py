@@ -209,3 +209,63 @@ def test_failing_cell_does_not_poison_batch():
     assert st[2] != 0
     assert (st[[0, 1, 3]] == 0).all()
     assert np.isfinite(y[[0, 1, 3]]).all()
+
+
+@pytest.mark.gpu
+def test_ord_time_series_of_the_selected_beat():
+    """CEP_OPT_TIME_SERIES (the reference's Is_Print_Graph files).  One beat in the window:
+    the selected beat is that beat, sampled every 1 ms at t = 1..BCL ms; its Vm follows the
+    oracle's Vm trace at the same times to the BDF tolerance of test_ord_vm_trace_vs_oracle,
+    and no Ca_i sample exceeds the Ca peak biomarker (a maximum over every dt point)."""
+    wl = cep.workload.ord_single_drug(seed=20251203)
+    with cep.CellBatch(cep.MODEL_ORD_CIPA, 2, cep.PREC_FP64) as b:
+        b.set_cells(celltype=wl["celltype"][:2], conc=wl["conc"][:2], herg=wl["herg"][:2],
+                    gscale=wl["gscale"][:2], bcl=wl["bcl"][:2])
+        b.set_run_config(dt=0.1, rtol=1e-6, atol=1e-8, nbeats_total=1, nwin=1)
+        b.set_model_option(cep.OPT_TIME_SERIES, 1)
+        b.init_consts()
+        b.run_beats(1)
+        ts = [b.time_series(i) for i in range(2)]
+        bm = b.biomarkers()
+    for i in range(2):
+        cfg = O.make_ord_cfg(celltype=wl["celltype"][i], conc=wl["conc"][i], herg=wl["herg"][i], nbeats=1)
+        _, _, _, vmo, _ = O.ord_run(cfg, trace=True)
+        t = ts[i]["t"]
+        n = int(round(wl["bcl"][i]))
+        assert len(t) == n and ts[i]["pace"] == 0
+        np.testing.assert_allclose(t, np.arange(1, n + 1, dtype=float), rtol=0, atol=1e-9)
+        vo = vmo[np.arange(1, n + 1) * 10 - 1]          # oracle sample k is at t = (k + 1) dt
+        assert np.abs(ts[i]["vm"] - vo).max() <= 0.5
+        assert np.median(np.abs(ts[i]["vm"] - vo)) <= 1e-3
+        if bm["ca_peak"][i] > 0:
+            assert ts[i]["cai"].max() <= bm["ca_peak"][i]
+        assert np.isfinite(ts[i]["ires"]).all() and ts[i]["ires"].shape == (n, 6)
+
+
+@pytest.mark.gpu
+def test_ord_time_series_follow_the_steepest_pace():
+    """Three beats in the window: the history holds one dvmdt_repol per eligible pace, the
+    selected pace is the steepest of them (the reported biomarker), and the kept samples are
+    that pace's, although later beats were recorded after it."""
+    pop = _ord_pop(4)
+    with cep.CellBatch(cep.MODEL_ORD_CIPA, 4, cep.PREC_MIXED) as b:
+        b.set_cells(celltype=pop["celltype"], conc=pop["conc"], herg=pop["herg"], gscale=pop["gscale"],
+                    bcl=pop["bcl"])
+        b.set_run_config(dt=0.1, rtol=1e-6, atol=1e-8, nbeats_total=3, nwin=3)
+        b.set_model_option(cep.OPT_TIME_SERIES, 1)
+        b.init_consts()
+        b.run_beats(3)
+        bm = b.biomarkers()
+        for i in range(4):
+            paces, vals = b.dvmdt_repol_history(i)
+            ts = b.time_series(i)
+            sel = ts["pace"]
+            bcl = pop["bcl"][i]
+            assert sel == int(bm["steepest_pace"][i])
+            assert list(paces) == sorted(paces) and all(0 <= p < 3 for p in paces)
+            if sel in list(paces):
+                k = list(paces).index(sel)
+                assert vals[k] == vals.max() == bm["dvmdt_repol"][i]
+            t = ts["t"]
+            assert len(t) == int(round(bcl))
+            assert t.min() > bcl * sel and t.max() <= bcl * (sel + 1) + 1e-6
