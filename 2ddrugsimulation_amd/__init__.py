@@ -51,6 +51,9 @@ EXPORTED = (
     "cep_tissue_set_conductivity", "cep_tissue_set_stimulus", "cep_tissue_step", "cep_tissue_diffuse",
     "cep_tissue_get_vm", "cep_tissue_get_states", "cep_tissue_set_states", "cep_tissue_time",
     "cep_tissue_kernel_times",
+    "cep_mesh_read_inp", "cep_mesh_create", "cep_mesh_destroy", "cep_mesh_info", "cep_mesh_get_nodes",
+    "cep_mesh_get_elements", "cep_mesh_get_celltype", "cep_mesh_get_targets", "cep_read_pacing_file",
+    "cep_tissue_create_mesh", "cep_tissue_write_checkpoint", "cep_tissue_read_checkpoint",
     "cep_read_ic50_csv", "cep_read_states_file", "cep_write_states_file", "cep_param_defaults",
     "cep_read_input_deck", "cep_param_concentrations", "cep_write_cipa_features", "cep_write_ord11_qni",
     "cep_write_time_series",
@@ -113,6 +116,23 @@ def library(path: str | None = None):
     L.cep_tissue_create.restype = C.c_void_p
     L.cep_tissue_create_model.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int]
     L.cep_tissue_create_model.restype = C.c_void_p
+    ip = C.POINTER(C.c_int)
+    L.cep_mesh_read_inp.argtypes = [C.c_char_p]
+    L.cep_mesh_read_inp.restype = C.c_void_p
+    L.cep_mesh_create.argtypes = [C.c_int, dp, C.c_int, C.c_int, C.c_int, ip, ip]
+    L.cep_mesh_create.restype = C.c_void_p
+    L.cep_mesh_destroy.argtypes = [C.c_void_p]
+    L.cep_mesh_destroy.restype = None
+    L.cep_mesh_info.argtypes = [C.c_void_p, ip, ip, ip, ip, ip]
+    L.cep_mesh_get_nodes.argtypes = [C.c_void_p, dp]
+    L.cep_mesh_get_elements.argtypes = [C.c_void_p, ip]
+    L.cep_mesh_get_celltype.argtypes = [C.c_void_p, ip]
+    L.cep_mesh_get_targets.argtypes = [C.c_void_p, ip]
+    L.cep_read_pacing_file.argtypes = [C.c_char_p, ip, C.c_int, ip]
+    L.cep_tissue_create_mesh.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.cep_tissue_create_mesh.restype = C.c_void_p
+    L.cep_tissue_write_checkpoint.argtypes = [C.c_void_p, C.c_char_p]
+    L.cep_tissue_read_checkpoint.argtypes = [C.c_void_p, C.c_char_p]
     L.cep_tissue_destroy.argtypes = [C.c_void_p]
     L.cep_tissue_destroy.restype = None
     L.cep_tissue_last_error.argtypes = [C.c_void_p]
@@ -314,22 +334,97 @@ class CellBatch:
         return n, avg.value, tot.value
 
 
+class Mesh:
+    """A tissue mesh of the reference's file format (Cmesh::loadInp, mesh.cpp:59-183), read
+    from a file or given as arrays (xyz: nnodes x 3, conn: nelem x nodes_per_elem, 0-based)."""
+
+    def __init__(self, xyz=None, conn=None, celltype=None, dimension: int = 2, path: str | None = None):
+        self.L = library()
+        if path is not None:
+            h = self.L.cep_mesh_read_inp(path.encode())
+            if not h:
+                raise CepError(f"cep_mesh_read_inp({path}) failed")
+        else:
+            x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+            c = np.ascontiguousarray(conn, dtype=np.int32)
+            ct = None if celltype is None else np.ascontiguousarray(celltype, dtype=np.int32)
+            h = self.L.cep_mesh_create(len(x), _p(x), c.shape[0], c.shape[1], dimension,
+                                       c.ctypes.data_as(C.POINTER(C.c_int)),
+                                       None if ct is None else ct.ctypes.data_as(C.POINTER(C.c_int)))
+            if not h:
+                raise CepError("cep_mesh_create failed")
+        self.h = h
+        v = [C.c_int() for _ in range(5)]
+        self.L.cep_mesh_info(h, *[C.byref(q) for q in v])
+        self.nnodes, self.nelem, self.nodes_per_elem, self.dimension, has_ct = (q.value for q in v)
+        self.has_celltype = bool(has_ct)
+
+    @classmethod
+    def read_inp(cls, path: str) -> "Mesh":
+        return cls(path=path)
+
+    def nodes(self) -> np.ndarray:
+        out = np.zeros((self.nnodes, 3))
+        self.L.cep_mesh_get_nodes(self.h, _p(out))
+        return out
+
+    def elements(self) -> np.ndarray:
+        out = np.zeros((self.nelem, self.nodes_per_elem), dtype=np.int32)
+        self.L.cep_mesh_get_elements(self.h, out.ctypes.data_as(C.POINTER(C.c_int)))
+        return out
+
+    def celltype(self):
+        if not self.has_celltype:
+            return None
+        out = np.zeros(self.nnodes, dtype=np.int32)
+        self.L.cep_mesh_get_celltype(self.h, out.ctypes.data_as(C.POINTER(C.c_int)))
+        return out
+
+    def targets(self) -> np.ndarray:
+        out = np.zeros(3, dtype=np.int32)
+        self.L.cep_mesh_get_targets(self.h, out.ctypes.data_as(C.POINTER(C.c_int)))
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.cep_mesh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
 class Tissue:
-    """2-D monodomain tissue of ORd-CiPA nodes on a structured Q1 grid (nx x ny elements of
-    hx x hy mm): the reference's Cheart reaction step + Cfem diffusion step (heart.cpp,
-    fem.cpp) on the device."""
+    """2-D monodomain tissue: the reference's Cheart reaction step + Cfem diffusion step
+    (heart.cpp, fem.cpp) on the device, on a structured Q1 grid (nx x ny elements of hx x hy
+    mm) or on an S4 mesh (Tissue.from_mesh)."""
 
     def __init__(self, nx: int, ny: int, hx: float, hy: float, precision: int = PREC_FP64,
-                 model: int = MODEL_ORD_CIPA):
+                 model: int = MODEL_ORD_CIPA, _handle=None, _nn=None, _ne=None):
         self.L = library()
-        h = self.L.cep_tissue_create_model(nx, ny, hx, hy, model, precision)
+        h = _handle if _handle is not None else self.L.cep_tissue_create_model(nx, ny, hx, hy, model, precision)
         if not h:
             raise CepError(f"cep_tissue_create({nx}, {ny}, {hx}, {hy}) failed")
         self.h = h
         self.nx, self.ny = nx, ny
-        self.nn = (nx + 1) * (ny + 1)
+        self.nn = _nn if _nn is not None else (nx + 1) * (ny + 1)
+        self.ne = _ne if _ne is not None else nx * ny
         self.model = model
         self.nstates = 41 if model == MODEL_ORD2011 else 52
+
+    @classmethod
+    def from_mesh(cls, mesh: "Mesh", precision: int = PREC_FP64, model: int = MODEL_ORD_CIPA) -> "Tissue":
+        L = library()
+        h = L.cep_tissue_create_mesh(mesh.h, model, precision)
+        if not h:
+            raise CepError("cep_tissue_create_mesh failed (an S4 mesh is required)")
+        return cls(0, 0, 0.0, 0.0, precision, model, _handle=h, _nn=mesh.nnodes, _ne=mesh.nelem)
+
+    def write_checkpoint(self, path: str):
+        self._chk(self.L.cep_tissue_write_checkpoint(self.h, path.encode()))
+
+    def read_checkpoint(self, path: str):
+        self._chk(self.L.cep_tissue_read_checkpoint(self.h, path.encode()))
 
     def _chk(self, rc):
         if rc != 0:
@@ -362,7 +457,7 @@ class Tissue:
         self._chk(self.L.cep_tissue_set_cells(self.h, _p(ct), _p(cc), _p(hg), w, _p(gs), 7))
 
     def set_conductivity(self, sxx, sxy, syy):
-        ne = self.nx * self.ny
+        ne = self.ne
         a = [np.ascontiguousarray(np.broadcast_to(np.asarray(v, np.float64), (ne,))) for v in (sxx, sxy, syy)]
         self._chk(self.L.cep_tissue_set_conductivity(self.h, _p(a[0]), _p(a[1]), _p(a[2])))
 

@@ -4,6 +4,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -1804,6 +1805,7 @@ int cep_kernel_time(cep_batch *b, double *avg_ms, double *total_ms)
 
 }  // extern "C"
 
+#include "mesh.hpp"
 #include "tissue.hpp"
 
 #include "io.hpp"

@@ -70,9 +70,21 @@ CEP_D double grid_total(const double *part, int nblk, double *sm, double *bc)
 
 struct Grid {
   int nxn, nyn, nn, nblk;   // nodes per row / column, nodes, workgroups
+  int lin;                  // 1: a mesh's nodes in a 1-D launch of TB-node workgroups
 };
 
 CEP_D int block_id() { return blockIdx.y * gridDim.x + blockIdx.x; }
+
+// this thread's node (-1 outside): a 16x16 tile of the grid, or TB consecutive mesh nodes
+CEP_D int node_index(const Grid &g)
+{
+  if (g.lin) {
+    const int i = blockIdx.x * TB + threadIdx.x;
+    return i < g.nn ? i : -1;
+  }
+  const int ix = blockIdx.x * TX + threadIdx.x % TX, iy = blockIdx.y * TY + threadIdx.x / TX;
+  return (ix < g.nxn && iy < g.nyn) ? iy * g.nxn + ix : -1;
+}
 
 // q = A p over a 16x16 tile (LDS-staged p with halo); partial p.q (mode 1)
 __global__ void __launch_bounds__(TB) cg_spmv(Grid g, const double *__restrict__ A, const double *__restrict__ p,
@@ -107,16 +119,36 @@ __global__ void __launch_bounds__(TB) cg_spmv(Grid g, const double *__restrict__
   if (threadIdx.x == 0) part[P_PQ * g.nblk + block_id()] = v[0];
 }
 
+// q = A p for a mesh matrix in CSR (columns ascending in each row); partial p.q (mode 1)
+__global__ void __launch_bounds__(TB) cg_spmv_csr(Grid g, const int *__restrict__ rp, const int *__restrict__ ci,
+                                                  const double *__restrict__ val, const double *__restrict__ p,
+                                                  double *__restrict__ q, double *__restrict__ part, const Scal *S,
+                                                  int mode)
+{
+  if (mode && S->done) return;
+  __shared__ double sm[TB / 64 * 2];
+  const int i = node_index(g);
+  double v[1] = { 0.0 };
+  if (i >= 0) {
+    double s = 0.0;
+    for (int j = rp[i]; j < rp[i + 1]; ++j) s += val[j] * p[ci[j]];
+    q[i] = s;
+    v[0] = p[i] * s;
+  }
+  if (mode == 0) return;                        // plain product (b = B v)
+  block_sum<1>(v, sm);
+  if (threadIdx.x == 0) part[P_PQ * g.nblk + block_id()] = v[0];
+}
+
 // CG start from x = 0: r = b, z = D^-1 r, p = z; partials r.z (into RZ(even)) and z.z (ZZ0)
 __global__ void __launch_bounds__(TB) cg_init(Grid g, const double *__restrict__ b, const double *__restrict__ dinv,
                                               double *__restrict__ x, double *__restrict__ r, double *__restrict__ z,
                                               double *__restrict__ p, double *__restrict__ part)
 {
   __shared__ double sm[TB / 64 * 2];
-  const int ix = blockIdx.x * TX + threadIdx.x % TX, iy = blockIdx.y * TY + threadIdx.x / TX;
+  const int i = node_index(g);
   double v[2] = { 0.0, 0.0 };
-  if (ix < g.nxn && iy < g.nyn) {
-    const int i = iy * g.nxn + ix;
+  if (i >= 0) {
     const double ri = b[i], zi = dinv[i] * ri;
     x[i] = 0.0; r[i] = ri; z[i] = zi; p[i] = zi;
     v[0] = ri * zi; v[1] = zi * zi;
@@ -140,10 +172,9 @@ __global__ void __launch_bounds__(TB) cg_update(Grid g, const double *__restrict
   const double rz = grid_total(part + ((k & 1) ? P_RZ1 : P_RZ0) * g.nblk, g.nblk, sm, &bc);
   const double pq = grid_total(part + P_PQ * g.nblk, g.nblk, sm, &bc);
   const double a = (pq != 0.0) ? rz / pq : 0.0;    // b = 0: x stays 0
-  const int ix = blockIdx.x * TX + threadIdx.x % TX, iy = blockIdx.y * TY + threadIdx.x / TX;
+  const int i = node_index(g);
   double v[2] = { 0.0, 0.0 };
-  if (ix < g.nxn && iy < g.nyn) {
-    const int i = iy * g.nxn + ix;
+  if (i >= 0) {
     x[i] += a * p[i];
     const double ri = r[i] - a * q[i];
     const double zi = dinv[i] * ri;
@@ -174,11 +205,8 @@ __global__ void __launch_bounds__(TB) cg_dir(Grid g, const double *__restrict__ 
   const double rz_new = grid_total(part + ((k & 1) ? P_RZ0 : P_RZ1) * g.nblk, g.nblk, sm, &bc);
   const double rz_old = grid_total(part + ((k & 1) ? P_RZ1 : P_RZ0) * g.nblk, g.nblk, sm, &bc);
   const double bt = rz_new / rz_old;
-  const int ix = blockIdx.x * TX + threadIdx.x % TX, iy = blockIdx.y * TY + threadIdx.x / TX;
-  if (ix < g.nxn && iy < g.nyn) {
-    const int i = iy * g.nxn + ix;
-    p[i] = z[i] + bt * p[i];
-  }
+  const int i = node_index(g);
+  if (i >= 0) p[i] = z[i] + bt * p[i];
   if (block_id() == 0 && threadIdx.x == 0) S->iters = k + 1;
 }
 
@@ -266,6 +294,13 @@ struct cep_tissue {
   double hx = 0, hy = 0, t = 0, dt_built = -1;
   std::string err;
   std::vector<double> sxx, sxy, syy;       // per element
+  // S4 mesh tissue: node coordinates, connectivity and the CSR pattern of A and B
+  bool mesh = false;
+  int ne = 0, nnz = 0;
+  std::vector<double> xy;                  // nn x 2
+  std::vector<int> conn;                   // ne x 4
+  std::vector<int> rp, ci;                 // CSR rows / columns (ascending)
+  int *d_rp = nullptr, *d_ci = nullptr;
   std::vector<int> stim_nodes;
   double stim_scale = 1.0, stim_bcl = 1000.0, stim_start = 0.0, stim_dur = 0.5;
   double *d_A = nullptr, *d_B = nullptr, *d_dinv = nullptr;
@@ -347,11 +382,71 @@ static void tissue_assemble(cep_tissue *t, double dt, std::vector<double> &A, st
   for (int i = 0; i < nn; ++i) dinv[i] = 1.0 / A[(size_t)4 * nn + i];
 }
 
+// CSR values of A = M/dt + K and B = M/dt on an S4 mesh.  Each element's matrices follow
+// Cfem::set_amat_bmat (fem.cpp:628-661): at the reference's four Gauss points (+-0.57735) the
+// Jacobian of the bilinear map of the element's nodes, its determinant and the physical
+// gradients of the shape functions; K uses the element's conductivity tensor.
+static void tissue_assemble_mesh(cep_tissue *t, double dt, std::vector<double> &A, std::vector<double> &Bm,
+                                 std::vector<double> &dinv)
+{
+  A.assign(t->nnz, 0.0);
+  Bm.assign(t->nnz, 0.0);
+  const double g[2] = { -0.57735, 0.57735 };
+  const double gs[4] = { g[0], g[1], g[1], g[0] }, gt[4] = { g[0], g[0], g[1], g[1] };
+  auto slot = [&](int i, int j) {             // CSR position of (i, j)
+    int lo = t->rp[i], hi = t->rp[i + 1] - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi) / 2;
+      if (t->ci[mid] < j) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+  };
+  for (int e = 0; e < t->ne; ++e) {
+    const int *nd = &t->conn[(size_t)e * 4];
+    double xe[4], ye[4];
+    for (int a = 0; a < 4; ++a) { xe[a] = t->xy[(size_t)nd[a] * 2]; ye[a] = t->xy[(size_t)nd[a] * 2 + 1]; }
+    double Me[16] = { 0 }, Ke[16] = { 0 };
+    for (int gp = 0; gp < 4; ++gp) {
+      const double s = gs[gp], u = gt[gp];
+      const double phi[4] = { (1. - s) * (1. - u) / 4., (1. + s) * (1. - u) / 4., (1. + s) * (1. + u) / 4., (1. - s) * (1. + u) / 4. };
+      const double dps[4] = { -(1. - u) / 4., (1. - u) / 4., (1. + u) / 4., -(1. + u) / 4. };
+      const double dpt[4] = { -(1. - s) / 4., -(1. + s) / 4., (1. + s) / 4., (1. - s) / 4. };
+      double j00 = 0, j01 = 0, j10 = 0, j11 = 0;      // J = dX/d(s,t): rows s, t; columns x, y
+      for (int a = 0; a < 4; ++a) {
+        j00 += dps[a] * xe[a]; j01 += dps[a] * ye[a];
+        j10 += dpt[a] * xe[a]; j11 += dpt[a] * ye[a];
+      }
+      const double det = j00 * j11 - j01 * j10;
+      const double i00 = j11 / det, i01 = -j01 / det, i10 = -j10 / det, i11 = j00 / det;
+      double dx[4], dy[4];
+      for (int a = 0; a < 4; ++a) {          // grad phi = J^-1 (dphi/ds, dphi/dt)
+        dx[a] = i00 * dps[a] + i01 * dpt[a];
+        dy[a] = i10 * dps[a] + i11 * dpt[a];
+      }
+      for (int j = 0; j < 4; ++j)
+        for (int k = 0; k < 4; ++k) {
+          Me[j * 4 + k] += phi[j] * phi[k] * det;
+          Ke[j * 4 + k] += (t->sxx[e] * dx[j] * dx[k] + t->sxy[e] * (dx[j] * dy[k] + dy[j] * dx[k]) +
+                            t->syy[e] * dy[j] * dy[k]) * det;
+        }
+    }
+    for (int j = 0; j < 4; ++j)
+      for (int k = 0; k < 4; ++k) {
+        const int q = slot(nd[j], nd[k]);
+        A[q] += Me[j * 4 + k] / dt + Ke[j * 4 + k];
+        Bm[q] += Me[j * 4 + k] / dt;
+      }
+  }
+  dinv.resize(t->nn);
+  for (int i = 0; i < t->nn; ++i) dinv[i] = 1.0 / A[slot(i, i)];
+}
+
 static int tissue_build(cep_tissue *t, double dt)
 {
   if (t->dt_built == dt) return 0;
   std::vector<double> A, Bm, dinv;
-  tissue_assemble(t, dt, A, Bm, dinv);
+  if (t->mesh) tissue_assemble_mesh(t, dt, A, Bm, dinv);
+  else tissue_assemble(t, dt, A, Bm, dinv);
   hipStream_t s = t->cells->stream;
   TCHK(t, hipMemcpyAsync(t->d_A, A.data(), A.size() * sizeof(double), hipMemcpyHostToDevice, s));
   TCHK(t, hipMemcpyAsync(t->d_B, Bm.data(), Bm.size() * sizeof(double), hipMemcpyHostToDevice, s));
@@ -385,13 +480,22 @@ static int tissue_solve(cep_tissue *t, double dt, double *x, double rtol, int ma
   using namespace cep::tis;
   if (int rc = tissue_build(t, dt)) return rc;
   hipStream_t s = t->cells->stream;
-  const Grid g{ t->nxn, t->nyn, t->nn, ((t->nxn + TX - 1) / TX) * ((t->nyn + TY - 1) / TY) };
-  const dim3 grid((t->nxn + TX - 1) / TX, (t->nyn + TY - 1) / TY), blk(TB);
+  const int nlin = (t->nn + TB - 1) / TB;
+  const Grid g = t->mesh ? Grid{ 0, 0, t->nn, nlin, 1 }
+                         : Grid{ t->nxn, t->nyn, t->nn, ((t->nxn + TX - 1) / TX) * ((t->nyn + TY - 1) / TY), 0 };
+  const dim3 grid = t->mesh ? dim3(nlin) : dim3((t->nxn + TX - 1) / TX, (t->nyn + TY - 1) / TY), blk(TB);
+  // q = M p with the matrix of this tissue (grid stencil or mesh CSR)
+  auto spmv = [&](const double *M, const double *p, double *q, int mode) {
+    if (t->mesh)
+      hipLaunchKernelGGL(cg_spmv_csr, grid, blk, 0, s, g, t->d_rp, t->d_ci, M, p, q, t->d_part, t->d_S, mode);
+    else
+      hipLaunchKernelGGL(cg_spmv, grid, blk, 0, s, g, M, p, q, t->d_part, t->d_S, mode);
+  };
   Scal init{};
   init.rtol2 = rtol * rtol;
   init.maxit = maxit;
   TCHK(t, hipMemcpyAsync(t->d_S, &init, sizeof(Scal), hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(cg_spmv, grid, blk, 0, s, g, t->d_B, x, t->d_b, t->d_part, t->d_S, 0);     // b = B v
+  spmv(t->d_B, x, t->d_b, 0);                                                                   // b = B v
   hipLaunchKernelGGL(cg_init, grid, blk, 0, s, g, t->d_b, t->d_dinv, x, t->d_r, t->d_z, t->d_p, t->d_part);
   TCHK(t, hipGetLastError());
   constexpr int CHUNK = 4;
@@ -401,7 +505,7 @@ static int tissue_solve(cep_tissue *t, double dt, double *x, double rtol, int ma
       // time the first iteration of each chunk (later ones may exit early once converged)
       const bool tm = k == it;
       if (tm) tissue_timed_begin(t, 1);
-      hipLaunchKernelGGL(cg_spmv, grid, blk, 0, s, g, t->d_A, t->d_p, t->d_q, t->d_part, t->d_S, 1);
+      spmv(t->d_A, t->d_p, t->d_q, 1);
       if (tm) { tissue_timed_end(t, 1); tissue_timed_begin(t, 2); }
       hipLaunchKernelGGL(cg_update, grid, blk, 0, s, g, t->d_p, t->d_q, t->d_dinv, x, t->d_r, t->d_z, t->d_part,
                          t->d_S, k);
@@ -519,6 +623,100 @@ cep_tissue *cep_tissue_create_model(int nx, int ny, double hx, double hy, int mo
   return t;
 }
 
+cep_tissue *cep_tissue_create_mesh(const cep_mesh *m, int model, int precision)
+{
+  if (!m || m->npe != 4 || m->dim != 2) return nullptr;          // S4 meshes
+  if (model != CEP_MODEL_ORD_CIPA && model != CEP_MODEL_ORD2011) return nullptr;
+  if (m->nn > 50000000) return nullptr;
+  cep_tissue *t = new cep_tissue();
+  t->mesh = true;
+  t->nn = m->nn; t->ne = m->ne; t->model = model;
+  t->xy.resize((size_t)m->nn * 2);
+  for (int i = 0; i < m->nn; ++i) { t->xy[(size_t)i * 2] = m->xyz[(size_t)i * 3]; t->xy[(size_t)i * 2 + 1] = m->xyz[(size_t)i * 3 + 1]; }
+  t->conn = m->conn;
+  // CSR pattern: every pair of nodes that share an element, columns ascending
+  std::vector<std::vector<int>> cols(t->nn);
+  for (int e = 0; e < t->ne; ++e)
+    for (int j = 0; j < 4; ++j)
+      for (int k = 0; k < 4; ++k) cols[t->conn[(size_t)e * 4 + j]].push_back(t->conn[(size_t)e * 4 + k]);
+  t->rp.assign(t->nn + 1, 0);
+  for (int i = 0; i < t->nn; ++i) {
+    auto &c = cols[i];
+    std::sort(c.begin(), c.end());
+    c.erase(std::unique(c.begin(), c.end()), c.end());
+    if (c.empty()) c.push_back(i);                     // a node in no element: identity row
+    t->rp[i + 1] = t->rp[i] + (int)c.size();
+  }
+  t->nnz = t->rp[t->nn];
+  t->ci.reserve(t->nnz);
+  for (auto &c : cols) t->ci.insert(t->ci.end(), c.begin(), c.end());
+  t->cells = batch_create(model, t->nn, precision, false);
+  if (!t->cells) { delete t; return nullptr; }
+  t->sxx.assign(t->ne, 0.0);
+  t->sxy.assign(t->ne, 0.0);
+  t->syy.assign(t->ne, 0.0);
+  const int nblk = (t->nn + cep::tis::TB - 1) / cep::tis::TB;
+  int rc = 0;
+  rc |= talloc(t, &t->d_A, (size_t)t->nnz);
+  rc |= talloc(t, &t->d_B, (size_t)t->nnz);
+  rc |= talloc(t, &t->d_rp, (size_t)t->nn + 1);
+  rc |= talloc(t, &t->d_ci, (size_t)t->nnz);
+  rc |= talloc(t, &t->d_dinv, t->nn);
+  rc |= talloc(t, &t->d_b, t->nn);
+  rc |= talloc(t, &t->d_r, t->nn);
+  rc |= talloc(t, &t->d_z, t->nn);
+  rc |= talloc(t, &t->d_p, t->nn);
+  rc |= talloc(t, &t->d_q, t->nn);
+  rc |= talloc(t, &t->d_part, (size_t)cep::tis::P_N * nblk);
+  rc |= talloc(t, &t->d_S, 1);
+  t->ev.resize(6);
+  for (auto &e : t->ev) rc |= hipEventCreate(&e) != hipSuccess;
+  if (!rc) {
+    hipStream_t st = t->cells->stream;
+    rc |= hipMemcpyAsync(t->d_rp, t->rp.data(), t->rp.size() * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess;
+    rc |= hipMemcpyAsync(t->d_ci, t->ci.data(), t->ci.size() * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess;
+  }
+  if (rc || hipStreamSynchronize(t->cells->stream) != hipSuccess) {
+    cep_tissue_destroy(t);
+    return nullptr;
+  }
+  return t;
+}
+
+int cep_tissue_write_checkpoint(const cep_tissue *t, const char *path)
+{
+  if (!t || !path) return CEP_E_ARG;
+  cep_batch *b = t->cells;
+  std::vector<double> y((size_t)t->nn * b->ns);
+  if (int rc = cep_tissue_get_states(t, y.data())) return rc;
+  FILE *f = fopen(path, "w");
+  if (!f) return tfail(const_cast<cep_tissue *>(t), CEP_E_ARG, std::string("cannot write ") + path);
+  fprintf(f, "%f \n", t->t);
+  for (int i = 0; i < t->nn; ++i) {
+    for (int j = 0; j < b->ns; ++j) fprintf(f, "%f ", y[(size_t)i * b->ns + j]);
+    fprintf(f, "\n");
+  }
+  fclose(f);
+  return 0;
+}
+
+int cep_tissue_read_checkpoint(cep_tissue *t, const char *path)
+{
+  if (!t || !path) return CEP_E_ARG;
+  FILE *f = fopen(path, "r");
+  if (!f) return tfail(t, CEP_E_ARG, std::string("cannot read ") + path);
+  cep_batch *b = t->cells;
+  double time = 0.0;
+  std::vector<double> y((size_t)t->nn * b->ns);
+  bool ok = fscanf(f, "%lf", &time) == 1;
+  for (size_t k = 0; ok && k < y.size(); ++k) ok = fscanf(f, "%lf", &y[k]) == 1;
+  fclose(f);
+  if (!ok) return tfail(t, CEP_E_ARG, std::string("short or malformed checkpoint ") + path);
+  if (int rc = cep_tissue_set_states(t, y.data())) return rc;
+  t->t = time;
+  return 0;
+}
+
 void cep_tissue_destroy(cep_tissue *t)
 {
   if (!t) return;
@@ -563,7 +761,7 @@ int cep_tissue_set_cells(cep_tissue *t, const double *celltype, const double *co
 int cep_tissue_set_conductivity(cep_tissue *t, const double *sxx, const double *sxy, const double *syy)
 {
   if (!t || !sxx || !syy) return tfail(t, CEP_E_ARG, "conductivity arrays");
-  const size_t ne = (size_t)t->nx * t->ny;
+  const size_t ne = t->mesh ? (size_t)t->ne : (size_t)t->nx * t->ny;
   for (size_t e = 0; e < ne; ++e) {
     t->sxx[e] = sxx[e];
     t->sxy[e] = sxy ? sxy[e] : 0.0;

@@ -122,3 +122,15 @@ def write_time_series(prefix: str, sample_id: int, ts: dict, history=None, heade
     _chk(_lib().cep_write_time_series(prefix.encode(), int(sample_id), _p(t), _p(vm), _p(ca), _p(ires), len(t),
                                       rpp, _p(rv), nrep, int(header)),
          "writing time series")
+
+
+def read_pacing_file(path: str, max_n: int = 10000000) -> np.ndarray:
+    """The S1/S2 pacing-site file (heart.cpp:966-998): 0-based node numbers."""
+    L = _lib()
+    L.cep_read_pacing_file.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]
+    n = C.c_int()
+    _chk(L.cep_read_pacing_file(path.encode(), None, 0, C.byref(n)), "reading pacing file")
+    out = np.zeros(n.value, dtype=np.int32)
+    _chk(L.cep_read_pacing_file(path.encode(), out.ctypes.data_as(C.POINTER(C.c_int)), min(n.value, max_n),
+                                C.byref(n)), "reading pacing file")
+    return out

@@ -8,6 +8,10 @@
  * isotropic case is sxx = syy = DE, sxy = 0).  Me is the mass term, Ke the conduction
  * term; the reference's amat is Me/dt + Ke and bmat is Me.  Node order: (0,0), (hx,0),
  * (hx,hy), (0,hy).
+ * fem_q4_element: the same element on any quadrilateral (x[4], y[4] in the node order
+ * above): at each Gauss point the Jacobian of the bilinear map from the node coordinates,
+ * its determinant and inverse give the physical gradients, as set_amat_bmat does with
+ * MultMatMat / inverseMat (:631-640) for every mesh element.
  * cg_solve_csr: CG with Jacobi preconditioning from a zero initial guess, stopped on
  * the preconditioned residual norm relative to its initial value, as PETSc's KSPCG +
  * PCJACOBI with default tolerances (rtol 1e-5) used by Cfem::assemble_Amat_Bmat
@@ -37,6 +41,39 @@ void fem_q1_element(double hx, double hy, double sxx, double sxy, double syy, do
       for (int k = 0; k < 4; ++k) {
         Me[j * 4 + k] += phi[j] * phi[k] * detJ;
         Ke[j * 4 + k] += (dx[j] * (sxx * dx[k] + sxy * dy[k]) + dy[j] * (sxy * dx[k] + syy * dy[k])) * detJ;
+      }
+  }
+}
+
+void fem_q4_element(const double *xe, const double *ye, double sxx, double sxy, double syy, double *Me, double *Ke)
+{
+  const double g[2] = { -0.57735, 0.57735 };
+  const double gs[4] = { g[0], g[1], g[1], g[0] }, gt[4] = { g[0], g[0], g[1], g[1] };
+  memset(Me, 0, 16 * sizeof(double));
+  memset(Ke, 0, 16 * sizeof(double));
+  for (int gp = 0; gp < 4; ++gp) {
+    const double s = gs[gp], t = gt[gp];
+    const double phi[4] = { (1. - s) * (1. - t) / 4., (1. + s) * (1. - t) / 4.,
+                            (1. + s) * (1. + t) / 4., (1. - s) * (1. + t) / 4. };
+    const double dps[4] = { -(1. - t) / 4., (1. - t) / 4., (1. + t) / 4., -(1. + t) / 4. };
+    const double dpt[4] = { -(1. - s) / 4., -(1. + s) / 4., (1. + s) / 4., (1. - s) / 4. };
+    /* Jacobian (rows: d/ds, d/dt; columns: x, y) = dPhiS^T * xyele */
+    double J[2][2] = { { 0, 0 }, { 0, 0 } };
+    for (int a = 0; a < 4; ++a) {
+      J[0][0] += dps[a] * xe[a]; J[0][1] += dps[a] * ye[a];
+      J[1][0] += dpt[a] * xe[a]; J[1][1] += dpt[a] * ye[a];
+    }
+    const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    const double inv[2][2] = { { J[1][1] / det, -J[0][1] / det }, { -J[1][0] / det, J[0][0] / det } };
+    double dx[4], dy[4];
+    for (int a = 0; a < 4; ++a) {
+      dx[a] = inv[0][0] * dps[a] + inv[0][1] * dpt[a];
+      dy[a] = inv[1][0] * dps[a] + inv[1][1] * dpt[a];
+    }
+    for (int j = 0; j < 4; ++j)
+      for (int k = 0; k < 4; ++k) {
+        Me[j * 4 + k] += phi[j] * phi[k] * det;
+        Ke[j * 4 + k] += (dx[j] * (sxx * dx[k] + sxy * dy[k]) + dy[j] * (sxy * dx[k] + syy * dy[k])) * det;
       }
   }
 }

@@ -101,3 +101,66 @@ def cg_csr(csr, b, rtol, maxit=1000):
                               col.ctypes.data_as(C.POINTER(C.c_int)), O.ptr(val), O.ptr(bb), O.ptr(x),
                               rtol, maxit, 1)
     return x, it
+
+
+def q4_element(xe, ye, sxx, sxy, syy):
+    """The oracle's element matrices on any quadrilateral (nodes counter-clockwise)."""
+    Me = np.zeros(16)
+    Ke = np.zeros(16)
+    x = np.ascontiguousarray(xe, dtype=np.float64)
+    y = np.ascontiguousarray(ye, dtype=np.float64)
+    O.lib().fem_q4_element(O.ptr(x), O.ptr(y), sxx, sxy, syy, O.ptr(Me), O.ptr(Ke))
+    return Me.reshape(4, 4), Ke.reshape(4, 4)
+
+
+def assemble_mesh(xy, conn, sxx, sxy, syy, dt):
+    """Dense A = M/dt + K and B = M/dt of an S4 mesh (xy: nn x 2, conn: ne x 4)."""
+    nn = len(xy)
+    A = np.zeros((nn, nn))
+    Bm = np.zeros((nn, nn))
+    for e, nd in enumerate(conn):
+        Me, Ke = q4_element(xy[nd, 0], xy[nd, 1], sxx[e], sxy[e], syy[e])
+        for j in range(4):
+            for k in range(4):
+                A[nd[j], nd[k]] += Me[j, k] / dt + Ke[j, k]
+                Bm[nd[j], nd[k]] += Me[j, k] / dt
+    return A, Bm
+
+
+def grid_mesh(nx, ny, hx, hy):
+    """The structured grid as an S4 mesh with the grid's node and element numbering."""
+    xs, ys = np.meshgrid(np.arange(nx + 1) * hx, np.arange(ny + 1) * hy)
+    xyz = np.stack([xs.ravel(), ys.ravel(), np.zeros(xs.size)], axis=1)
+    conn = [[(ey + OY[j]) * (nx + 1) + ex + OX[j] for j in range(4)] for ey in range(ny) for ex in range(nx)]
+    return xyz, np.array(conn, dtype=np.int32)
+
+
+def distorted_mesh(nx, ny, h, rng, amp=0.25):
+    """A grid whose interior nodes are moved by up to amp*h: every element stays convex."""
+    xyz, conn = grid_mesh(nx, ny, h, h)
+    ix = np.tile(np.arange(nx + 1), ny + 1)
+    iy = np.repeat(np.arange(ny + 1), nx + 1)
+    inner = (ix > 0) & (ix < nx) & (iy > 0) & (iy < ny)
+    xyz[inner, 0] += rng.uniform(-amp, amp, inner.sum()) * h
+    xyz[inner, 1] += rng.uniform(-amp, amp, inner.sum()) * h
+    return xyz, conn
+
+
+def write_inp(path, xyz, conn, targets=(1, 1, 1), celltype=None, etype="S4"):
+    """A mesh file in the section layout Cmesh::loadInp reads (mesh.cpp:59-183)."""
+    with open(path, "w") as f:
+        f.write("** synthetic mesh\n")
+        f.write("CML\n")
+        f.write(f"{len(xyz)} {len(conn)} {targets[0]} {targets[1]} {targets[2]}\n")
+        f.write("*NODE\n")
+        for i, p in enumerate(xyz):
+            f.write(f"{i + 1}, {p[0]:.17g}, {p[1]:.17g}, {p[2]:.17g}\n")
+        f.write("**HWCOLOR COMP 1 4\n")
+        f.write(f"*ELEMENT,TYPE={etype},ELSET=tissue\n")
+        for e, nd in enumerate(conn):
+            f.write(f"{e + 1}, " + ", ".join(str(int(v) + 1) for v in nd) + "\n")
+        if celltype is not None:
+            f.write("CELLTYPE\n")
+            for c in celltype:
+                f.write(f"{int(c)}\n")
+        f.write("eof\n")
