@@ -54,6 +54,7 @@ EXPORTED = (
     "cep_mesh_read_inp", "cep_mesh_create", "cep_mesh_destroy", "cep_mesh_info", "cep_mesh_get_nodes",
     "cep_mesh_get_elements", "cep_mesh_get_celltype", "cep_mesh_get_targets", "cep_read_pacing_file",
     "cep_tissue_create_mesh", "cep_tissue_write_checkpoint", "cep_tissue_read_checkpoint",
+    "cep_tissue_set_s1s2", "cep_tissue_s2_time",
     "cep_read_ic50_csv", "cep_read_states_file", "cep_write_states_file", "cep_param_defaults",
     "cep_read_input_deck", "cep_param_concentrations", "cep_write_cipa_features", "cep_write_ord11_qni",
     "cep_write_time_series",
@@ -132,6 +133,8 @@ def library(path: str | None = None):
     L.cep_tissue_create_mesh.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.cep_tissue_create_mesh.restype = C.c_void_p
     L.cep_tissue_write_checkpoint.argtypes = [C.c_void_p, C.c_char_p]
+    L.cep_tissue_set_s1s2.argtypes = [C.c_void_p, C.c_int, ip, C.c_int, C.c_int]
+    L.cep_tissue_s2_time.argtypes = [C.c_void_p, dp]
     L.cep_tissue_read_checkpoint.argtypes = [C.c_void_p, C.c_char_p]
     L.cep_tissue_destroy.argtypes = [C.c_void_p]
     L.cep_tissue_destroy.restype = None
@@ -419,6 +422,18 @@ class Tissue:
         if not h:
             raise CepError("cep_tissue_create_mesh failed (an S4 mesh is required)")
         return cls(0, 0, 0.0, 0.0, precision, model, _handle=h, _nn=mesh.nnodes, _ne=mesh.nelem)
+
+    def set_s1s2(self, npace1: int, s2_nodes=(), target: int = 0):
+        """S1 pacing stops after npace1 paces; during the last one the S2 nodes are reset to
+        their initial states once the target node has repolarised (heart.cpp:315-323,475-501)."""
+        idx = np.ascontiguousarray(np.asarray(s2_nodes, dtype=np.int32))
+        self._chk(self.L.cep_tissue_set_s1s2(self.h, int(npace1), idx.ctypes.data_as(C.POINTER(C.c_int)), len(idx),
+                                             int(target)))
+
+    def s2_time(self) -> float:
+        v = C.c_double()
+        self._chk(self.L.cep_tissue_s2_time(self.h, C.byref(v)))
+        return v.value
 
     def write_checkpoint(self, path: str):
         self._chk(self.L.cep_tissue_write_checkpoint(self.h, path.encode()))

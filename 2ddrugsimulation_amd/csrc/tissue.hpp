@@ -210,6 +210,35 @@ __global__ void __launch_bounds__(TB) cg_dir(Grid g, const double *__restrict__ 
   if (block_id() == 0 && threadIdx.x == 0) S->iters = k + 1;
 }
 
+// S2 of the S1-S2 protocol (Cheart::apply_s2, heart.cpp:475-501): one workgroup; the target
+// node's Vm arms the reset when above -45 mV and fires it when below again; the reset writes
+// the S2 nodes' initial states (s2init: count x ns, node-major) and records the time.
+struct S2State {
+  int up, down, done, pad;
+  double time;
+};
+__global__ void __launch_bounds__(256) tissue_s2(int target, int pitch, int ns, double t, double *__restrict__ y,
+                                                 S2State *st, const int *__restrict__ nodes, int count,
+                                                 const double *__restrict__ s2init)
+{
+  __shared__ int fire;
+  if (threadIdx.x == 0) {
+    fire = 0;
+    if (!st->done) {
+      const double v = y[target];
+      if (v > -45.0) st->up = 1;
+      if (st->up && v < -45.0) st->down = 1;
+      if (st->down) { fire = 1; st->done = 1; st->time = t; }
+    }
+  }
+  __syncthreads();
+  if (!fire) return;
+  for (int k = threadIdx.x; k < count * ns; k += blockDim.x) {
+    const int n = k / ns, j = k % ns;
+    y[(size_t)j * pitch + nodes[n]] = s2init[k];
+  }
+}
+
 // reaction step of every node over h (the reference's solveAnalytical role; ORd-CiPA has
 // none there, :1113-1115): Rush-Larsen for the Hodgkin-Huxley gates and the Jrel
 // relaxations, backward Euler for the Markov IKr block (linear in its states; drug binding
@@ -303,6 +332,12 @@ struct cep_tissue {
   int *d_rp = nullptr, *d_ci = nullptr;
   std::vector<int> stim_nodes;
   double stim_scale = 1.0, stim_bcl = 1000.0, stim_start = 0.0, stim_dur = 0.5;
+  // S1-S2 protocol
+  int npace1 = 0, s2_count = 0, s2_target = 0;
+  bool s1_stopped = false;
+  int *d_s2nodes = nullptr;
+  double *d_s2init = nullptr;
+  cep::tis::S2State *d_s2 = nullptr;
   double *d_A = nullptr, *d_B = nullptr, *d_dinv = nullptr;
   double *d_b = nullptr, *d_r = nullptr, *d_z = nullptr, *d_p = nullptr, *d_q = nullptr, *d_part = nullptr;
   cep::tis::Scal *d_S = nullptr;
@@ -532,7 +567,7 @@ static int tissue_react_launch(cep_tissue *t, double h)
     // paced nodes are stimulated while the beat time lies in [start, start + duration]
     // (heart.cpp:287-297); unpaced nodes carry a zero amplitude
     const double tb = t->t - floor(t->t / t->stim_bcl) * t->stim_bcl;
-    const int on = (tb >= t->stim_start && tb <= t->stim_start + t->stim_dur) ? 1 : 0;
+    const int on = (!t->s1_stopped && tb >= t->stim_start && tb <= t->stim_start + t->stim_dur) ? 1 : 0;
     if (b->prec == CEP_PREC_FP64)
       hipLaunchKernelGGL(cep::tis::tissue_react11<double>, dim3(blocks), dim3(64), 0, b->stream, t->nn, b->pitch, t->t,
                          h, on, b->d_y, b->d_prm);
@@ -561,7 +596,8 @@ static int tissue_upload_params(cep_tissue *t)
   const size_t P = b->pitch;
   std::vector<double> prm(P * b->np, 0.0);
   std::vector<char> paced(t->nn, 0);
-  for (int k : t->stim_nodes) paced[k] = 1;
+  if (!t->s1_stopped)
+    for (int k : t->stim_nodes) paced[k] = 1;
   double p[64];
   for (int i = 0; i < t->nn; ++i) {
     if (t->model == CEP_MODEL_ORD2011) {
@@ -786,6 +822,62 @@ int cep_tissue_set_stimulus(cep_tissue *t, const int *nodes, int count, double a
   return tissue_upload_params(t);
 }
 
+int cep_tissue_set_s1s2(cep_tissue *t, int npace1, const int *s2_nodes, int count, int target_node)
+{
+  if (!t || count < 0 || (count && !s2_nodes) || target_node < 0 || target_node >= t->nn)
+    return tfail(t, CEP_E_ARG, "S1-S2 arguments");
+  cep_batch *b = t->cells;
+  const int ns = b->ns;
+  std::vector<double> init((size_t)(count > 0 ? count : 1) * ns, 0.0);
+  for (int k = 0; k < count; ++k) {
+    const int n = s2_nodes[k];
+    if (n < 0 || n >= t->nn) return tfail(t, CEP_E_ARG, "S2 node out of range");
+    double *y = &init[(size_t)k * ns];
+    if (t->model == CEP_MODEL_ORD2011) {
+      for (int j = 0; j < cep::ord11::NS; ++j) y[j] = cep::host::kO11Init[j];
+    } else {
+      for (int j = 0; j < 49; ++j) y[j] = cep::host::kOrdInitStates[j];
+      y[43] = b->ord_in[n].conc;
+    }
+  }
+  if (!t->d_s2) {
+    int rc = talloc(t, &t->d_s2, 1);
+    if (rc) return rc;
+  }
+  if (count > t->s2_count || !t->d_s2nodes) {
+    int rc = talloc(t, &t->d_s2nodes, (size_t)(count > 0 ? count : 1));
+    rc |= talloc(t, &t->d_s2init, init.size());
+    if (rc) return rc;
+  }
+  hipStream_t st = b->stream;
+  const cep::tis::S2State zero{ 0, 0, 0, 0, -1.0 };
+  TCHK(t, hipMemcpyAsync(t->d_s2, &zero, sizeof(zero), hipMemcpyHostToDevice, st));
+  if (count) {
+    TCHK(t, hipMemcpyAsync(t->d_s2nodes, s2_nodes, (size_t)count * sizeof(int), hipMemcpyHostToDevice, st));
+    TCHK(t, hipMemcpyAsync(t->d_s2init, init.data(), init.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  }
+  TCHK(t, hipStreamSynchronize(st));
+  t->npace1 = npace1;
+  t->s2_count = count;
+  t->s2_target = target_node;
+  if (t->s1_stopped) {                         // a new protocol: pacing on again
+    t->s1_stopped = false;
+    return tissue_upload_params(t);
+  }
+  return 0;
+}
+
+int cep_tissue_s2_time(const cep_tissue *t, double *time)
+{
+  if (!t || !time) return CEP_E_ARG;
+  *time = -1.0;
+  if (!t->d_s2) return 0;
+  cep::tis::S2State h{};
+  TCHK(const_cast<cep_tissue *>(t), hipMemcpy(&h, t->d_s2, sizeof(h), hipMemcpyDeviceToHost));
+  *time = h.done ? h.time : -1.0;
+  return 0;
+}
+
 int cep_tissue_step(cep_tissue *t, int nsteps, double dt, int split, double cg_rtol, int cg_maxit,
                     long long *cg_iters)
 {
@@ -796,6 +888,18 @@ int cep_tissue_step(cep_tissue *t, int nsteps, double dt, int split, double cg_r
   double *vm = t->cells->d_y;                  // state plane 0
   for (int s = 0; s < nsteps; ++s) {
     const double t0 = t->t;
+    if (t->npace1 > 0) {                       // heart.cpp:315-323, before the node loop
+      const int pace = (int)(t0 / t->stim_bcl);
+      if (pace >= t->npace1 && !t->s1_stopped) {
+        t->s1_stopped = true;
+        if (int rc = tissue_upload_params(t)) return rc;
+      }
+      if (pace == t->npace1 - 1 && t->s2_count > 0) {
+        cep_batch *b = t->cells;
+        hipLaunchKernelGGL(cep::tis::tissue_s2, dim3(1), dim3(256), 0, b->stream, t->s2_target, b->pitch, b->ns, t0,
+                           b->d_y, t->d_s2, t->d_s2nodes, t->s2_count, t->d_s2init);
+      }
+    }
     if (split == CEP_SPLIT_GODUNOV) {
       if (int rc = tissue_react_launch(t, dt)) return rc;
     } else {

@@ -49,6 +49,14 @@ int cep_tissue_set_conductivity(cep_tissue *t, const double *sxx, const double *
    and duration in ms; the reference's isS1 nodes, heart.cpp:287-318). */
 int cep_tissue_set_stimulus(cep_tissue *t, const int *nodes, int count, double amp_scale, double bcl,
                             double start, double duration);
+/* S1-S2 protocol of the reference driver (heart.cpp:315-323, Cheart::apply_s2 :475-501):
+   S1 pacing stops once int(t / bcl) >= npace1 (npace1 <= 0: never); during the last S1
+   pace, once the target node's Vm has risen above -45 mV and fallen below it again, the
+   listed S2 nodes are reset to their initial states (once).  Call after cep_tissue_set_cells
+   (the initial states are the nodes' own) and cep_tissue_set_stimulus (bcl). */
+int cep_tissue_set_s1s2(cep_tissue *t, int npace1, const int *s2_nodes, int count, int target_node);
+/* time (ms) at which the S2 nodes were reset, or a negative value if not yet */
+int cep_tissue_s2_time(const cep_tissue *t, double *time);
 
 #define CEP_SPLIT_GODUNOV 0    /* reaction(dt), then diffusion(dt): the reference's order */
 #define CEP_SPLIT_STRANG 1     /* reaction(dt/2), diffusion(dt), reaction(dt/2) */

@@ -171,3 +171,47 @@ def test_checkpoint_round_trip(tmp_path):
         tis2.read_checkpoint(p)
         np.testing.assert_allclose(tis2.states(), y, rtol=0, atol=5e-7)   # "%f": six decimals
         assert abs(tis2.time - t0) <= 5e-7
+
+
+@pytest.mark.gpu
+def test_s1s2_protocol():
+    """S1-S2 (heart.cpp:315-323, 475-501) on uncoupled ORd 2011 nodes, nodes 0 and 1 paced at
+    BCL 400 ms for one S1 pace.  Inside that pace, once node 0 has gone above -45 mV and back
+    below, every S2 node is reset to the initial states, once; from then on each node follows
+    the trajectory of a fresh unpaced node started at the reset time.  Pacing stops at 400 ms:
+    without the protocol the paced nodes fire again at 400 ms, with it they do not."""
+    dt, t_end = 0.02, 450.0
+    nsteps = int(round(t_end / dt))
+
+    def run(protocol, stim_start=0.0, steps=nsteps):
+        with cep.Tissue(4, 3, 0.1, 0.1, model=cep.MODEL_ORD2011) as tis:
+            tis.set_cells()
+            tis.set_conductivity(0.0, 0.0, 0.0)
+            tis.set_stimulus([0, 1], amp_scale=1.0, bcl=400.0, start=stim_start, duration=1.0)
+            if protocol is not None:
+                tis.set_s1s2(1, protocol, target=0)
+            v410 = None
+            done = 0
+            while done < steps:
+                k = min(500, steps - done)
+                tis.step(k, dt, cg_rtol=1e-10)
+                done += k
+                if v410 is None and done * dt >= 410.0 - 1e-9:
+                    v410 = tis.vm()
+            return tis.s2_time(), tis.states(), v410
+
+    t2, y, v410 = run(list(range(20)))
+    assert 150.0 < t2 < 400.0
+    # every node, paced or not, restarted from the same states at t2
+    assert np.abs(np.ptp(y, axis=0)).max() <= 1e-9 * np.abs(y).max()
+    # ... and matches a node that was never paced, stepped from the initial states for the
+    # same number of steps (its stimulus window lies beyond the beat)
+    k2 = int(round(t2 / dt))
+    _, yf, _ = run(None, stim_start=1e4, steps=nsteps - k2)
+    np.testing.assert_allclose(y, yf, rtol=1e-9, atol=1e-12)
+    assert v410[[0, 1]].max() < -70.0
+    # S1 stop alone: no S2 reset, no fire at 400 ms; no protocol: the paced nodes fire again
+    t2n, _, v410n = run([])
+    assert t2n < 0 and v410n[[0, 1]].max() < -70.0
+    _, _, v410p = run(None, steps=int(round(410.0 / dt)))
+    assert v410p[[0, 1]].min() > -20.0
