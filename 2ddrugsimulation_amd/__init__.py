@@ -54,7 +54,7 @@ EXPORTED = (
     "cep_mesh_read_inp", "cep_mesh_create", "cep_mesh_destroy", "cep_mesh_info", "cep_mesh_get_nodes",
     "cep_mesh_get_elements", "cep_mesh_get_celltype", "cep_mesh_get_targets", "cep_read_pacing_file",
     "cep_tissue_create_mesh", "cep_tissue_write_checkpoint", "cep_tissue_read_checkpoint",
-    "cep_tissue_set_s1s2", "cep_tissue_s2_time",
+    "cep_tissue_set_s1s2", "cep_tissue_s2_time", "cep_tissue_set_conductivity_tensor",
     "cep_read_ic50_csv", "cep_read_states_file", "cep_write_states_file", "cep_param_defaults",
     "cep_read_input_deck", "cep_param_concentrations", "cep_write_cipa_features", "cep_write_ord11_qni",
     "cep_write_time_series",
@@ -135,6 +135,7 @@ def library(path: str | None = None):
     L.cep_tissue_write_checkpoint.argtypes = [C.c_void_p, C.c_char_p]
     L.cep_tissue_set_s1s2.argtypes = [C.c_void_p, C.c_int, ip, C.c_int, C.c_int]
     L.cep_tissue_s2_time.argtypes = [C.c_void_p, dp]
+    L.cep_tissue_set_conductivity_tensor.argtypes = [C.c_void_p, dp, C.c_int]
     L.cep_tissue_read_checkpoint.argtypes = [C.c_void_p, C.c_char_p]
     L.cep_tissue_destroy.argtypes = [C.c_void_p]
     L.cep_tissue_destroy.restype = None
@@ -398,9 +399,10 @@ class Mesh:
 
 
 class Tissue:
-    """2-D monodomain tissue: the reference's Cheart reaction step + Cfem diffusion step
+    """Monodomain tissue: the reference's Cheart reaction step + Cfem diffusion step
     (heart.cpp, fem.cpp) on the device, on a structured Q1 grid (nx x ny elements of hx x hy
-    mm) or on an S4 mesh (Tissue.from_mesh)."""
+    mm) or on a mesh of S4 quadrilaterals, C3D8I hexahedra or C3D4 tetrahedra
+    (Tissue.from_mesh)."""
 
     def __init__(self, nx: int, ny: int, hx: float, hy: float, precision: int = PREC_FP64,
                  model: int = MODEL_ORD_CIPA, _handle=None, _nn=None, _ne=None):
@@ -413,6 +415,7 @@ class Tissue:
         self.nn = _nn if _nn is not None else (nx + 1) * (ny + 1)
         self.ne = _ne if _ne is not None else nx * ny
         self.model = model
+        self.dimension = 2
         self.nstates = 41 if model == MODEL_ORD2011 else 52
 
     @classmethod
@@ -420,8 +423,21 @@ class Tissue:
         L = library()
         h = L.cep_tissue_create_mesh(mesh.h, model, precision)
         if not h:
-            raise CepError("cep_tissue_create_mesh failed (an S4 mesh is required)")
-        return cls(0, 0, 0.0, 0.0, precision, model, _handle=h, _nn=mesh.nnodes, _ne=mesh.nelem)
+            raise CepError("cep_tissue_create_mesh failed (S4, C3D8I or C3D4 elements with a positive "
+                           "Jacobian at every Gauss point are required)")
+        t = cls(0, 0, 0.0, 0.0, precision, model, _handle=h, _nn=mesh.nnodes, _ne=mesh.nelem)
+        t.dimension = mesh.dimension
+        return t
+
+    def set_conductivity_tensor(self, d):
+        """Per-element tensor: shape (ne,) or a scalar = isotropic DE (fem.cpp:162-170);
+        (ne, 3) = 2-D (xx, xy, yy); (ne, 6) = 3-D (xx, xy, xz, yy, yz, zz)."""
+        a = np.asarray(d, dtype=np.float64)
+        if a.ndim == 0:
+            a = np.full(self.ne, float(a))
+        ncomp = 1 if a.ndim == 1 else a.shape[1]
+        a = np.ascontiguousarray(a.reshape(self.ne, ncomp))
+        self._chk(self.L.cep_tissue_set_conductivity_tensor(self.h, _p(a), ncomp))
 
     def set_s1s2(self, npace1: int, s2_nodes=(), target: int = 0):
         """S1 pacing stops after npace1 paces; during the last one the S2 nodes are reset to

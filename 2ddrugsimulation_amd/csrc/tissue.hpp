@@ -1,5 +1,6 @@
-// tissue.hpp -- 2-D monodomain tissue step on gfx950: ORd-CiPA reaction per node + Q1 FEM
-// diffusion solved by Jacobi-preconditioned CG.  Included once, at the end of cep_gpu.hip
+// tissue.hpp -- monodomain tissue step on gfx950: ORd-CiPA reaction per node + FEM diffusion
+// (a 2-D Q1 grid, or a mesh of quadrilaterals, hexahedra or tetrahedra in CSR) solved by
+// Jacobi-preconditioned CG.  Included once, at the end of cep_gpu.hip
 // (it uses the batch internals defined there).
 //
 // Reference: Cheart::cellular_reaction (heart.cpp:274-430) and Cfem::set_amat_bmat /
@@ -323,11 +324,14 @@ struct cep_tissue {
   double hx = 0, hy = 0, t = 0, dt_built = -1;
   std::string err;
   std::vector<double> sxx, sxy, syy;       // per element
-  // S4 mesh tissue: node coordinates, connectivity and the CSR pattern of A and B
+  // mesh tissue: element kind (quadrilateral, hexahedron, tetrahedron), node coordinates,
+  // connectivity, tensor per element and the CSR pattern of A and B
   bool mesh = false;
+  int ekind = 0, npe = 4, dim = 2;
   int ne = 0, nnz = 0;
-  std::vector<double> xy;                  // nn x 2
-  std::vector<int> conn;                   // ne x 4
+  std::vector<double> xyz;                 // nn x 3
+  std::vector<int> conn;                   // ne x npe
+  std::vector<double> dten;                // ne x 6: xx, xy, xz, yy, yz, zz
   std::vector<int> rp, ci;                 // CSR rows / columns (ascending)
   int *d_rp = nullptr, *d_ci = nullptr;
   std::vector<int> stim_nodes;
@@ -421,13 +425,139 @@ static void tissue_assemble(cep_tissue *t, double dt, std::vector<double> &A, st
 // Cfem::set_amat_bmat (fem.cpp:628-661): at the reference's four Gauss points (+-0.57735) the
 // Jacobian of the bilinear map of the element's nodes, its determinant and the physical
 // gradients of the shape functions; K uses the element's conductivity tensor.
+// Element tables of the mesh tissue (fem.cpp: set_gpt_* :209-270, set_Phi_dPhiS_* :398-479):
+// shape functions and their derivatives in reference coordinates at the Gauss points, with
+// the reference's points (+-0.57735 for the quadrilateral and the hexahedron, the 4-point
+// rule (5 -+ sqrt 5)/20 for the tetrahedron) and node orders.
+enum ElemKind { EK_QUAD4 = 0, EK_HEX8 = 1, EK_TET4 = 2 };
+
+struct ElemTable {
+  int npe = 0, dim = 0, ng = 0;
+  double w[8];                 // quadrature weight per point
+  double phi[8][8];            // [point][node]
+  double dphi[8][8][3];        // [point][node][reference coordinate]
+};
+
+static ElemTable elem_table(int kind)
+{
+  ElemTable E{};
+  const double a = 0.57735;
+  if (kind == EK_QUAD4) {
+    E.npe = 4; E.dim = 2; E.ng = 4;
+    const double gs[4] = { -a, a, a, -a }, gt[4] = { -a, -a, a, a };
+    const int sx[4] = { -1, 1, 1, -1 }, sy[4] = { -1, -1, 1, 1 };
+    for (int g = 0; g < 4; ++g) {
+      E.w[g] = 1.0;
+      for (int n = 0; n < 4; ++n) {
+        E.phi[g][n] = (1 + sx[n] * gs[g]) * (1 + sy[n] * gt[g]) / 4.;
+        E.dphi[g][n][0] = sx[n] * (1 + sy[n] * gt[g]) / 4.;
+        E.dphi[g][n][1] = sy[n] * (1 + sx[n] * gs[g]) / 4.;
+      }
+    }
+  } else if (kind == EK_HEX8) {
+    // points in the order of the reference's ix, iy, iz loops; nodes: the bottom face
+    // counter-clockwise, then the top face
+    E.npe = 8; E.dim = 3; E.ng = 8;
+    const int sx[8] = { -1, 1, 1, -1, -1, 1, 1, -1 }, sy[8] = { -1, -1, 1, 1, -1, -1, 1, 1 };
+    const int sz[8] = { -1, -1, -1, -1, 1, 1, 1, 1 };
+    int g = 0;
+    for (int ix = 0; ix < 2; ++ix)
+      for (int iy = 0; iy < 2; ++iy)
+        for (int iz = 0; iz < 2; ++iz, ++g) {
+          const double p[3] = { ix ? a : -a, iy ? a : -a, iz ? a : -a };
+          E.w[g] = 1.0;
+          for (int n = 0; n < 8; ++n) {
+            const double fx = 1 + sx[n] * p[0], fy = 1 + sy[n] * p[1], fz = 1 + sz[n] * p[2];
+            E.phi[g][n] = fx * fy * fz / 8.;
+            E.dphi[g][n][0] = sx[n] * fy * fz / 8.;
+            E.dphi[g][n][1] = sy[n] * fx * fz / 8.;
+            E.dphi[g][n][2] = sz[n] * fx * fy / 8.;
+          }
+        }
+  } else {
+    // linear tetrahedron: phi = (1-s-t-z, s, t, z).  The reference sums its four points with
+    // weight 1, which scales both A and B by 24 and leaves A x = B v unchanged; the weight
+    // 1/24 here makes the mass matrix integrate to the volume.
+    E.npe = 4; E.dim = 3; E.ng = 4;
+    const double lo = (5. - sqrt(5.)) / 20., hi = (5. + 3. * sqrt(5.)) / 20.;
+    const double P[4][3] = { { lo, lo, lo }, { lo, lo, hi }, { lo, hi, lo }, { hi, lo, lo } };
+    for (int g = 0; g < 4; ++g) {
+      E.w[g] = 1.0 / 24.0;
+      E.phi[g][0] = 1. - P[g][0] - P[g][1] - P[g][2];
+      for (int n = 1; n < 4; ++n) E.phi[g][n] = P[g][n - 1];
+      for (int c = 0; c < 3; ++c) {
+        E.dphi[g][0][c] = -1.;
+        for (int n = 1; n < 4; ++n) E.dphi[g][n][c] = (n - 1 == c) ? 1. : 0.;
+      }
+    }
+  }
+  return E;
+}
+
+// Element mass and conduction matrices (npe x npe) of one element at coordinates X (npe x 3)
+// with the tensor D = (xx, xy, xz, yy, yz, zz), as Cfem::set_amat_bmat (fem.cpp:628-661):
+// per Gauss point J = dPhiS^T X, grad phi = J^-1 dphi/d(reference), then
+// Me += w phi_j phi_k det J and Ke += w grad phi_j . D grad phi_k det J.  Returns the smallest
+// det J over the points (a folded or inverted element has one <= 0).
+static double elem_matrices(const ElemTable &E, const double (*X)[3], const double *D, double *Me, double *Ke)
+{
+  const int n = E.npe, d = E.dim;
+  const double Dm[3][3] = { { D[0], D[1], D[2] }, { D[1], D[3], D[4] }, { D[2], D[4], D[5] } };
+  for (int q = 0; q < n * n; ++q) { Me[q] = 0.0; Ke[q] = 0.0; }
+  double detmin = 1e300;
+  for (int g = 0; g < E.ng; ++g) {
+    double J[3][3] = { { 0, 0, 0 }, { 0, 0, 0 }, { 0, 0, 0 } };   // rows: reference coord., columns: x, y, z
+    for (int r = 0; r < d; ++r)
+      for (int c = 0; c < d; ++c)
+        for (int a = 0; a < n; ++a) J[r][c] += E.dphi[g][a][r] * X[a][c];
+    double inv[3][3] = { { 0, 0, 0 }, { 0, 0, 0 }, { 0, 0, 0 } }, det;
+    if (d == 2) {
+      det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+      inv[0][0] = J[1][1] / det; inv[0][1] = -J[0][1] / det;
+      inv[1][0] = -J[1][0] / det; inv[1][1] = J[0][0] / det;
+    } else {
+      const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1], c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2],
+                   c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+      det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+      // inverse = adjugate / det (the adjugate is the transposed cofactor matrix)
+      inv[0][0] = c00 / det;
+      inv[1][0] = c01 / det;
+      inv[2][0] = c02 / det;
+      inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+      inv[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+      inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+      inv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+      inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+      inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+    }
+    detmin = det < detmin ? det : detmin;
+    double gr[8][3] = {};
+    for (int a = 0; a < n; ++a)
+      for (int c = 0; c < d; ++c)
+        for (int r = 0; r < d; ++r) gr[a][c] += inv[c][r] * E.dphi[g][a][r];
+    const double wd = E.w[g] * det;
+    for (int j = 0; j < n; ++j) {
+      double Dg[3] = { 0, 0, 0 };
+      for (int c = 0; c < d; ++c)
+        for (int r = 0; r < d; ++r) Dg[c] += Dm[c][r] * gr[j][r];
+      for (int k = 0; k < n; ++k) {
+        Me[j * n + k] += E.phi[g][j] * E.phi[g][k] * wd;
+        double kk = 0.0;
+        for (int c = 0; c < d; ++c) kk += Dg[c] * gr[k][c];
+        Ke[j * n + k] += kk * wd;
+      }
+    }
+  }
+  return detmin;
+}
+
 static void tissue_assemble_mesh(cep_tissue *t, double dt, std::vector<double> &A, std::vector<double> &Bm,
                                  std::vector<double> &dinv)
 {
   A.assign(t->nnz, 0.0);
   Bm.assign(t->nnz, 0.0);
-  const double g[2] = { -0.57735, 0.57735 };
-  const double gs[4] = { g[0], g[1], g[1], g[0] }, gt[4] = { g[0], g[0], g[1], g[1] };
+  const ElemTable E = elem_table(t->ekind);
+  const int n = E.npe;
   auto slot = [&](int i, int j) {             // CSR position of (i, j)
     int lo = t->rp[i], hi = t->rp[i + 1] - 1;
     while (lo < hi) {
@@ -436,40 +566,17 @@ static void tissue_assemble_mesh(cep_tissue *t, double dt, std::vector<double> &
     }
     return lo;
   };
+  double Me[64], Ke[64], X[8][3];
   for (int e = 0; e < t->ne; ++e) {
-    const int *nd = &t->conn[(size_t)e * 4];
-    double xe[4], ye[4];
-    for (int a = 0; a < 4; ++a) { xe[a] = t->xy[(size_t)nd[a] * 2]; ye[a] = t->xy[(size_t)nd[a] * 2 + 1]; }
-    double Me[16] = { 0 }, Ke[16] = { 0 };
-    for (int gp = 0; gp < 4; ++gp) {
-      const double s = gs[gp], u = gt[gp];
-      const double phi[4] = { (1. - s) * (1. - u) / 4., (1. + s) * (1. - u) / 4., (1. + s) * (1. + u) / 4., (1. - s) * (1. + u) / 4. };
-      const double dps[4] = { -(1. - u) / 4., (1. - u) / 4., (1. + u) / 4., -(1. + u) / 4. };
-      const double dpt[4] = { -(1. - s) / 4., -(1. + s) / 4., (1. + s) / 4., (1. - s) / 4. };
-      double j00 = 0, j01 = 0, j10 = 0, j11 = 0;      // J = dX/d(s,t): rows s, t; columns x, y
-      for (int a = 0; a < 4; ++a) {
-        j00 += dps[a] * xe[a]; j01 += dps[a] * ye[a];
-        j10 += dpt[a] * xe[a]; j11 += dpt[a] * ye[a];
-      }
-      const double det = j00 * j11 - j01 * j10;
-      const double i00 = j11 / det, i01 = -j01 / det, i10 = -j10 / det, i11 = j00 / det;
-      double dx[4], dy[4];
-      for (int a = 0; a < 4; ++a) {          // grad phi = J^-1 (dphi/ds, dphi/dt)
-        dx[a] = i00 * dps[a] + i01 * dpt[a];
-        dy[a] = i10 * dps[a] + i11 * dpt[a];
-      }
-      for (int j = 0; j < 4; ++j)
-        for (int k = 0; k < 4; ++k) {
-          Me[j * 4 + k] += phi[j] * phi[k] * det;
-          Ke[j * 4 + k] += (t->sxx[e] * dx[j] * dx[k] + t->sxy[e] * (dx[j] * dy[k] + dy[j] * dx[k]) +
-                            t->syy[e] * dy[j] * dy[k]) * det;
-        }
-    }
-    for (int j = 0; j < 4; ++j)
-      for (int k = 0; k < 4; ++k) {
+    const int *nd = &t->conn[(size_t)e * n];
+    for (int a = 0; a < n; ++a)
+      for (int c = 0; c < 3; ++c) X[a][c] = t->xyz[(size_t)nd[a] * 3 + c];
+    elem_matrices(E, X, &t->dten[(size_t)e * 6], Me, Ke);
+    for (int j = 0; j < n; ++j)
+      for (int k = 0; k < n; ++k) {
         const int q = slot(nd[j], nd[k]);
-        A[q] += Me[j * 4 + k] / dt + Ke[j * 4 + k];
-        Bm[q] += Me[j * 4 + k] / dt;
+        A[q] += Me[j * n + k] / dt + Ke[j * n + k];
+        Bm[q] += Me[j * n + k] / dt;
       }
   }
   dinv.resize(t->nn);
@@ -661,20 +768,38 @@ cep_tissue *cep_tissue_create_model(int nx, int ny, double hx, double hy, int mo
 
 cep_tissue *cep_tissue_create_mesh(const cep_mesh *m, int model, int precision)
 {
-  if (!m || m->npe != 4 || m->dim != 2) return nullptr;          // S4 meshes
+  if (!m) return nullptr;
+  // the element types the reference's FEM assembles (fem.cpp:830-860): square, hex, tet
+  int kind;
+  if (m->npe == 4 && m->dim == 2) kind = EK_QUAD4;
+  else if (m->npe == 8 && m->dim == 3) kind = EK_HEX8;
+  else if (m->npe == 4 && m->dim == 3) kind = EK_TET4;
+  else return nullptr;
   if (model != CEP_MODEL_ORD_CIPA && model != CEP_MODEL_ORD2011) return nullptr;
   if (m->nn > 50000000) return nullptr;
+  const int n = m->npe;
+  // every element must map the reference element one-to-one: det J > 0 at its Gauss points
+  {
+    const ElemTable E = elem_table(kind);
+    const double D[6] = { 1, 0, 0, 1, 0, 1 };
+    double Me[64], Ke[64], X[8][3];
+    for (int e = 0; e < m->ne; ++e) {
+      for (int a = 0; a < n; ++a)
+        for (int c = 0; c < 3; ++c) X[a][c] = m->xyz[(size_t)m->conn[(size_t)e * n + a] * 3 + c];
+      if (!(elem_matrices(E, X, D, Me, Ke) > 0.0)) return nullptr;
+    }
+  }
   cep_tissue *t = new cep_tissue();
   t->mesh = true;
+  t->ekind = kind; t->npe = n; t->dim = kind == EK_QUAD4 ? 2 : 3;
   t->nn = m->nn; t->ne = m->ne; t->model = model;
-  t->xy.resize((size_t)m->nn * 2);
-  for (int i = 0; i < m->nn; ++i) { t->xy[(size_t)i * 2] = m->xyz[(size_t)i * 3]; t->xy[(size_t)i * 2 + 1] = m->xyz[(size_t)i * 3 + 1]; }
+  t->xyz = m->xyz;
   t->conn = m->conn;
   // CSR pattern: every pair of nodes that share an element, columns ascending
   std::vector<std::vector<int>> cols(t->nn);
   for (int e = 0; e < t->ne; ++e)
-    for (int j = 0; j < 4; ++j)
-      for (int k = 0; k < 4; ++k) cols[t->conn[(size_t)e * 4 + j]].push_back(t->conn[(size_t)e * 4 + k]);
+    for (int j = 0; j < n; ++j)
+      for (int k = 0; k < n; ++k) cols[t->conn[(size_t)e * n + j]].push_back(t->conn[(size_t)e * n + k]);
   t->rp.assign(t->nn + 1, 0);
   for (int i = 0; i < t->nn; ++i) {
     auto &c = cols[i];
@@ -686,11 +811,9 @@ cep_tissue *cep_tissue_create_mesh(const cep_mesh *m, int model, int precision)
   t->nnz = t->rp[t->nn];
   t->ci.reserve(t->nnz);
   for (auto &c : cols) t->ci.insert(t->ci.end(), c.begin(), c.end());
+  t->dten.assign((size_t)t->ne * 6, 0.0);
   t->cells = batch_create(model, t->nn, precision, false);
   if (!t->cells) { delete t; return nullptr; }
-  t->sxx.assign(t->ne, 0.0);
-  t->sxy.assign(t->ne, 0.0);
-  t->syy.assign(t->ne, 0.0);
   const int nblk = (t->nn + cep::tis::TB - 1) / cep::tis::TB;
   int rc = 0;
   rc |= talloc(t, &t->d_A, (size_t)t->nnz);
@@ -794,16 +917,59 @@ int cep_tissue_set_cells(cep_tissue *t, const double *celltype, const double *co
   return tissue_upload_params(t);
 }
 
+// a symmetric 3x3 tensor (xx, xy, xz, yy, yz, zz) is positive semi-definite when every
+// principal minor is non-negative
+static bool tensor_psd(const double *D)
+{
+  const double xx = D[0], xy = D[1], xz = D[2], yy = D[3], yz = D[4], zz = D[5];
+  if (!(xx >= 0) || !(yy >= 0) || !(zz >= 0)) return false;
+  const double tol = 1e-12 * (xx + yy + zz) * (xx + yy + zz);
+  if (xx * yy - xy * xy < -tol || xx * zz - xz * xz < -tol || yy * zz - yz * yz < -tol) return false;
+  const double det = xx * (yy * zz - yz * yz) - xy * (xy * zz - yz * xz) + xz * (xy * yz - yy * xz);
+  return det >= -tol * (xx + yy + zz);
+}
+
 int cep_tissue_set_conductivity(cep_tissue *t, const double *sxx, const double *sxy, const double *syy)
 {
   if (!t || !sxx || !syy) return tfail(t, CEP_E_ARG, "conductivity arrays");
-  const size_t ne = t->mesh ? (size_t)t->ne : (size_t)t->nx * t->ny;
+  if (t->mesh && t->dim == 3) return tfail(t, CEP_E_ARG, "3-D mesh: give the tensor with cep_tissue_set_conductivity_tensor");
+  if (t->mesh) {
+    std::vector<double> d((size_t)t->ne * 3);
+    for (int e = 0; e < t->ne; ++e) { d[e * 3] = sxx[e]; d[e * 3 + 1] = sxy ? sxy[e] : 0.0; d[e * 3 + 2] = syy[e]; }
+    return cep_tissue_set_conductivity_tensor(t, d.data(), 3);
+  }
+  const size_t ne = (size_t)t->nx * t->ny;
   for (size_t e = 0; e < ne; ++e) {
     t->sxx[e] = sxx[e];
     t->sxy[e] = sxy ? sxy[e] : 0.0;
     t->syy[e] = syy[e];
     if (!(sxx[e] >= 0) || !(syy[e] >= 0) || t->sxy[e] * t->sxy[e] > sxx[e] * syy[e])
       return tfail(t, CEP_E_ARG, "diffusion tensor not positive semi-definite");
+  }
+  t->dt_built = -1;
+  return 0;
+}
+
+int cep_tissue_set_conductivity_tensor(cep_tissue *t, const double *d, int ncomp)
+{
+  if (!t || !d) return tfail(t, CEP_E_ARG, "conductivity array");
+  const int dim = t->mesh ? t->dim : 2;
+  if (!(ncomp == 1 || (dim == 2 && ncomp == 3) || (dim == 3 && ncomp == 6)))
+    return tfail(t, CEP_E_ARG, "conductivity components: 1 (isotropic), 3 (2-D) or 6 (3-D)");
+  const size_t ne = t->mesh ? (size_t)t->ne : (size_t)t->nx * t->ny;
+  std::vector<double> D(ne * 6, 0.0);
+  for (size_t e = 0; e < ne; ++e) {
+    const double *v = d + e * ncomp;
+    double *o = &D[e * 6];
+    if (ncomp == 1) { o[0] = v[0]; o[3] = v[0]; o[5] = dim == 3 ? v[0] : 0.0; }
+    else if (ncomp == 3) { o[0] = v[0]; o[1] = v[1]; o[3] = v[2]; }
+    else for (int k = 0; k < 6; ++k) o[k] = v[k];
+    if (!tensor_psd(o)) return tfail(t, CEP_E_ARG, "diffusion tensor not positive semi-definite");
+  }
+  if (t->mesh) {
+    t->dten.swap(D);
+  } else {
+    for (size_t e = 0; e < ne; ++e) { t->sxx[e] = D[e * 6]; t->sxy[e] = D[e * 6 + 1]; t->syy[e] = D[e * 6 + 3]; }
   }
   t->dt_built = -1;
   return 0;

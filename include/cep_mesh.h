@@ -43,10 +43,14 @@ int cep_mesh_get_targets(const cep_mesh *m, int *targets);       /* 3, 0-based *
 /* Pacing-site file: *n node numbers (0-based), the first max_n copied. */
 int cep_read_pacing_file(const char *path, int *nodes, int max_n, int *n);
 
-/* Tissue on an S4 mesh (Cfem::set_amat_bmat with the isoparametric map of every element,
-   fem.cpp:628-661): the same calls as a grid tissue (cep_tissue.h), with the conductivity
-   given per element of the mesh and the node numbering of the mesh.  The diffusion matrices
-   are stored in CSR. */
+/* Tissue on a mesh of the element types the reference's FEM assembles (Cfem::set_gpt /
+   set_Phi_dPhiS, fem.cpp:830-860): S4 quadrilaterals ("square"), C3D8I hexahedra ("hex")
+   and C3D4 tetrahedra ("tet"), each through Cfem::set_amat_bmat with the isoparametric map
+   of every element (fem.cpp:628-661).  The same calls as a grid tissue (cep_tissue.h), with
+   the conductivity given per element of the mesh (3-D meshes: cep_tissue_set_conductivity_tensor)
+   and the node numbering of the mesh.  The diffusion matrices are stored in CSR.  Returns
+   null for another element type or for an element whose Jacobian determinant is not
+   positive at every Gauss point (inverted or folded; node order as in the reference). */
 cep_tissue *cep_tissue_create_mesh(const cep_mesh *m, int model, int precision);
 
 /* The reference's checkpoint file of every node's states, written and read back. */

@@ -45,6 +45,11 @@ int cep_tissue_set_cells(cep_tissue *t, const double *celltype, const double *co
 /* Diffusion tensor per element (nx*ny each, mm^2/ms): the reference's DE*I for the
    isotropic case (fem.cpp:787-828), or the fibre-rotated tensor. */
 int cep_tissue_set_conductivity(cep_tissue *t, const double *sxx, const double *sxy, const double *syy);
+/* The same for every tissue kind, with ncomp values per element: 1 = the reference's
+   isotropic DE (Cfem::make_diffusion_tensor, fem.cpp:162-170, D = DE*I), 3 = a 2-D tensor
+   (xx, xy, yy), 6 = a 3-D tensor of a hexahedral or tetrahedral mesh (xx, xy, xz, yy, yz,
+   zz).  Tensors must be positive semi-definite. */
+int cep_tissue_set_conductivity_tensor(cep_tissue *t, const double *d, int ncomp);
 /* S1 pacing: the listed nodes get the model's stimulus (amplitude scale, period bcl, start
    and duration in ms; the reference's isS1 nodes, heart.cpp:287-318). */
 int cep_tissue_set_stimulus(cep_tissue *t, const int *nodes, int count, double amp_scale, double bcl,

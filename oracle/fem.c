@@ -12,6 +12,15 @@
  * above): at each Gauss point the Jacobian of the bilinear map from the node coordinates,
  * its determinant and inverse give the physical gradients, as set_amat_bmat does with
  * MultMatMat / inverseMat (:631-640) for every mesh element.
+ * fem_hex8_element: the trilinear hexahedron (C3D8I, "hex") of set_gpt_hex (:231-248, points
+ * +-0.57735 in the order of its ix, iy, iz loops) and set_Phi_dPhiS_hex (:427-479, nodes
+ * (-,-,-), (+,-,-), (+,+,-), (-,+,-), then the same at z = +1); 3x3 Jacobian, det3D and
+ * inverseMat (:549-589).  D is the tensor (xx, xy, xz, yy, yz, zz); Me and Ke are 8x8.
+ * fem_tet4_element: the linear tetrahedron (C3D4, "tet") of set_gpt_tet (:209-229, points
+ * (5 -+ sqrt 5)/20) and set_Phi_dPhiS_tet (:342-374, phi = 1-s-t-z, s, t, z), 4x4.  The
+ * reference adds its four points with weight 1; the weight here is 1/24, the volume of the
+ * reference tetrahedron over four: both A and B of the reference are 24 times these, which
+ * leaves A x = B v unchanged.
  * cg_solve_csr: CG with Jacobi preconditioning from a zero initial guess, stopped on
  * the preconditioned residual norm relative to its initial value, as PETSc's KSPCG +
  * PCJACOBI with default tolerances (rtol 1e-5) used by Cfem::assemble_Amat_Bmat
@@ -75,6 +84,87 @@ void fem_q4_element(const double *xe, const double *ye, double sxx, double sxy, 
         Me[j * 4 + k] += phi[j] * phi[k] * det;
         Ke[j * 4 + k] += (dx[j] * (sxx * dx[k] + sxy * dy[k]) + dy[j] * (sxy * dx[k] + syy * dy[k])) * det;
       }
+  }
+}
+
+/* the physical gradients G (n x 3) of n shape functions with reference derivatives dS
+   (n x 3) through the 3x3 Jacobian J_rc = sum_a dS[a][r] X[a][c]; returns det J */
+static double grad3(int n, const double *X, const double (*dS)[3], double (*G)[3])
+{
+  double J[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      J[r][c] = 0.0;
+      for (int a = 0; a < n; ++a) J[r][c] += dS[a][r] * X[a * 3 + c];
+    }
+  const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                     J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                     J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+  double inv[3][3];
+  inv[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
+  inv[1][0] = -(J[1][0] * J[2][2] - J[1][2] * J[2][0]) / det;
+  inv[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
+  inv[0][1] = -(J[0][1] * J[2][2] - J[0][2] * J[2][1]) / det;
+  inv[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+  inv[2][1] = -(J[0][0] * J[2][1] - J[0][1] * J[2][0]) / det;
+  inv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+  inv[1][2] = -(J[0][0] * J[1][2] - J[0][2] * J[1][0]) / det;
+  inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+  /* dphi/dx_c = sum_r (J^-1)_{c r} dphi/ds_r */
+  for (int a = 0; a < n; ++a)
+    for (int c = 0; c < 3; ++c) G[a][c] = inv[c][0] * dS[a][0] + inv[c][1] * dS[a][1] + inv[c][2] * dS[a][2];
+  return det;
+}
+
+static void add_point(int n, const double *phi, const double (*G)[3], const double *D, double wdet, double *Me,
+                      double *Ke)
+{
+  const double Dm[3][3] = { { D[0], D[1], D[2] }, { D[1], D[3], D[4] }, { D[2], D[4], D[5] } };
+  for (int j = 0; j < n; ++j)
+    for (int k = 0; k < n; ++k) {
+      double kk = 0.0;
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) kk += G[j][r] * Dm[r][c] * G[k][c];
+      Me[j * n + k] += phi[j] * phi[k] * wdet;
+      Ke[j * n + k] += kk * wdet;
+    }
+}
+
+void fem_hex8_element(const double *X, const double *D, double *Me, double *Ke)
+{
+  const double g[2] = { -0.57735, 0.57735 };
+  const double ns[8] = { -1, 1, 1, -1, -1, 1, 1, -1 }, nt[8] = { -1, -1, 1, 1, -1, -1, 1, 1 };
+  const double nz[8] = { -1, -1, -1, -1, 1, 1, 1, 1 };
+  memset(Me, 0, 64 * sizeof(double));
+  memset(Ke, 0, 64 * sizeof(double));
+  for (int ix = 0; ix < 2; ++ix)
+    for (int iy = 0; iy < 2; ++iy)
+      for (int iz = 0; iz < 2; ++iz) {
+        const double s = g[ix], t = g[iy], z = g[iz];
+        double phi[8], dS[8][3], G[8][3];
+        for (int a = 0; a < 8; ++a) {
+          phi[a] = (1 + ns[a] * s) * (1 + nt[a] * t) * (1 + nz[a] * z) / 8.;
+          dS[a][0] = ns[a] * (1 + nt[a] * t) * (1 + nz[a] * z) / 8.;
+          dS[a][1] = nt[a] * (1 + ns[a] * s) * (1 + nz[a] * z) / 8.;
+          dS[a][2] = nz[a] * (1 + ns[a] * s) * (1 + nt[a] * t) / 8.;
+        }
+        const double det = grad3(8, X, (const double(*)[3])dS, G);
+        add_point(8, phi, (const double(*)[3])G, D, det, Me, Ke);
+      }
+}
+
+void fem_tet4_element(const double *X, const double *D, double *Me, double *Ke)
+{
+  const double aa = (5. - sqrt(5.)) / 20., bb = (5. + 3. * sqrt(5.)) / 20.;
+  const double P[4][3] = { { aa, aa, aa }, { aa, aa, bb }, { aa, bb, aa }, { bb, aa, aa } };
+  const double dS[4][3] = { { -1, -1, -1 }, { 1, 0, 0 }, { 0, 1, 0 }, { 0, 0, 1 } };
+  memset(Me, 0, 16 * sizeof(double));
+  memset(Ke, 0, 16 * sizeof(double));
+  for (int q = 0; q < 4; ++q) {
+    const double phi[4] = { 1. - P[q][0] - P[q][1] - P[q][2], P[q][0], P[q][1], P[q][2] };
+    double G[4][3];
+    const double det = grad3(4, X, dS, G);
+    add_point(4, phi, (const double(*)[3])G, D, det / 24.0, Me, Ke);
   }
 }
 

@@ -175,6 +175,8 @@ double oracle_tt_run_many(const tt_cell_cfg_t *cfgs, int ncell, int nthreads,
 double oracle_ord_rates_many(int n, const double *C, const double *S, double *R, double t);
 
 /* ---------------- tissue (fem.cpp / heart.cpp) */
+void fem_hex8_element(const double *X, const double *D, double *Me, double *Ke);
+void fem_tet4_element(const double *X, const double *D, double *Me, double *Ke);
 void fem_q4_element(const double *xe, const double *ye, double sxx, double sxy, double syy, double *Me,
                     double *Ke);
 void fem_q1_element(double hx, double hy, double sxx, double sxy, double syy,

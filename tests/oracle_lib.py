@@ -113,6 +113,8 @@ def lib():
         L.bm_free.argtypes = [C.c_void_p]
         L.fem_q1_element.argtypes = [C.c_double] * 5 + [dp, dp]
         L.fem_q4_element.argtypes = [dp, dp] + [C.c_double] * 3 + [dp, dp]
+        L.fem_hex8_element.argtypes = [dp, dp, dp, dp]
+        L.fem_tet4_element.argtypes = [dp, dp, dp, dp]
         L.cg_solve_csr.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), dp, dp, dp,
                                    C.c_double, C.c_int, C.c_int]
         _LIB = L

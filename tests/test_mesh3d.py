@@ -1,0 +1,209 @@
+"""3-D tissue meshes: the hexahedron (C3D8I, "hex") and tetrahedron (C3D4, "tet") elements
+of the reference's FEM (fem.cpp:209-248 Gauss points, :342-374 and :427-479 shape
+functions, :628-661 element matrices) and the mesh tissue's diffusion solve on them.
+
+CPU: the oracle's elements against what fixes them independently of their code:
+  * tetrahedron: the closed forms M = V (1 + delta_jk) / 20 and K = V grad(l_j) . D grad(l_k)
+    with the barycentric gradients from the inverse of the 4 x 4 vertex matrix;
+  * hexahedron on a box: the tensor product of 1-D two-point elements with the same points;
+    a hexahedron extruded from any quadrilateral: the quadrilateral element (itself pinned to
+    closed forms) times the 1-D element;
+  * both: rotation invariance with the tensor rotated alike, zero row sums of K, symmetry,
+    positive semi-definiteness, mass = volume for affine elements;
+  * mesh creation rejects inverted elements and element types the reference cannot assemble.
+GPU: the device solve on a distorted hexahedral box and on its tetrahedral split against the
+oracle's assembly and CG; a one-element-thick box of hexahedra steps like the 2-D grid.
+"""
+import importlib
+
+import numpy as np
+import pytest
+
+import tissue_oracle as T
+
+cep = importlib.import_module("2ddrugsimulation_amd")
+
+A = 0.57735                                           # the reference's Gauss point
+HEX_S = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1],
+                  [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]])
+
+
+def _d6(D):
+    return np.array([D[0, 0], D[0, 1], D[0, 2], D[1, 1], D[1, 2], D[2, 2]])
+
+
+def _line(h):
+    """1-D linear element with the two points +-A: mass and stiffness."""
+    phi = np.array([[(1 - s) / 2, (1 + s) / 2] for s in (-A, A)])
+    m = (h / 2) * phi.T @ phi
+    k = np.array([[1.0, -1.0], [-1.0, 1.0]]) / h
+    return m, k
+
+
+def _rotation(rng):
+    q, r = np.linalg.qr(rng.normal(size=(3, 3)))
+    q = q * np.sign(np.diag(r))
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    return q
+
+
+def _random_tet(rng):
+    while True:
+        X = rng.uniform(-1, 1, (4, 3)) * 0.1
+        v = np.linalg.det(np.stack([X[1] - X[0], X[2] - X[0], X[3] - X[0]])) / 6
+        if v > 2e-5:
+            return X, v
+
+
+def test_tet4_element_closed_forms():
+    rng = np.random.default_rng(31)
+    for _ in range(10):
+        X, V = _random_tet(rng)
+        D = T.random_tensors3(rng, 1)[0]
+        Dm = np.array([[D[0], D[1], D[2]], [D[1], D[3], D[4]], [D[2], D[4], D[5]]])
+        Me, Ke = T.tet4_element(X, D)
+        np.testing.assert_allclose(Me, V * (np.ones((4, 4)) + np.eye(4)) / 20, rtol=1e-12, atol=0)
+        C = np.vstack([np.ones(4), X.T])              # barycentric l = C^-1 (1, x, y, z)
+        G = np.linalg.inv(C)[:, 1:]                   # row j: grad l_j
+        np.testing.assert_allclose(Ke, V * G @ Dm @ G.T, rtol=1e-10, atol=1e-14)
+
+
+def test_hex8_element_on_a_box_is_the_tensor_product_of_line_elements():
+    hx, hy, hz = 0.1, 0.13, 0.07
+    dxx, dyy, dzz = 0.12, 0.05, 0.03
+    X = (HEX_S + 1) / 2 * np.array([hx, hy, hz]) + np.array([0.4, -0.2, 1.0])
+    Me, Ke = T.hex8_element(X, [dxx, 0, 0, dyy, 0, dzz])
+    (mx, kx), (my, ky), (mz, kz) = _line(hx), _line(hy), _line(hz)
+    i = (HEX_S + 1) // 2                              # 1-D node index per direction
+    Mt = mx[np.ix_(i[:, 0], i[:, 0])] * my[np.ix_(i[:, 1], i[:, 1])] * mz[np.ix_(i[:, 2], i[:, 2])]
+    Kt = (dxx * kx[np.ix_(i[:, 0], i[:, 0])] * my[np.ix_(i[:, 1], i[:, 1])] * mz[np.ix_(i[:, 2], i[:, 2])]
+          + dyy * mx[np.ix_(i[:, 0], i[:, 0])] * ky[np.ix_(i[:, 1], i[:, 1])] * mz[np.ix_(i[:, 2], i[:, 2])]
+          + dzz * mx[np.ix_(i[:, 0], i[:, 0])] * my[np.ix_(i[:, 1], i[:, 1])] * kz[np.ix_(i[:, 2], i[:, 2])])
+    np.testing.assert_allclose(Me, Mt, rtol=1e-12, atol=0)
+    np.testing.assert_allclose(Ke, Kt, rtol=1e-10, atol=1e-15)
+
+
+def test_hex8_element_extruded_from_a_quadrilateral():
+    """A hexahedron over any quadrilateral, straight in z, with a tensor that does not couple
+    z: M = Mq x mz and K = Kq x mz + Mq x kz dzz, the quadrilateral element being the one
+    pinned in the 2-D tests."""
+    rng = np.random.default_rng(32)
+    for _ in range(5):
+        x = np.array([0.0, 1.0, 1.0, 0.0]) * 0.1 + rng.uniform(-0.02, 0.02, 4)
+        y = np.array([0.0, 0.0, 1.0, 1.0]) * 0.1 + rng.uniform(-0.02, 0.02, 4)
+        hz, dzz = 0.08, 0.04
+        sxx, sxy, syy = (v[0] for v in T.random_tensors(rng, 1))
+        X = np.array([[x[j], y[j], 0.0] for j in range(4)] + [[x[j], y[j], hz] for j in range(4)])
+        Me, Ke = T.hex8_element(X, [sxx, sxy, 0, syy, 0, dzz])
+        Mq, Kq = T.q4_element(x, y, sxx, sxy, syy)
+        mz, kz = _line(hz)
+        iz = np.repeat([0, 1], 4)
+        iq = np.tile(np.arange(4), 2)
+        np.testing.assert_allclose(Me, Mq[np.ix_(iq, iq)] * mz[np.ix_(iz, iz)], rtol=1e-12, atol=1e-20)
+        np.testing.assert_allclose(Ke, Kq[np.ix_(iq, iq)] * mz[np.ix_(iz, iz)]
+                                   + dzz * Mq[np.ix_(iq, iq)] * kz[np.ix_(iz, iz)], rtol=1e-10, atol=1e-15)
+
+
+@pytest.mark.parametrize("kind", ["hex", "tet"])
+def test_3d_elements_invariants_and_rotation(kind):
+    rng = np.random.default_rng(33 if kind == "hex" else 34)
+    for _ in range(8):
+        if kind == "hex":
+            X = (HEX_S + 1) / 2 * 0.1 + rng.uniform(-0.015, 0.015, (8, 3))
+            el = T.hex8_element
+        else:
+            X, _ = _random_tet(rng)
+            el = T.tet4_element
+        D = T.random_tensors3(rng, 1)[0]
+        Dm = np.array([[D[0], D[1], D[2]], [D[1], D[3], D[4]], [D[2], D[4], D[5]]])
+        Me, Ke = el(X, D)
+        np.testing.assert_allclose(Ke.sum(axis=1), 0.0, atol=1e-14)     # constants do not diffuse
+        np.testing.assert_allclose(Me, Me.T, atol=0)
+        np.testing.assert_allclose(Ke, Ke.T, atol=1e-15)
+        assert np.linalg.eigvalsh(Ke).min() >= -1e-14
+        assert np.linalg.eigvalsh(Me).min() > 0
+        R = _rotation(rng)
+        Mr, Kr = el(X @ R.T, _d6(R @ Dm @ R.T))
+        np.testing.assert_allclose(Mr, Me, rtol=1e-10, atol=1e-18)
+        np.testing.assert_allclose(Kr, Ke, rtol=1e-9, atol=1e-14)
+    # an affine hexahedron (parallelepiped): det J is constant, the mass is its volume
+    E = np.array([[0.1, 0.01, 0.0], [0.02, 0.12, 0.01], [0.0, 0.03, 0.09]])
+    Xp = ((HEX_S + 1) / 2) @ E
+    Mp, _ = T.hex8_element(Xp, [1, 0, 0, 1, 0, 1])
+    assert abs(Mp.sum() - abs(np.linalg.det(E))) <= 1e-15
+
+
+def test_mesh_creation_rejects_what_the_reference_cannot_assemble():
+    xyz, conn = T.box_hex_mesh(2, 2, 1, 0.1, 0.1, 0.1)
+    bad = conn.copy()
+    bad[1] = bad[1][[4, 5, 6, 7, 0, 1, 2, 3]]          # top and bottom faces swapped: inverted
+    with pytest.raises(cep.CepError):
+        cep.Tissue.from_mesh(cep.Mesh(xyz, bad, dimension=3))
+    tets = T.tet_mesh_from_hex(xyz, conn)
+    tets[0] = tets[0][[0, 1, 3, 2]]
+    with pytest.raises(cep.CepError):
+        cep.Tissue.from_mesh(cep.Mesh(xyz, tets, dimension=3))
+    with pytest.raises(cep.CepError):                  # S3 triangles ("tri"): no FEM in the reference
+        cep.Tissue.from_mesh(cep.Mesh(xyz, conn[:, :3], dimension=3))
+
+
+# ---------------------------------------------------------------- GPU
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["hex", "tet"])
+def test_3d_mesh_diffusion_matches_oracle(kind):
+    rng = np.random.default_rng(41 if kind == "hex" else 42)
+    xyz, conn = T.distorted_box(5, 4, 3, 0.1, rng)
+    if kind == "tet":
+        conn = T.tet_mesh_from_hex(xyz, conn)
+    D = T.random_tensors3(rng, len(conn))
+    v = rng.normal(size=len(xyz)) * 20 - 60
+    dt = 0.01
+    with cep.Tissue.from_mesh(cep.Mesh(xyz, conn, dimension=3)) as tis:
+        tis.set_cells()
+        tis.set_conductivity_tensor(D)
+        x, it = tis.diffuse(dt, v, cg_rtol=1e-12)
+        x5, it5 = tis.diffuse(dt, v, cg_rtol=1e-5)
+        with pytest.raises(cep.CepError):              # a 3-D mesh takes the 6-component tensor
+            tis.set_conductivity(0.1, 0.0, 0.1)
+        with pytest.raises(cep.CepError):
+            tis.set_conductivity_tensor(np.tile([0.1, 0.2, 0.0, 0.1, 0.0, 0.1], (len(conn), 1)))  # indefinite
+        tis.set_conductivity_tensor(0.1)               # isotropic DE
+    A, Bm = T.assemble_mesh3(xyz, conn, D, dt)
+    xo, ito = T.cg(A, Bm @ v, 1e-12)
+    xo5, ito5 = T.cg(A, Bm @ v, 1e-5)
+    assert np.abs(x - xo).max() <= 1e-9 * np.abs(xo).max()
+    assert abs(it - ito) <= 1 and abs(it5 - ito5) <= 1
+    assert np.abs(x5 - xo5).max() <= 1e-6 * np.abs(xo).max()
+    assert abs((Bm @ x).sum() - (Bm @ v).sum()) <= 1e-10 * np.abs(Bm @ v).sum()
+
+
+@pytest.mark.gpu
+def test_one_layer_of_hexahedra_steps_like_the_grid():
+    """A box one hexahedron thick over a 16 x 12 grid, the tensor not coupling z: a field
+    constant in z stays so, and the exact solve reduces to the grid's (the z factor of M
+    times (1, 1) is hz/2 on both faces, that of K is zero).  A paced corner column after 200
+    steps: both faces agree with the 2-D grid far below the solver tolerance."""
+    nx, ny, h, hz, dt = 16, 12, 0.1, 0.05, 0.01
+    rng = np.random.default_rng(43)
+    sxx, sxy, syy = T.random_tensors(rng, nx * ny)
+    corner = [iy * (nx + 1) + ix for iy in range(3) for ix in range(3)]
+    with cep.Tissue(nx, ny, h, h) as tis:
+        tis.set_cells()
+        tis.set_conductivity(sxx, sxy, syy)
+        tis.set_stimulus(corner, amp_scale=2.0, bcl=1000.0, start=0.0, duration=1.0)
+        tis.step(200, dt, cg_rtol=1e-12)
+        v2 = tis.vm()
+    xyz, conn = T.box_hex_mesh(nx, ny, 1, h, h, hz)
+    nl = (nx + 1) * (ny + 1)
+    D = np.stack([sxx, sxy, np.zeros_like(sxx), syy, np.zeros_like(sxx), np.full_like(sxx, 0.07)], axis=1)
+    with cep.Tissue.from_mesh(cep.Mesh(xyz, conn, dimension=3)) as tis:
+        tis.set_cells()
+        tis.set_conductivity_tensor(D)
+        tis.set_stimulus(corner + [nl + c for c in corner], amp_scale=2.0, bcl=1000.0, start=0.0, duration=1.0)
+        tis.step(200, dt, cg_rtol=1e-12)
+        v3 = tis.vm()
+    assert np.abs(v3[:nl] - v2).max() <= 1e-6
+    assert np.abs(v3[nl:] - v2).max() <= 1e-6
+    assert v2[corner].max() > 0.0

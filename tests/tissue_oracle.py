@@ -164,3 +164,96 @@ def write_inp(path, xyz, conn, targets=(1, 1, 1), celltype=None, etype="S4"):
             for c in celltype:
                 f.write(f"{int(c)}\n")
         f.write("eof\n")
+
+
+# ---------------------------------------------------------------- 3-D meshes
+
+def hex8_element(X, D):
+    """The oracle's trilinear hexahedron: X 8 x 3 (the reference's node order), D = (xx, xy,
+    xz, yy, yz, zz)."""
+    Me = np.zeros(64)
+    Ke = np.zeros(64)
+    x = np.ascontiguousarray(X, dtype=np.float64)
+    d = np.ascontiguousarray(D, dtype=np.float64)
+    O.lib().fem_hex8_element(O.ptr(x), O.ptr(d), O.ptr(Me), O.ptr(Ke))
+    return Me.reshape(8, 8), Ke.reshape(8, 8)
+
+
+def tet4_element(X, D):
+    Me = np.zeros(16)
+    Ke = np.zeros(16)
+    x = np.ascontiguousarray(X, dtype=np.float64)
+    d = np.ascontiguousarray(D, dtype=np.float64)
+    O.lib().fem_tet4_element(O.ptr(x), O.ptr(d), O.ptr(Me), O.ptr(Ke))
+    return Me.reshape(4, 4), Ke.reshape(4, 4)
+
+
+def assemble_mesh3(xyz, conn, D, dt):
+    """Dense A = M/dt + K and B = M/dt of a hexahedral (8 nodes) or tetrahedral (4) mesh."""
+    nn = len(xyz)
+    A = np.zeros((nn, nn))
+    Bm = np.zeros((nn, nn))
+    for e, nd in enumerate(conn):
+        Me, Ke = (hex8_element if len(nd) == 8 else tet4_element)(xyz[nd], D[e])
+        n = len(nd)
+        for j in range(n):
+            for k in range(n):
+                A[nd[j], nd[k]] += Me[j, k] / dt + Ke[j, k]
+                Bm[nd[j], nd[k]] += Me[j, k] / dt
+    return A, Bm
+
+
+def box_hex_mesh(nx, ny, nz, hx, hy, hz):
+    """A box of nx x ny x nz hexahedra; node (ix, iy, iz) at index (iz*(ny+1) + iy)*(nx+1) + ix,
+    elements in the reference's node order."""
+    xs, ys, zs = np.meshgrid(np.arange(nx + 1) * hx, np.arange(ny + 1) * hy, np.arange(nz + 1) * hz,
+                             indexing="ij")
+    xyz = np.stack([xs.transpose(2, 1, 0).ravel(), ys.transpose(2, 1, 0).ravel(), zs.transpose(2, 1, 0).ravel()],
+                   axis=1)
+
+    def node(ix, iy, iz):
+        return (iz * (ny + 1) + iy) * (nx + 1) + ix
+    conn = []
+    for ez in range(nz):
+        for ey in range(ny):
+            for ex in range(nx):
+                bot = [node(ex + OX[j], ey + OY[j], ez) for j in range(4)]
+                top = [node(ex + OX[j], ey + OY[j], ez + 1) for j in range(4)]
+                conn.append(bot + top)
+    return xyz, np.array(conn, dtype=np.int32)
+
+
+def distorted_box(nx, ny, nz, h, rng, amp=0.2):
+    """A box of hexahedra whose interior nodes are moved by up to amp*h in each direction."""
+    xyz, conn = box_hex_mesh(nx, ny, nz, h, h, h)
+    idx = np.arange(len(xyz))
+    ix, iy, iz = idx % (nx + 1), (idx // (nx + 1)) % (ny + 1), idx // ((nx + 1) * (ny + 1))
+    inner = (ix > 0) & (ix < nx) & (iy > 0) & (iy < ny) & (iz > 0) & (iz < nz)
+    xyz[inner] += rng.uniform(-amp, amp, (inner.sum(), 3)) * h
+    return xyz, conn
+
+
+# the six tetrahedra of a cube along its (0,0,0)-(1,1,1) diagonal (local hexahedron node
+# numbers): neighbouring cubes share their face diagonals, so the mesh is conforming
+_KUHN = [(0, 1, 2, 6), (0, 2, 3, 6), (0, 3, 7, 6), (0, 7, 4, 6), (0, 4, 5, 6), (0, 5, 1, 6)]
+
+
+def tet_mesh_from_hex(xyz, conn):
+    """Split every hexahedron into six tetrahedra, each with a positive orientation."""
+    tets = []
+    for nd in conn:
+        for q in _KUHN:
+            t = [nd[k] for k in q]
+            p = xyz[t]
+            if np.linalg.det(np.stack([p[1] - p[0], p[2] - p[0], p[3] - p[0]])) < 0:
+                t[2], t[3] = t[3], t[2]
+            tets.append(t)
+    return np.array(tets, dtype=np.int32)
+
+
+def random_tensors3(rng, ne, dl=0.12, dt_=0.03):
+    """Fibre tensors D = dt I + (dl - dt) f f^T with random unit fibre directions f."""
+    f = rng.normal(size=(ne, 3))
+    f /= np.linalg.norm(f, axis=1, keepdims=True)
+    D = dt_ * np.eye(3)[None] + (dl - dt_) * f[:, :, None] * f[:, None, :]
+    return np.stack([D[:, 0, 0], D[:, 0, 1], D[:, 0, 2], D[:, 1, 1], D[:, 1, 2], D[:, 2, 2]], axis=1)
