@@ -55,6 +55,7 @@ EXPORTED = (
     "cep_mesh_get_elements", "cep_mesh_get_celltype", "cep_mesh_get_targets", "cep_read_pacing_file",
     "cep_tissue_create_mesh", "cep_tissue_write_checkpoint", "cep_tissue_read_checkpoint",
     "cep_tissue_set_s1s2", "cep_tissue_s2_time", "cep_tissue_set_conductivity_tensor",
+    "cep_release_cached_memory",
     "cep_read_ic50_csv", "cep_read_states_file", "cep_write_states_file", "cep_param_defaults",
     "cep_read_input_deck", "cep_param_concentrations", "cep_write_cipa_features", "cep_write_ord11_qni",
     "cep_write_time_series",
@@ -84,6 +85,7 @@ def library(path: str | None = None):
     L.cep_batch_create.restype = C.c_void_p
     L.cep_batch_destroy.argtypes = [C.c_void_p]
     L.cep_batch_destroy.restype = None
+    L.cep_release_cached_memory.argtypes = []
     L.cep_last_error.argtypes = [C.c_void_p]
     L.cep_last_error.restype = C.c_char_p
     L.cep_batch_info.argtypes = [C.c_void_p] + [C.POINTER(C.c_int)] * 5
@@ -533,6 +535,13 @@ class Tissue:
         n = (C.c_longlong * 3)()
         self._chk(self.L.cep_tissue_kernel_times(self.h, _p(avg), n))
         return avg, np.array(list(n))
+
+
+def release_cached_memory() -> None:
+    """Return the device memory kept from destroyed batches (it is otherwise reused by later
+    batches); waits for the device."""
+    if library().cep_release_cached_memory() != 0:
+        raise CepError("cep_release_cached_memory failed")
 
 
 def device_bytes(model: int, ncells: int, max_bcl: float, dt: float, vm_trace: bool = False) -> int:

@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1104,8 +1105,60 @@ struct cep_batch {
   double *d_fac = nullptr, *d_sd = nullptr, *d_tcur = nullptr, *d_vtrace = nullptr;
   int *d_si = nullptr, *d_bmi = nullptr, *d_status = nullptr, *d_iout = nullptr;
   long long *d_stats = nullptr;
-  std::vector<void *> allocs;
+  std::vector<std::pair<void *, size_t>> allocs;   // device blocks and their sizes
 };
+
+// Device memory of batches is recycled, not freed: hipFree waits for the work of every stream
+// on the device, so in a serving loop the end of one job would hold the host until every
+// other job's running beat kernel had finished.  A destroyed batch's blocks (its stream
+// drained first) go to a free list per device and are handed to later batches; the smallest
+// block of at least the request and at most twice it is taken.  cep_release_cached_memory()
+// returns the list to the device.
+namespace blockcache {
+struct Block {
+  void *p;
+  size_t bytes;
+  int dev;
+};
+static std::mutex mu;
+static std::vector<Block> cached;
+
+static size_t round_bytes(size_t n) { return (n + 65535) & ~(size_t)65535; }
+
+static hipError_t get(void **p, size_t bytes, size_t *got)
+{
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  bytes = round_bytes(bytes);
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    size_t best = cached.size();
+    for (size_t i = 0; i < cached.size(); ++i) {
+      const Block &c = cached[i];
+      if (c.dev == dev && c.bytes >= bytes && c.bytes <= 2 * bytes &&
+          (best == cached.size() || c.bytes < cached[best].bytes))
+        best = i;
+    }
+    if (best < cached.size()) {
+      *p = cached[best].p;
+      *got = cached[best].bytes;
+      cached.erase(cached.begin() + best);
+      return hipSuccess;
+    }
+  }
+  *got = bytes;
+  return hipMalloc(p, bytes);
+}
+
+static void put(void *p, size_t bytes)
+{
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lk(mu);
+  cached.push_back(Block{ p, bytes, dev });
+}
+}  // namespace blockcache
 
 static int fail(cep_batch *b, int code, const std::string &msg)
 {
@@ -1123,13 +1176,14 @@ template <typename T>
 static int dalloc(cep_batch *b, T **p, size_t count)
 {
   void *q = nullptr;
-  hipError_t e = hipMalloc(&q, count * sizeof(T) + 64);
+  size_t got = 0;
+  hipError_t e = blockcache::get(&q, count * sizeof(T) + 64, &got);
   if (e != hipSuccess) return fail(b, CEP_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+  b->allocs.push_back({ q, got });
   // on the batch stream: a memset on the null stream is not ordered with the batch's
   // (non-blocking) stream and could land after the first uploads into the buffer
   e = hipMemsetAsync(q, 0, count * sizeof(T) + 64, b->stream);
   if (e != hipSuccess) return fail(b, CEP_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
-  b->allocs.push_back(q);
   *p = (T *)q;
   return 0;
 }
@@ -1227,11 +1281,21 @@ static cep_batch *batch_create(int model, int ncells, int precision, bool integr
 
 extern "C" {
 
+int cep_release_cached_memory(void)
+{
+  std::lock_guard<std::mutex> lk(blockcache::mu);
+  int rc = 0;
+  for (const auto &c : blockcache::cached) rc |= hipFree(c.p) != hipSuccess;
+  blockcache::cached.clear();
+  return rc ? CEP_E_HIP : 0;
+}
+
 void cep_batch_destroy(cep_batch *b)
 {
   if (!b) return;
+  // the stream drained, no kernel of this batch touches its blocks any more
   if (b->stream) (void)hipStreamSynchronize(b->stream);
-  for (void *p : b->allocs) (void)hipFree(p);
+  for (auto &a : b->allocs) blockcache::put(a.first, a.second);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
   if (b->stream) (void)hipStreamDestroy(b->stream);

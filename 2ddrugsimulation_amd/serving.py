@@ -16,6 +16,13 @@ their drug / IC50 / conductance parameters and a beat count; the server
 
 A job's result does not depend on what else runs: its cells live in their own batch, and
 inside a batch every cell is integrated independently of its neighbours.
+
+Launches of different jobs overlap only on different hardware queues: HIP maps streams onto
+``GPU_MAX_HW_QUEUES`` queues (4 unless set, some taken by the runtime), and a beat kernel holds
+its queue for up to a beat of device time.  A serving process sets ``GPU_MAX_HW_QUEUES`` well
+above ``max_active`` (e.g. 24 for 16) in its environment before the first HIP call.  Device
+memory of finished jobs is recycled by the library (``release_cached_memory`` returns it):
+freeing it would wait for every job's running kernel.
 """
 from __future__ import annotations
 

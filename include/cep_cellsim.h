@@ -92,6 +92,11 @@ typedef struct cep_batch cep_batch;
    (e.g. ohara_rudy_cipa_v1_2017.cpp:525-534). */
 cep_batch *cep_batch_create(int model, int ncells, int precision);
 void cep_batch_destroy(cep_batch *b);
+/* Device memory of destroyed batches is kept by the library and reused by later batches
+   (hipFree would wait for every stream on the device, stalling concurrent jobs).  This returns
+   it to the device; it waits for the device like hipFree.  No counterpart in the reference
+   (its cells live in host memory). */
+int cep_release_cached_memory(void);
 const char *cep_last_error(const cep_batch *b);
 /* model, ncells, states per cell, constants per cell (CellML layout), algebraic slots */
 int cep_batch_info(const cep_batch *b, int *model, int *ncells, int *nstates, int *nconsts, int *nalg);
