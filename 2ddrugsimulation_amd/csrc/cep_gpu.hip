@@ -570,8 +570,12 @@ struct TtBuffers {
   int *bmi, *iout, *status;
 };
 
+// FP32 rates: at most 256 registers, two waves per SIMD (measured on 200,000 cells: 0.60 s per
+// beat against 0.78 s with 320 registers and one wave, whose 3125 waves took four rounds of
+// the 1024 SIMDs).  FP64 keeps its registers: capped, it would spill 512 bytes per lane.
 template <typename R>
-__global__ void __launch_bounds__(256) tt_rl_kernel(TtBuffers B, RunCfg rc, int nbeats)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 2 : 1)))
+tt_rl_kernel(TtBuffers B, RunCfg rc, int nbeats)
 {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= B.ncell) return;

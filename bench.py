@@ -15,6 +15,9 @@ beats), whose wall time the reference published, and a batched population of tha
 `--workload serving` sweeps the Poisson arrival rate of the serving session (default 1, 2, 4,
 8, 16, 32 and 64 jobs/s, the same seeded jobs at every rate) and reports p50/p99 latency and served
 cell-beats/s per rate; the value is the highest served rate in cell-beats/s.
+`--workload tt06` times the ten Tusscher 2006 population: 200,000 cells (endo/epi/M, Hill
+block of IKr/ICaL/INa, ten lognormal(0, 0.25) conductance scales), BCL 1000 ms, Rush-Larsen
+dt 0.02 ms, on-device APD90/CaTD; a step is one beat of every cell.
 `--workload tissue` instead times the 2-D monodomain step (512 x 512 Q1 elements, ORd-CiPA
 nodes, Strang splitting, Jacobi-PCG diffusion) in node-steps/s; a "step" is one time step of
 the whole tissue.
@@ -202,6 +205,97 @@ def tissue_cpu_baseline(tis, tensors, drug, nx: int):
             "sample": f"one oracle Jacobi-PCG solve of the {nn}-node system ({it} iterations, "
                       f"{t_cg:.2f} s) + oracle RHS on {ns} node states x 2 half-steps, scaled to "
                       f"{nn} nodes; assembly {t1 - t0:.1f} s not counted"}
+
+
+# TT06 Rush-Larsen: counted by oracle/opcount.cpp, flops per step = RHS 460 + gate/state update
+# 74 (63 exp, 4 log and 27 pow per step are not counted)
+F_TT_STEP = 460 + 74
+
+
+def tt06_main(args):
+    """ten Tusscher 2006 population, one lane per cell, fixed-dt Rush-Larsen."""
+    prec = cep.PREC_MIXED if args.precision == "mixed" else cep.PREC_FP64
+    K = args.steps if args.steps is not None else 3
+    W = args.warmup if args.warmup is not None else 1
+    n = args.tt_cells
+    dt = 0.02
+    pop = cep.workload.tt_population(20251205, n)
+    herg = np.concatenate([pop["ic50"], pop["hill"]], axis=1)
+
+    def batch(sel, pr):
+        b = cep.CellBatch(cep.MODEL_TT06, len(sel), pr)
+        b.set_cells(celltype=pop["celltype"][sel], conc=pop["conc"][sel], herg=herg[sel],
+                    gscale=pop["gscale"][sel], bcl=pop["bcl"][sel])
+        b.set_run_config(dt=dt, nbeats_total=500, nwin=500)
+        b.init_consts()
+        return b
+
+    allc = np.arange(n)
+    b = batch(allc, prec)
+    log(f"bench: TT06 {n} cells, warmup {W} beat(s)")
+    if W:
+        b.run_beats(W)
+    b.kernel_time()
+    t0 = time.perf_counter()
+    for _ in range(K):
+        b.run_beats(1)
+    t1 = time.perf_counter()
+    nl, kavg_ms, _ = b.kernel_time()
+    bm = b.biomarker_array()
+    bad = int(np.sum(b.status() != 0))
+    b.close()
+    elapsed = t1 - t0
+    value = n * K / elapsed
+    steps_per_beat = int(round(1000.0 / dt))
+    flops_launch = n * steps_per_beat * F_TT_STEP
+    achieved = flops_launch / (kavg_ms * 1e-3) if kavg_ms > 0 else 0.0
+    peak = PEAK_FP32 if prec == cep.PREC_MIXED else PEAK_FP64
+    chk = {}
+    if args.check_cells > 0 and prec == cep.PREC_MIXED:
+        sel = np.linspace(0, n - 1, args.check_cells).astype(int)
+        b64 = batch(sel, cep.PREC_FP64)
+        b64.run_beats(W + K)
+        bm64 = b64.biomarker_array()
+        b64.close()
+        bi = {k: i for i, k in enumerate(cep.BM_FIELDS)}
+        chk = {"apd90_max_abs_err_ms": float(np.abs(bm[sel, bi["apd90"]] - bm64[:, bi["apd90"]]).max()),
+               "cad90_max_abs_err_ms": float(np.abs(bm[sel, bi["cad90"]] - bm64[:, bi["cad90"]]).max()),
+               "cells_checked": int(len(sel))}
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        threads = min(16, os.cpu_count() or 1)
+        ncpu = max(threads, int(args.cpu_seconds * threads / 0.12))
+        sel = np.linspace(0, n - 1, ncpu).astype(int)
+        cfgs = (O.TtCellCfg * ncpu)()
+        for k, i in enumerate(sel):
+            cfgs[k] = O.make_tt_cfg(celltype=pop["celltype"][i], conc=pop["conc"][i], ic50=pop["ic50"][i],
+                                    hill=pop["hill"][i], gscale=pop["gscale"][i], bcl=pop["bcl"][i], nbeats=1,
+                                    nwin=1, dt=dt)
+        res = (O.BmResult * ncpu)()
+        log(f"bench: CPU baseline, {ncpu} cells x 1 beat")
+        wall = O.lib().oracle_tt_run_many(cfgs, ncpu, threads, res, None)
+        cpu = {"value": ncpu / wall, "unit": "cell-beats/s", "cores": threads, "kind": "port",
+               "sample": f"{ncpu} cells x 1 beat of the same population, FP64 Rush-Larsen oracle, "
+                         f"{threads} threads, {wall:.1f} s"}
+    print(json.dumps({
+        "metric": "TT06 cell-beats/s on 1 GPU (Rush-Larsen dt 0.02 ms)",
+        "value": value, "unit": "cell-beats/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": 1e3 * elapsed / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32-rates/f64-state" if prec == cep.PREC_MIXED else "f64",
+        "data": "synthetic (seeded drug/IC50/conductance population)",
+        "config": {"workload": f"ten Tusscher 2006 population, {n} cells endo/epi/M, Hill block of "
+                               "IKr/ICaL/INa at 1-4 x Cmax, 10 lognormal(0, 0.25) conductance scales, BCL "
+                               "1000 ms, Rush-Larsen dt 0.02 ms, step = 1 beat", "cells": n},
+        "failed_cells": bad,
+        "accuracy_vs_fp64": chk,
+        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None, "kernel": f"tt_rl_kernel<{'float' if prec == cep.PREC_MIXED else 'double'}>",
+                     "kernel_avg_ms": kavg_ms, "flops_per_launch": flops_launch,
+                     "flops_note": "534 counted flops per cell-step; 63 exp, 4 log, 27 pow per step not counted"},
+        "cpu_baseline": cpu,
+    }), flush=True)
 
 
 def serving_main(args):
@@ -396,7 +490,9 @@ def ord11_cpu_baseline(q, s0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("population", "tissue", "ord2011", "serving"), default="population")
+    ap.add_argument("--workload", choices=("population", "tissue", "ord2011", "serving", "tt06"),
+                    default="population")
+    ap.add_argument("--tt-cells", type=int, default=200000)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--tissue-nx", type=int, default=TISSUE_NX)
@@ -419,6 +515,10 @@ def main():
         return
     if args.workload == "serving":
         serving_main(args)
+        return
+    if args.workload == "tt06":
+        args.precision = args.precision or "mixed"
+        tt06_main(args)
         return
     if args.workload == "tissue":
         args.precision = args.precision or "fp64"

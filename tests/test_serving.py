@@ -143,3 +143,27 @@ def test_burst_of_jobs_all_start_from_their_initial_states():
     assert len(srv.finished) == 16
     for j in srv.finished:
         assert (j.status == 0).all(), (j.jid, j.model, j.status)
+
+
+@pytest.mark.gpu
+def test_recycled_device_memory_gives_the_same_results():
+    """Batches take the blocks of destroyed batches (the library's free list): a batch built
+    on recycled blocks, whose previous owner left other states in them, gives bitwise the
+    results of one built on fresh blocks after cep_release_cached_memory."""
+    jobs = serving.synthetic_jobs(12, 2, max_cells=40, beats=(2, 2), tt06_fraction=0.0)
+    a, t = jobs[0], jobs[1]
+
+    def run(job):
+        srv = serving.Server(mem_budget=8 << 30, max_active=2)
+        srv.submit(serving.Job(job.jid, job.model, job.cells, job.nbeats, precision=job.precision))
+        srv.drain()
+        return srv.finished[0]
+
+    cep.release_cached_memory()
+    fresh = run(t)
+    run(a)                                  # leaves its blocks, with its states, in the list
+    recycled = run(t)
+    assert (fresh.status == 0).all() and (recycled.status == 0).all()
+    np.testing.assert_array_equal(fresh.biomarkers, recycled.biomarkers)
+    np.testing.assert_array_equal(fresh.states, recycled.states)
+    cep.release_cached_memory()
