@@ -258,7 +258,9 @@ def tt06_main(args):
         bm64 = b64.biomarker_array()
         b64.close()
         bi = {k: i for i, k in enumerate(cep.BM_FIELDS)}
-        chk = {"apd90_max_abs_err_ms": float(np.abs(bm[sel, bi["apd90"]] - bm64[:, bi["apd90"]]).max()),
+        ea = np.abs(bm[sel, bi["apd90"]] - bm64[:, bi["apd90"]])
+        chk = {"apd90_max_abs_err_ms": float(ea.max()), "apd90_median_abs_err_ms": float(np.median(ea)),
+               "apd90_frac_within_1ms": float(np.mean(ea <= 1.0)),
                "cad90_max_abs_err_ms": float(np.abs(bm[sel, bi["cad90"]] - bm64[:, bi["cad90"]]).max()),
                "cells_checked": int(len(sel))}
     cpu = None
