@@ -24,6 +24,7 @@ OPT_DUTTA = 1
 OPT_TIME_POINT = 2
 OPT_INTEGRATOR = 3
 OPT_TIME_SERIES = 4
+OPT_ORD11_WAVES = 5
 INTEGRATOR_RUSH_LARSEN = 0
 INTEGRATOR_ADAMS = 1
 PREC_FP64 = 0

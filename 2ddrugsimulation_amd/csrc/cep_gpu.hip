@@ -754,6 +754,173 @@ __global__ void __launch_bounds__(64) ord11_rl_kernel(O11Buffers B, int beats_en
   B.stats[c] += n;              // steps (one right-hand side each)
 }
 
+// The same loop with each cell's step split over the two wavefronts of a 128-thread workgroup
+// (lane l of both holds cell blockIdx.x * 64 + l), on two SIMDs: wave 0 evaluates the gate
+// kinetics (gate_rates, about half of the transcendentals) and their Rush-Larsen update;
+// wave 1 the currents, the step size, the Jrel / Euler update and the features.  Per step
+// they exchange the step size, then the new gates and V / cass, through LDS between two
+// barriers.  Per cell the arithmetic is that of ord11_rl_kernel.  Both waves see the same t
+// and step count for every lane, so they run the same number of passes of the loop.
+template <typename R>
+__global__ void __launch_bounds__(128) ord11_rl2_kernel(O11Buffers B, int beats_end, long long max_steps)
+{
+  constexpr int NGT = 38 - ord11::G0 + 1;       // gates 10..38 (incl. nca)
+  __shared__ double s_gate[NGT][64];
+  __shared__ double s_vc[2][64];
+  __shared__ double s_dt[64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c = blockIdx.x * 64 + lane;
+  const bool valid = c < B.ncell && B.status[c] == ST_OK;
+  const size_t P_ = (size_t)B.pitch;
+  const PrmAcc P{B.prm + (valid ? c : 0), B.pitch};
+  double y[ord11::NS];
+  double st[O11_SD_N];
+  double *sd = B.bmd + c;
+  int *si = B.bmi + c;
+  int pace = 0, elig = 0, steep = 0;
+  double best_dvdt = -999.0, t = 0.0, bcl = 1.0;
+  if (valid) {
+#pragma unroll
+    for (int i = 0; i < ord11::NS; ++i) y[i] = B.y[i * P_ + c];
+#pragma unroll
+    for (int k = 0; k < O11_SD_N; ++k) st[k] = sd[k * P_];
+    bcl = P(ord11::P_BCL);
+    t = st[O11_T];
+    if (wave == 1) {
+      pace = si[O11_I_PACE * P_]; elig = si[O11_I_ELIG * P_]; steep = si[O11_I_STEEP * P_];
+      best_dvdt = sd[(O11_BEST + CEP_BM11_DVMDT_REPOL) * P_];
+      if (si[O11_I_INIT * P_] == 0) {
+        st[O11_TMP_DVDT] = -999.0; st[O11_TMP_PEAK] = -999.0; st[O11_TMP_VALLEY] = y[0];
+        for (int k = 0; k < CEP_NBM; ++k) sd[(O11_BEST + k) * P_] = 0.0;
+        sd[(O11_BEST + CEP_BM11_DVMDT_REPOL) * P_] = -999.0;
+        sd[(O11_BEST + CEP_BM11_VM_PEAK) * P_] = -999.0;
+        sd[(O11_BEST + CEP_BM11_VM_VALLEY) * P_] = y[0];
+        best_dvdt = -999.0;
+        si[O11_I_INIT * P_] = 1;
+      }
+    }
+  }
+  const double tmax = beats_end * bcl;
+  const int first_win = B.nbeats_total - B.nwin;
+  long long n = 0;
+  bool active = valid && t < tmax && n < max_steps;
+  while (__ballot(active) != 0ull) {
+    ord11::Gates g;
+    double f[ord11::NS];
+    ord11::Currents cur;
+    double dt = 0.0;
+    if (wave == 0) {
+      if (active) ord11::gate_rates<R>(y, P, g);
+    } else if (active) {
+      ord11::current_rates<R>(t, y, f, P, cur, g);
+      const double dts = ord11::time_step(t, B.time_point, B.max_dt, bcl, f[0]);
+      if (floor((t + dts) / bcl) == floor(t / bcl)) {
+        dt = dts;
+      } else {
+        dt = (floor(t / bcl) + 1) * bcl - t;
+        if (elig && pace >= first_win) {
+          if (st[O11_TMP_DVDT] > best_dvdt) {
+            best_dvdt = st[O11_TMP_DVDT];
+            steep = pace;
+            double *bs = sd + O11_BEST * P_;
+            bs[CEP_BM11_QNET_AP * P_] = st[O11_QAP];
+            bs[CEP_BM11_QNET4_AP * P_] = st[O11_Q4AP];
+            bs[CEP_BM11_INAL_AUC_AP * P_] = st[O11_INALAP];
+            bs[CEP_BM11_ICAL_AUC_AP * P_] = st[O11_ICALAP];
+            bs[CEP_BM11_QNET_CL * P_] = st[O11_QCL];
+            bs[CEP_BM11_QNET4_CL * P_] = st[O11_Q4CL];
+            bs[CEP_BM11_INAL_AUC_CL * P_] = st[O11_INALCL];
+            bs[CEP_BM11_ICAL_AUC_CL * P_] = st[O11_ICALCL];
+            bs[CEP_BM11_VM_PEAK * P_] = st[O11_TMP_PEAK];
+            bs[CEP_BM11_VM_VALLEY * P_] = st[O11_TMP_VALLEY];
+            bs[CEP_BM11_DVMDT_REPOL * P_] = st[O11_TMP_DVDT];
+            bs[CEP_BM11_T_PEAK * P_] = st[O11_TPEAK];
+            bs[CEP_BM11_VM_REPOL30 * P_] = st[O11_R30];
+            bs[CEP_BM11_VM_REPOL50 * P_] = st[O11_R50];
+            bs[CEP_BM11_VM_REPOL90 * P_] = st[O11_R90];
+          }
+        }
+#pragma unroll
+        for (int k = O11_QAP; k <= O11_ICALCL; ++k) st[k] = 0.0;
+        st[O11_TPEAK] = 0.0;
+        st[O11_TMP_DVDT] = -999.0; st[O11_TMP_PEAK] = -999.0; st[O11_TMP_VALLEY] = y[0];
+        pace++;
+        elig = 0;
+      }
+      s_dt[lane] = dt;
+    }
+    __syncthreads();
+    if (wave == 0) {
+      if (active) {
+        dt = s_dt[lane];
+        ord11::rl_gates<R>(dt, y, g);
+#pragma unroll
+        for (int i = 0; i < NGT; ++i) s_gate[i][lane] = y[ord11::G0 + i];
+      }
+    } else if (active) {
+      ord11::rl_core<R>(dt, y, f, g);
+      s_vc[0][lane] = y[0];
+      s_vc[1][lane] = y[2];
+      if (pace >= first_win) {
+        const double t_beat = bcl * pace + ord11::stim_start;
+        if (t > t_beat + 2 && t < t_beat + 10 && fabs(cur.INa) < 1) {
+          if (y[0] > st[O11_TMP_PEAK]) {
+            st[O11_TMP_PEAK] = y[0];
+            if (y[0] > 0) {
+              const double amp = st[O11_TMP_PEAK] - st[O11_TMP_VALLEY];
+              st[O11_R30] = st[O11_TMP_PEAK] - 0.3 * amp;
+              st[O11_R50] = st[O11_TMP_PEAK] - 0.5 * amp;
+              st[O11_R90] = st[O11_TMP_PEAK] - 0.9 * amp;
+              elig = 1;
+              st[O11_TPEAK] = t;
+            } else {
+              elig = 0;
+            }
+          }
+        } else if (t > t_beat + 10 && elig) {
+          if (f[0] > st[O11_TMP_DVDT] && y[0] <= st[O11_R30] && y[0] >= st[O11_R90]) st[O11_TMP_DVDT] = f[0];
+        }
+        const double inet = cur.INaL + cur.ICaL + cur.Ito + cur.IKr + cur.IKs + cur.IK1;
+        const double inet4 = cur.INaL + cur.ICaL + cur.IKr + cur.INa;
+        if (elig && y[0] > st[O11_R90]) {
+          st[O11_QAP] += (inet * dt) / 1000.;
+          st[O11_Q4AP] += (inet4 * dt) / 1000.;
+          st[O11_INALAP] += cur.INaL * dt;
+          st[O11_ICALAP] += cur.ICaL * dt;
+        }
+        st[O11_QCL] += (inet * dt) / 1000.;
+        st[O11_Q4CL] += (inet4 * dt) / 1000.;
+        st[O11_INALCL] += cur.INaL * dt;
+        st[O11_ICALCL] += cur.ICaL * dt;
+      }
+    }
+    __syncthreads();
+    if (active) {
+      if (wave == 0) {
+        y[0] = s_vc[0][lane];
+        y[2] = s_vc[1][lane];
+      } else {
+#pragma unroll
+        for (int i = 0; i < NGT; ++i) y[ord11::G0 + i] = s_gate[i][lane];
+      }
+      ++n;
+      t += dt;
+      active = t < tmax && n < max_steps;
+    }
+  }
+  if (!valid || wave == 0) return;
+  st[O11_T] = t;
+  bool fin = true;
+#pragma unroll
+  for (int i = 0; i < ord11::NS; ++i) { B.y[i * P_ + c] = y[i]; fin = fin && isfinite(y[i]); }
+  if (!fin) B.status[c] = ST_NONFINITE;
+  else if (t < tmax) B.status[c] = ST_MAX_STEPS;
+#pragma unroll
+  for (int k = 0; k < O11_SD_N; ++k) sd[k * P_] = st[k];
+  si[O11_I_PACE * P_] = pace; si[O11_I_ELIG * P_] = elig; si[O11_I_STEEP * P_] = steep;
+  B.stats[c] += n;              // steps (one right-hand side each)
+}
+
 // ---- ORd 2011 with the Adams PECE solver of the hackathon driver
 // (CEPCell_HACKATHON/modules/cipa_drug_shago_bench.cpp:119-146): nout[c] further output points
 // of dt each, every one an ode() call continuing the integration; optional Vm trace of the
@@ -1101,6 +1268,7 @@ struct cep_batch {
   double time_point = 25.0;
   int beats_done = 0;
   int integrator = CEP_INTEGRATOR_RUSH_LARSEN;
+  int o11_waves = 0;             // CEP_OPT_ORD11_WAVES: 0 automatic, 1 or 2 wavefronts per 64 cells
   double *d_aphi = nullptr, *d_avec = nullptr, *d_acoef = nullptr, *d_asc = nullptr;
   int *d_asi = nullptr;
   // device memory
@@ -1377,10 +1545,30 @@ int cep_set_model_option(cep_batch *b, int option, double value)
   if (b->model != CEP_MODEL_ORD2011) return fail(b, CEP_E_ARG, "no options for this model");
   if (option == CEP_OPT_DUTTA) b->dutta = value != 0.0;
   else if (option == CEP_OPT_TIME_POINT && value >= 0.0) b->time_point = value;
+  else if (option == CEP_OPT_ORD11_WAVES && (value == 0.0 || value == 1.0 || value == 2.0)) b->o11_waves = (int)value;
   else if (option == CEP_OPT_INTEGRATOR && (value == CEP_INTEGRATOR_RUSH_LARSEN || value == CEP_INTEGRATOR_ADAMS))
     b->integrator = (int)value;
   else return fail(b, CEP_E_ARG, "unknown option or value");
   return 0;
+}
+
+// ORd 2011 Rush-Larsen: two wavefronts per 64 cells while the doubled wave count still fits
+// one wave per SIMD (CUs x 4); beyond that the one-wave kernel, whose waves already fill the
+// SIMDs.  CEP_OPT_ORD11_WAVES forces either.
+static bool ord11_two_waves(const cep_batch *b)
+{
+  if (b->o11_waves == 1) return false;
+  if (b->o11_waves == 2) return true;
+  static int simds = 0;
+  if (simds == 0) {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+      cus = 256;
+    simds = 4 * cus;
+  }
+  const int blocks = (b->n + 63) / 64;
+  return 2 * blocks <= simds;
 }
 
 static double max_bcl(const cep_batch *b)
@@ -1622,10 +1810,16 @@ int cep_run_beats_async(cep_batch *b, int nbeats)
     // guard only: the smallest step the rule produces is 0.0005 ms
     const long long max_steps = (long long)nbeats * (long long)(max_bcl(b) / 0.0005) + 1000;
     const int blocks = (b->n + 63) / 64;
-    if (b->prec == CEP_PREC_FP64)
+    if (ord11_two_waves(b)) {
+      if (b->prec == CEP_PREC_FP64)
+        hipLaunchKernelGGL(ord11_rl2_kernel<double>, dim3(blocks), dim3(128), 0, b->stream, B, b->beats_done + nbeats, max_steps);
+      else
+        hipLaunchKernelGGL(ord11_rl2_kernel<float>, dim3(blocks), dim3(128), 0, b->stream, B, b->beats_done + nbeats, max_steps);
+    } else if (b->prec == CEP_PREC_FP64) {
       hipLaunchKernelGGL(ord11_rl_kernel<double>, dim3(blocks), dim3(64), 0, b->stream, B, b->beats_done + nbeats, max_steps);
-    else
+    } else {
       hipLaunchKernelGGL(ord11_rl_kernel<float>, dim3(blocks), dim3(64), 0, b->stream, B, b->beats_done + nbeats, max_steps);
+    }
     b->beats_done += nbeats;
   } else {
     TtBuffers B;

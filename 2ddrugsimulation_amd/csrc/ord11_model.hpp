@@ -47,6 +47,7 @@ constexpr int G0 = 10, NG = NS - G0;
 struct Gates {
   double inf[NG];
   double tau[NG];
+  double nca_on;          // anca * k2n: the production term of dn/dt
 };
 
 template <typename R>
@@ -65,26 +66,16 @@ CEP_D float ghk11<float>(float A, float U, float eU)
   return A * __fdividef(U, eU - 1.0f);
 }
 
-// computeRates.  y, f: NS doubles.  g: steady states / time constants of the gates for the
-// Rush-Larsen step.  tissue_stim < 0: the single-cell periodic stimulus; 0 / 1: stimulus
-// off / on (the TISSUE build).
+// The voltage- and calcium-dependent gate kinetics of computeRates (:750-801): the steady
+// states and time constants of gates 10-38 (nca: its production term and rate).  They read
+// only V, cass and jca, so a second wavefront can compute them beside current_rates.
 template <typename R, typename PA, typename YV>
-CEP_D void rhs(double t, const YV &y, double *f, const PA &P, Currents &cur, Gates &g, int tissue_stim = -1)
+CEP_D void gate_rates(const YV &y, const PA &P, Gates &g)
 {
   using Mx = M<R>;
-  using namespace ord;   // shared physical constants (same values in both models)
+  using namespace ord;
   const R v = (R)y[0];
-  const R camkt = (R)y[1], cass = (R)y[2], nai = (R)y[3], nass = (R)y[4], ki = (R)y[5];
-  const R kss = (R)y[6], cansr = (R)y[7], cajsr = (R)y[8], cai = (R)y[9];
-
-  double Istim;
-  if (tissue_stim >= 0) {
-    Istim = tissue_stim ? P(P_STIM_AMP) : 0.0;
-  } else {
-    const double ts = t - stim_start, bcl = P(P_BCL);
-    Istim = (t >= stim_start && ts - floor(ts / bcl) * bcl <= stim_dur) ? P(P_STIM_AMP) : 0.0;
-  }
-  cur.Istim = Istim;
+  const R cass = (R)y[2];
 
   // ---- gates (:750-801)
   const R hLss = Mx::rcp(cc<R>(1) + Mx::ex((v + cc<R>(87.61)) / cc<R>(7.488)));
@@ -121,7 +112,6 @@ CEP_D void rhs(double t, const YV &y, double *f, const PA &P, Currents &cur, Gat
   const R tffp = cc<R>(2.5) * tff;
   const R xs2ss = xs1ss;
   const R txs2 = Mx::rcp(cc<R>(0.01) * Mx::ex((v - cc<R>(50)) / cc<R>(20)) + cc<R>(0.0193) * Mx::ex(-(v + cc<R>(66.54)) / cc<R>(31)));
-  const R CaMKb = Mx::dv(cc<R>(CaMKo) * (cc<R>(1) - camkt), cc<R>(1) + Mx::dv(cc<R>(KmCaM), cass));
   const R hssp = Mx::rcp(cc<R>(1) + Mx::ex((v + cc<R>(89.1)) / cc<R>(6.086)));
   const R thsp = cc<R>(3) * ths;
   const R tjp = cc<R>(1.46) * tj;
@@ -135,6 +125,46 @@ CEP_D void rhs(double t, const YV &y, double *f, const PA &P, Currents &cur, Gat
   const R dti_recover = cc<R>(1) - Mx::dv(cc<R>(0.5), cc<R>(1) + Mx::ex((v + cc<R>(70)) / cc<R>(20)));
   const R tiFp = dti_develop * dti_recover * tiF;
   const R tiSp = dti_develop * dti_recover * tiS;
+
+  auto gate = [&](int i, R inf, R tau) {
+    g.inf[i - G0] = (double)inf;
+    g.tau[i - G0] = (double)tau;
+  };
+  gate(10, mss, tm); gate(11, hss, thf); gate(12, hss, ths); gate(13, jss, tj); gate(14, hssp, thsp);
+  gate(15, jss, tjp); gate(16, mLss, tmL); gate(17, hLss, cc<R>(thL)); gate(18, hLssp, cc<R>(thLp));
+  gate(19, ass, ta); gate(20, iss, tiF); gate(21, iss, tiS); gate(22, assp, ta); gate(23, iss, tiFp);
+  gate(24, iss, tiSp); gate(25, dss, td); gate(26, fss, tff); gate(27, fss, tfs); gate(28, fcass, tfcaf);
+  gate(29, fcass, tfcas); gate(30, fcass, (R)P(P_TJCA)); gate(31, fss, tffp); gate(32, fcass, tfcafp);
+  gate(34, xrss, txrf); gate(35, xrss, txrs); gate(36, xs1ss, txs1); gate(37, xs2ss, txs2);
+  gate(38, xk1ss, txk1);
+  // nca: dn/dt = anca k2n - n km2n; the step uses inf = anca k2n / km2n and rate km2n
+  g.nca_on = (double)(anca * cc<R>(k2n));
+  g.inf[33 - G0] = (double)(anca * cc<R>(k2n) / km2n);
+  g.tau[33 - G0] = (double)km2n;     // a rate, not a time constant (see rl_step)
+}
+
+// The rest of computeRates (:802-989): stimulus, currents, fluxes, the Jrel relaxations (39, 40:
+// they depend on ICaL) and the rates f[0..9] of V, CaMKt and the concentrations.
+// tissue_stim < 0: the single-cell periodic stimulus; 0 / 1: stimulus off / on (the TISSUE build).
+template <typename R, typename PA, typename YV>
+CEP_D void current_rates(double t, const YV &y, double *f, const PA &P, Currents &cur, Gates &g, int tissue_stim = -1)
+{
+  using Mx = M<R>;
+  using namespace ord;   // shared physical constants (same values in both models)
+  const R v = (R)y[0];
+  const R camkt = (R)y[1], cass = (R)y[2], nai = (R)y[3], nass = (R)y[4], ki = (R)y[5];
+  const R kss = (R)y[6], cansr = (R)y[7], cajsr = (R)y[8], cai = (R)y[9];
+
+  double Istim;
+  if (tissue_stim >= 0) {
+    Istim = tissue_stim ? P(P_STIM_AMP) : 0.0;
+  } else {
+    const double ts = t - stim_start, bcl = P(P_BCL);
+    Istim = (t >= stim_start && ts - floor(ts / bcl) * bcl <= stim_dur) ? P(P_STIM_AMP) : 0.0;
+  }
+  cur.Istim = Istim;
+  const R jca = (R)y[30];
+  const R CaMKb = Mx::dv(cc<R>(CaMKo) * (cc<R>(1) - camkt), cc<R>(1) + Mx::dv(cc<R>(KmCaM), cass));
 
   // ---- ICaL / ICaNa / ICaK
   const R Aff = (R)P(P_AFF);
@@ -278,23 +308,12 @@ CEP_D void rhs(double t, const YV &y, double *f, const PA &P, Currents &cur, Gat
   const R Jtr = (cansr - cajsr) / cc<R>(100);
   const R Bcajsr = Mx::rcp(cc<R>(1) + Mx::dv(cc<R>(csqnmax * kmcsqn), (cc<R>(kmcsqn) + cajsr) * (cc<R>(kmcsqn) + cajsr)));
 
-  // ---- gate relaxations: rates and the Rush-Larsen data
-  auto gate = [&](int i, R inf, R tau) {
-    g.inf[i - G0] = (double)inf;
-    g.tau[i - G0] = (double)tau;
-    f[i] = ((double)inf - y[i]) / (double)tau;
-  };
-  gate(10, mss, tm); gate(11, hss, thf); gate(12, hss, ths); gate(13, jss, tj); gate(14, hssp, thsp);
-  gate(15, jss, tjp); gate(16, mLss, tmL); gate(17, hLss, cc<R>(thL)); gate(18, hLssp, cc<R>(thLp));
-  gate(19, ass, ta); gate(20, iss, tiF); gate(21, iss, tiS); gate(22, assp, ta); gate(23, iss, tiFp);
-  gate(24, iss, tiSp); gate(25, dss, td); gate(26, fss, tff); gate(27, fss, tfs); gate(28, fcass, tfcaf);
-  gate(29, fcass, tfcas); gate(30, fcass, (R)P(P_TJCA)); gate(31, fss, tffp); gate(32, fcass, tfcafp);
-  gate(34, xrss, txrf); gate(35, xrss, txrs); gate(36, xs1ss, txs1); gate(37, xs2ss, txs2);
-  gate(38, xk1ss, txk1); gate(39, Jrel_inf, tau_rel); gate(40, Jrel_infp, tau_relp);
-  // nca: dn/dt = anca k2n - n km2n; the step uses inf = anca k2n / km2n and rate km2n
-  f[33] = (double)(anca * cc<R>(k2n)) - y[33] * (double)km2n;
-  g.inf[33 - G0] = (double)(anca * cc<R>(k2n) / km2n);
-  g.tau[33 - G0] = (double)km2n;     // a rate, not a time constant (see rl_step)
+  g.inf[39 - G0] = (double)Jrel_inf;
+  g.tau[39 - G0] = (double)tau_rel;
+  f[39] = ((double)Jrel_inf - y[39]) / (double)tau_rel;
+  g.inf[40 - G0] = (double)Jrel_infp;
+  g.tau[40 - G0] = (double)tau_relp;
+  f[40] = ((double)Jrel_infp - y[40]) / (double)tau_relp;
 
   // ---- CaMK, concentrations, voltage (in double from the R-precision currents)
   f[1] = (double)(cc<R>(aCaMK) * CaMKb * (CaMKb + camkt) - cc<R>(bCaMK) * camkt);
@@ -318,14 +337,30 @@ CEP_D void rhs(double t, const YV &y, double *f, const PA &P, Currents &cur, Gat
   cur.INa = INa; cur.INaL = INaL; cur.Ito = Ito; cur.ICaL = ICaL; cur.IKr = IKr; cur.IKs = IKs; cur.IK1 = IK1;
 }
 
-// solveAnalytical: Rush-Larsen for the gates and the two Jrel states, forward Euler for CaMKt,
-// V and the concentrations, with the rates and relaxations of the last rhs() call
-template <typename R>
-CEP_D void rl_step(double dt, double *y, const double *f, const Gates &g)
+// computeRates.  y, f: NS doubles.  g: steady states / time constants of the gates for the
+// Rush-Larsen step.  tissue_stim < 0: the single-cell periodic stimulus; 0 / 1: stimulus
+// off / on (the TISSUE build).
+template <typename R, typename PA, typename YV>
+CEP_D void rhs(double t, const YV &y, double *f, const PA &P, Currents &cur, Gates &g, int tissue_stim = -1)
+{
+  gate_rates<R>(y, P, g);
+  current_rates<R>(t, y, f, P, cur, g, tissue_stim);
+#pragma unroll
+  for (int i = G0; i <= 38; ++i)
+    if (i != 33) f[i] = (g.inf[i - G0] - y[i]) / g.tau[i - G0];
+  f[33] = g.nca_on - y[33] * g.tau[33 - G0];
+}
+
+// solveAnalytical (:991-1095) in two parts, each element updated as the reference does:
+// rl_gates, Rush-Larsen for the gates 10-38 (nca with its rate); rl_core, Rush-Larsen for the
+// two Jrel states and forward Euler for V, CaMKt and the concentrations.  rl_step is both,
+// with the rates and relaxations of the last rhs() call.
+template <typename R, typename YV>
+CEP_D void rl_gates(double dt, YV &y, const Gates &g)
 {
   using Mx = M<R>;
 #pragma unroll
-  for (int i = G0; i < NS; ++i) {
+  for (int i = G0; i <= 38; ++i) {
     if (i == 33) continue;
     const double inf = g.inf[i - G0];
     // fcafp: exp(-25 / tfcafp), the reference's use of the d gate's index in place of dt
@@ -333,8 +368,26 @@ CEP_D void rl_step(double dt, double *y, const double *f, const Gates &g)
     y[i] = inf - (inf - y[i]) * (double)Mx::ex((R)arg);
   }
   y[33] = g.inf[33 - G0] - (g.inf[33 - G0] - y[33]) * (double)Mx::ex((R)(-g.tau[33 - G0] * dt));
+}
+
+template <typename R>
+CEP_D void rl_core(double dt, double *y, const double *f, const Gates &g)
+{
+  using Mx = M<R>;
+#pragma unroll
+  for (int i = 39; i <= 40; ++i) {
+    const double inf = g.inf[i - G0];
+    y[i] = inf - (inf - y[i]) * (double)Mx::ex((R)(-dt / g.tau[i - G0]));
+  }
 #pragma unroll
   for (int i = 0; i <= 9; ++i) y[i] += f[i] * dt;
+}
+
+template <typename R>
+CEP_D void rl_step(double dt, double *y, const double *f, const Gates &g)
+{
+  rl_gates<R>(dt, y, g);
+  rl_core<R>(dt, y, f, g);
 }
 
 // set_time_step: 0.005 ms inside the first time_point ms of every beat, else from |dV/dt|

@@ -137,6 +137,11 @@ int cep_set_cells(cep_batch *b, int first, int count, const double *celltype, co
    Is_Print_Graph (CEPCell_NEO/modules/cipa_drug_bench.cpp:168-199,230-235,262,338-350):
    t, Vm, Ca_i and INaL ICaL Ito IKr IKs IK1 at every dt_write (1 ms) point of the beat. */
 #define CEP_OPT_TIME_SERIES 4
+/* CEP_OPT_ORD11_WAVES (ORd 2011, Rush-Larsen): wavefronts per 64 cells.  0 (default) chooses:
+   two (gate kinetics on one, currents and features on the other, on two SIMDs) while twice the
+   wave count fits one wave per SIMD, else one; 1 or 2 forces it.  Results are the same either
+   way up to floating-point contraction.  No reference counterpart (a launch-shape choice). */
+#define CEP_OPT_ORD11_WAVES 5
 int cep_set_model_option(cep_batch *b, int option, double value);
 
 /* Integration / biomarker configuration shared by the batch.
