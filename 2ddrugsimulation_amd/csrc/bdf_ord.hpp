@@ -194,6 +194,15 @@ struct Lane {
   unsigned c;
   int p;
 
+  // powers of the step-size and order heuristics (the eta factors, the order-change
+  // quotient), in the precision of the rates: they only steer the choice of step and order,
+  // and an FP64 pow is a long serial sequence on the integrator lanes
+  CEP_D static double hpow(double x, double e)
+  {
+    if constexpr (sizeof(R) == 4) return (double)__powf((float)x, (float)e);
+    else return pow(x, e);
+  }
+
   // (the mask tells the compiler the slot offset is small, so constants fold into the
   // instruction's immediate offset)
   CEP_D BElem blk(Rsrc r, int k) const { return BElem{r, (ln & 63u) * 8u, k * CPB * 8}; }
@@ -903,7 +912,7 @@ struct Lane {
     m.c_save = save;
     m.c_dn = norms && q > 1;
     m.c_up = up;
-    m.c_cquot = up ? (m.tq5 / m.saved_tq5) * pow(m.h / m.tau[2], (double)(q + 1)) : 0.0;
+    m.c_cquot = up ? (m.tq5 / m.saved_tq5) * hpow(m.h / m.tau[2], (double)(q + 1)) : 0.0;
     m.c_dsm = dsm;
   }
   // complete_step's sweep over group g's components; the norm terms go to t2 / t3
@@ -1060,7 +1069,7 @@ struct Lane {
     const bool norms = (m.etamax != 1.0) && (m.qwait == 0);
     const bool dn = norms && q > 1;
     const bool up = norms && (q != B.maxord) && (m.saved_tq5 != 0.0);
-    const double cquot = up ? (m.tq5 / m.saved_tq5) * pow(m.h / m.tau[2], (double)(q + 1)) : 0.0;
+    const double cquot = up ? (m.tq5 / m.saved_tq5) * hpow(m.h / m.tau[2], (double)(q + 1)) : 0.0;
     // coefficients masked to the current order (no order-dependent branches in the sweep);
     // oh selects row q for the order-decrease norm
     double l[QMAX + 1], oh[QMAX + 1];
@@ -1128,7 +1137,7 @@ struct Lane {
       return;
     }
     const int q = m.q;
-    const double etaq = 1.0 / (pow(BIAS2 * dsm, 1.0 / (q + 1)) + ADDON);
+    const double etaq = 1.0 / (hpow(BIAS2 * dsm, 1.0 / (q + 1)) + ADDON);
     if (m.qwait != 0) {
       m.eta = etaq;
       m.qprime = q;
@@ -1139,11 +1148,11 @@ struct Lane {
     double etaqm1 = 0.0, etaqp1 = 0.0;
     if (dn) {
       const double ddn = sqrt(s2 / N) * m.tq1;
-      etaqm1 = 1.0 / (pow(BIAS1 * ddn, 1.0 / q) + ADDON);
+      etaqm1 = 1.0 / (hpow(BIAS1 * ddn, 1.0 / q) + ADDON);
     }
     if (up) {
       const double dup = sqrt(s3 / N) * m.tq3;
-      etaqp1 = 1.0 / (pow(BIAS3 * dup, 1.0 / (q + 2)) + ADDON);
+      etaqp1 = 1.0 / (hpow(BIAS3 * dup, 1.0 / (q + 2)) + ADDON);
     }
     double etam = etaq;
     if (etaqm1 > etam) etam = etaqm1;
