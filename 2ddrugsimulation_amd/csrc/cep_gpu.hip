@@ -293,7 +293,7 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
           } else {
             m.etamax = 1.0;
             if (m.nef <= MXNEF1) {
-              double eta = 1.0 / (pow(BIAS2 * dsm, 1.0 / (m.q + 1)) + ADDON);
+              double eta = 1.0 / (Lane<R>::hpow(BIAS2 * dsm, 1.0 / (m.q + 1)) + ADDON);
               if (eta < ETAMIN) eta = ETAMIN;
               if (m.nef >= SMALL_NEF && eta > ETAMXF) eta = ETAMXF;
               m.eta = eta;
@@ -752,6 +752,242 @@ __global__ void __launch_bounds__(64) ord11_rl_kernel(O11Buffers B, int beats_en
   for (int k = 0; k < O11_SD_N; ++k) sd[k * P_] = st[k];
   si[O11_I_PACE * P_] = pace; si[O11_I_ELIG * P_] = elig; si[O11_I_STEEP * P_] = steep;
   B.stats[c] += n;              // steps (one right-hand side each)
+}
+
+// The loop's bookkeeping shared by the split kernels below (the one-wavefront kernel above has
+// the same code inline): the beat-end selection of the steepest beat's features, and the
+// per-step features (drug_sim.cpp:306-420).
+struct O11Loop {
+  double st[O11_SD_N];
+  int pace, elig, steep;
+  double best_dvdt;
+};
+
+CEP_D void o11_beat_end(O11Loop &L, double *sd, size_t P_, int first_win, double v0)
+{
+  double *st = L.st;
+  if (L.elig && L.pace >= first_win) {
+    if (st[O11_TMP_DVDT] > L.best_dvdt) {
+      L.best_dvdt = st[O11_TMP_DVDT];
+      L.steep = L.pace;
+      double *bs = sd + O11_BEST * P_;
+      bs[CEP_BM11_QNET_AP * P_] = st[O11_QAP];
+      bs[CEP_BM11_QNET4_AP * P_] = st[O11_Q4AP];
+      bs[CEP_BM11_INAL_AUC_AP * P_] = st[O11_INALAP];
+      bs[CEP_BM11_ICAL_AUC_AP * P_] = st[O11_ICALAP];
+      bs[CEP_BM11_QNET_CL * P_] = st[O11_QCL];
+      bs[CEP_BM11_QNET4_CL * P_] = st[O11_Q4CL];
+      bs[CEP_BM11_INAL_AUC_CL * P_] = st[O11_INALCL];
+      bs[CEP_BM11_ICAL_AUC_CL * P_] = st[O11_ICALCL];
+      bs[CEP_BM11_VM_PEAK * P_] = st[O11_TMP_PEAK];
+      bs[CEP_BM11_VM_VALLEY * P_] = st[O11_TMP_VALLEY];
+      bs[CEP_BM11_DVMDT_REPOL * P_] = st[O11_TMP_DVDT];
+      bs[CEP_BM11_T_PEAK * P_] = st[O11_TPEAK];
+      bs[CEP_BM11_VM_REPOL30 * P_] = st[O11_R30];
+      bs[CEP_BM11_VM_REPOL50 * P_] = st[O11_R50];
+      bs[CEP_BM11_VM_REPOL90 * P_] = st[O11_R90];
+    }
+  }
+#pragma unroll
+  for (int k = O11_QAP; k <= O11_ICALCL; ++k) st[k] = 0.0;
+  st[O11_TPEAK] = 0.0;
+  st[O11_TMP_DVDT] = -999.0; st[O11_TMP_PEAK] = -999.0; st[O11_TMP_VALLEY] = v0;
+  L.pace++;
+  L.elig = 0;
+}
+
+CEP_D void o11_features(O11Loop &L, int first_win, double bcl, double t, double dt, double v, double dvdt,
+                        const ord11::Currents &cur)
+{
+  double *st = L.st;
+  if (L.pace < first_win) return;
+  const double t_beat = bcl * L.pace + ord11::stim_start;
+  if (t > t_beat + 2 && t < t_beat + 10 && fabs(cur.INa) < 1) {
+    if (v > st[O11_TMP_PEAK]) {
+      st[O11_TMP_PEAK] = v;
+      if (v > 0) {
+        const double amp = st[O11_TMP_PEAK] - st[O11_TMP_VALLEY];
+        st[O11_R30] = st[O11_TMP_PEAK] - 0.3 * amp;
+        st[O11_R50] = st[O11_TMP_PEAK] - 0.5 * amp;
+        st[O11_R90] = st[O11_TMP_PEAK] - 0.9 * amp;
+        L.elig = 1;
+        st[O11_TPEAK] = t;
+      } else {
+        L.elig = 0;
+      }
+    }
+  } else if (t > t_beat + 10 && L.elig) {
+    if (dvdt > st[O11_TMP_DVDT] && v <= st[O11_R30] && v >= st[O11_R90]) st[O11_TMP_DVDT] = dvdt;
+  }
+  const double inet = cur.INaL + cur.ICaL + cur.Ito + cur.IKr + cur.IKs + cur.IK1;
+  const double inet4 = cur.INaL + cur.ICaL + cur.IKr + cur.INa;
+  if (L.elig && v > st[O11_R90]) {
+    st[O11_QAP] += (inet * dt) / 1000.;
+    st[O11_Q4AP] += (inet4 * dt) / 1000.;
+    st[O11_INALAP] += cur.INaL * dt;
+    st[O11_ICALAP] += cur.ICaL * dt;
+  }
+  st[O11_QCL] += (inet * dt) / 1000.;
+  st[O11_Q4CL] += (inet4 * dt) / 1000.;
+  st[O11_INALCL] += cur.INaL * dt;
+  st[O11_ICALCL] += cur.ICaL * dt;
+}
+
+// loads the loop state of cell c (and initialises it on the first launch); false if the
+// cell does not run
+CEP_D void o11_load_loop(O11Loop &L, const O11Buffers &B, int c, size_t P_, double v0)
+{
+  double *sd = B.bmd + c;
+  int *si = B.bmi + c;
+#pragma unroll
+  for (int k = 0; k < O11_SD_N; ++k) L.st[k] = sd[k * P_];
+  L.pace = si[O11_I_PACE * P_]; L.elig = si[O11_I_ELIG * P_]; L.steep = si[O11_I_STEEP * P_];
+  L.best_dvdt = sd[(O11_BEST + CEP_BM11_DVMDT_REPOL) * P_];
+  if (si[O11_I_INIT * P_] == 0) {
+    L.st[O11_TMP_DVDT] = -999.0; L.st[O11_TMP_PEAK] = -999.0; L.st[O11_TMP_VALLEY] = v0;
+    for (int k = 0; k < CEP_NBM; ++k) sd[(O11_BEST + k) * P_] = 0.0;
+    sd[(O11_BEST + CEP_BM11_DVMDT_REPOL) * P_] = -999.0;
+    sd[(O11_BEST + CEP_BM11_VM_PEAK) * P_] = -999.0;
+    sd[(O11_BEST + CEP_BM11_VM_VALLEY) * P_] = v0;
+    L.best_dvdt = -999.0;
+    si[O11_I_INIT * P_] = 1;
+  }
+}
+
+// stores the states, status, loop state and step count of cell c
+CEP_D void o11_store(const O11Buffers &B, int c, size_t P_, const double *y, O11Loop &L, double t, double tmax,
+                     long long n)
+{
+  double *sd = B.bmd + c;
+  int *si = B.bmi + c;
+  L.st[O11_T] = t;
+  bool fin = true;
+#pragma unroll
+  for (int i = 0; i < ord11::NS; ++i) { B.y[i * P_ + c] = y[i]; fin = fin && isfinite(y[i]); }
+  if (!fin) B.status[c] = ST_NONFINITE;
+  else if (t < tmax) B.status[c] = ST_MAX_STEPS;
+#pragma unroll
+  for (int k = 0; k < O11_SD_N; ++k) sd[k * P_] = L.st[k];
+  si[O11_I_PACE * P_] = L.pace; si[O11_I_ELIG * P_] = L.elig; si[O11_I_STEEP * P_] = L.steep;
+  B.stats[c] += n;              // steps (one right-hand side each)
+}
+
+// The step of each cell split over the four wavefronts of a 256-thread workgroup (lane l of
+// each holds cell blockIdx.x * 64 + l), on four SIMDs:
+//   wave 0: the INa / INaL / Ito gate kinetics and their Rush-Larsen update (gates 10-24);
+//   wave 1: the ICaL / IKr / IKs / IK1 gate kinetics and their update (gates 25-38);
+//   wave 2: the Na and K currents;
+//   wave 3: the Ca currents, pumps, exchangers and fluxes, then the rates, the step size, the
+//           Jrel / Euler update of V and the concentrations, and the features.
+// Per step, three barriers: after the rates and currents (wave 2's currents to LDS), after
+// the step size (to LDS), after the updates (gates, V and concentrations to LDS).  Every wave
+// sees the same t and step count for every lane, so all run the same number of passes.  Per
+// cell the arithmetic is that of ord11_rl_kernel.
+template <typename R>
+__global__ void __launch_bounds__(256) ord11_rl4_kernel(O11Buffers B, int beats_end, long long max_steps)
+{
+  constexpr int NGT = 38 - ord11::G0 + 1;       // gates 10..38 (incl. nca)
+  __shared__ double s_gate[NGT][64];
+  __shared__ double s_core[10][64];             // V, CaMKt and the concentrations
+  __shared__ double s_nak[8][64];               // wave 2's currents
+  __shared__ double s_dt[64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c = blockIdx.x * 64 + lane;
+  const bool valid = c < B.ncell && B.status[c] == ST_OK;
+  const size_t P_ = (size_t)B.pitch;
+  const PrmAcc P{B.prm + (valid ? c : 0), B.pitch};
+  double y[ord11::NS];
+  O11Loop L;
+  double t = 0.0, bcl = 1.0;
+  if (valid) {
+#pragma unroll
+    for (int i = 0; i < ord11::NS; ++i) y[i] = B.y[i * P_ + c];
+    bcl = P(ord11::P_BCL);
+    t = B.bmd[c + O11_T * P_];
+    if (wave == 3) o11_load_loop(L, B, c, P_, y[0]);
+  }
+  const double tmax = beats_end * bcl;
+  const int first_win = B.nbeats_total - B.nwin;
+  long long n = 0;
+  bool active = valid && t < tmax && n < max_steps;
+  while (__ballot(active) != 0ull) {
+    ord11::Gates g;
+    ord11::CaCurrents C;
+    ord11::Currents cur;
+    double f[ord11::NS];
+    double dt = 0.0;
+    // ---- rates and currents
+    if (active) {
+      if (wave == 0) {
+        ord11::gate_rates_na<R>(y, P, g);
+      } else if (wave == 1) {
+        ord11::gate_rates_ca<R>(y, P, g);
+      } else if (wave == 2) {
+        ord11::NaKCurrents N;
+        ord11::nak_currents<R>(y, P, N);
+        s_nak[0][lane] = N.INa; s_nak[1][lane] = N.INaL; s_nak[2][lane] = N.Ito; s_nak[3][lane] = N.IKr;
+        s_nak[4][lane] = N.IKs; s_nak[5][lane] = N.IK1; s_nak[6][lane] = N.IKb; s_nak[7][lane] = N.INab;
+      } else {
+        ord11::ca_currents<R>(y, P, C);
+      }
+    }
+    __syncthreads();
+    // ---- rates of V and the concentrations, step size, beat end
+    if (active && wave == 3) {
+      ord11::NaKCurrents N;
+      N.INa = s_nak[0][lane]; N.INaL = s_nak[1][lane]; N.Ito = s_nak[2][lane]; N.IKr = s_nak[3][lane];
+      N.IKs = s_nak[4][lane]; N.IK1 = s_nak[5][lane]; N.IKb = s_nak[6][lane]; N.INab = s_nak[7][lane];
+      ord11::rates_from_currents<R>(t, y, f, P, cur, g, N, C);
+      const double dts = ord11::time_step(t, B.time_point, B.max_dt, bcl, f[0]);
+      if (floor((t + dts) / bcl) == floor(t / bcl)) {
+        dt = dts;
+      } else {
+        dt = (floor(t / bcl) + 1) * bcl - t;
+        o11_beat_end(L, B.bmd + c, P_, first_win, y[0]);
+      }
+      s_dt[lane] = dt;
+    }
+    __syncthreads();
+    // ---- updates
+    if (active) {
+      if (wave != 3) dt = s_dt[lane];
+      if (wave == 0) {
+        ord11::rl_gates_na<R>(dt, y, g);
+#pragma unroll
+        for (int i = ord11::G0; i <= 24; ++i) s_gate[i - ord11::G0][lane] = y[i];
+      } else if (wave == 1) {
+        ord11::rl_gates_ca<R>(dt, y, g);
+#pragma unroll
+        for (int i = 25; i <= 38; ++i) s_gate[i - ord11::G0][lane] = y[i];
+      } else if (wave == 3) {
+        ord11::rl_core<R>(dt, y, f, g);
+#pragma unroll
+        for (int i = 0; i <= 9; ++i) s_core[i][lane] = y[i];
+        o11_features(L, first_win, bcl, t, dt, y[0], f[0], cur);
+      }
+    }
+    __syncthreads();
+    // ---- every wave takes the new state it reads
+    if (active) {
+      if (wave == 0) {
+        y[0] = s_core[0][lane];
+      } else if (wave == 1) {
+        y[0] = s_core[0][lane];
+        y[2] = s_core[2][lane];
+      } else {
+#pragma unroll
+        for (int i = 0; i <= 9; ++i) if (wave == 2) y[i] = s_core[i][lane];
+#pragma unroll
+        for (int i = ord11::G0; i <= 38; ++i)
+          if (wave == 3 || i <= 24 || i >= 34) y[i] = s_gate[i - ord11::G0][lane];
+      }
+      ++n;
+      t += dt;
+      active = t < tmax && n < max_steps;
+    }
+  }
+  if (!valid || wave != 3) return;
+  o11_store(B, c, P_, y, L, t, tmax, n);
 }
 
 // The same loop with each cell's step split over the two wavefronts of a 128-thread workgroup
@@ -1268,7 +1504,7 @@ struct cep_batch {
   double time_point = 25.0;
   int beats_done = 0;
   int integrator = CEP_INTEGRATOR_RUSH_LARSEN;
-  int o11_waves = 0;             // CEP_OPT_ORD11_WAVES: 0 automatic, 1 or 2 wavefronts per 64 cells
+  int o11_waves = 0;             // CEP_OPT_ORD11_WAVES: 0 automatic, 1, 2 or 4 wavefronts per 64 cells
   double *d_aphi = nullptr, *d_avec = nullptr, *d_acoef = nullptr, *d_asc = nullptr;
   int *d_asi = nullptr;
   // device memory
@@ -1545,20 +1781,20 @@ int cep_set_model_option(cep_batch *b, int option, double value)
   if (b->model != CEP_MODEL_ORD2011) return fail(b, CEP_E_ARG, "no options for this model");
   if (option == CEP_OPT_DUTTA) b->dutta = value != 0.0;
   else if (option == CEP_OPT_TIME_POINT && value >= 0.0) b->time_point = value;
-  else if (option == CEP_OPT_ORD11_WAVES && (value == 0.0 || value == 1.0 || value == 2.0)) b->o11_waves = (int)value;
+  else if (option == CEP_OPT_ORD11_WAVES && (value == 0.0 || value == 1.0 || value == 2.0 || value == 4.0))
+    b->o11_waves = (int)value;
   else if (option == CEP_OPT_INTEGRATOR && (value == CEP_INTEGRATOR_RUSH_LARSEN || value == CEP_INTEGRATOR_ADAMS))
     b->integrator = (int)value;
   else return fail(b, CEP_E_ARG, "unknown option or value");
   return 0;
 }
 
-// ORd 2011 Rush-Larsen: two wavefronts per 64 cells while the doubled wave count still fits
-// one wave per SIMD (CUs x 4); beyond that the one-wave kernel, whose waves already fill the
-// SIMDs.  CEP_OPT_ORD11_WAVES forces either.
-static bool ord11_two_waves(const cep_batch *b)
+// ORd 2011 Rush-Larsen: wavefronts per 64 cells.  Four while four times the wave count still
+// fits one wave per SIMD (CUs x 4), else two while that fits, else one, whose waves already
+// fill the SIMDs.  CEP_OPT_ORD11_WAVES forces a choice.
+static int ord11_waves(const cep_batch *b)
 {
-  if (b->o11_waves == 1) return false;
-  if (b->o11_waves == 2) return true;
+  if (b->o11_waves == 1 || b->o11_waves == 2 || b->o11_waves == 4) return b->o11_waves;
   static int simds = 0;
   if (simds == 0) {
     int dev = 0, cus = 0;
@@ -1568,7 +1804,7 @@ static bool ord11_two_waves(const cep_batch *b)
     simds = 4 * cus;
   }
   const int blocks = (b->n + 63) / 64;
-  return 2 * blocks <= simds;
+  return 4 * blocks <= simds ? 4 : 2 * blocks <= simds ? 2 : 1;
 }
 
 static double max_bcl(const cep_batch *b)
@@ -1810,7 +2046,13 @@ int cep_run_beats_async(cep_batch *b, int nbeats)
     // guard only: the smallest step the rule produces is 0.0005 ms
     const long long max_steps = (long long)nbeats * (long long)(max_bcl(b) / 0.0005) + 1000;
     const int blocks = (b->n + 63) / 64;
-    if (ord11_two_waves(b)) {
+    const int waves = ord11_waves(b);
+    if (waves == 4) {
+      if (b->prec == CEP_PREC_FP64)
+        hipLaunchKernelGGL(ord11_rl4_kernel<double>, dim3(blocks), dim3(256), 0, b->stream, B, b->beats_done + nbeats, max_steps);
+      else
+        hipLaunchKernelGGL(ord11_rl4_kernel<float>, dim3(blocks), dim3(256), 0, b->stream, B, b->beats_done + nbeats, max_steps);
+    } else if (waves == 2) {
       if (b->prec == CEP_PREC_FP64)
         hipLaunchKernelGGL(ord11_rl2_kernel<double>, dim3(blocks), dim3(128), 0, b->stream, B, b->beats_done + nbeats, max_steps);
       else

@@ -67,17 +67,17 @@ CEP_D float ghk11<float>(float A, float U, float eU)
 }
 
 // The voltage- and calcium-dependent gate kinetics of computeRates (:750-801): the steady
-// states and time constants of gates 10-38 (nca: its production term and rate).  They read
-// only V, cass and jca, so a second wavefront can compute them beside current_rates.
+// states and time constants of gates 10-38 (nca: its production term and rate), in two
+// independent groups: gate_rates_na, the INa / INaL / Ito gates 10-24 (V only), and
+// gate_rates_ca, the ICaL / IKr / IKs / IK1 gates 25-38 (V, cass and jca).  Other wavefronts
+// can compute them beside the currents.
 template <typename R, typename PA, typename YV>
-CEP_D void gate_rates(const YV &y, const PA &P, Gates &g)
+CEP_D void gate_rates_na(const YV &y, const PA &P, Gates &g)
 {
   using Mx = M<R>;
   using namespace ord;
+  (void)P;
   const R v = (R)y[0];
-  const R cass = (R)y[2];
-
-  // ---- gates (:750-801)
   const R hLss = Mx::rcp(cc<R>(1) + Mx::ex((v + cc<R>(87.61)) / cc<R>(7.488)));
   const R hLssp = Mx::rcp(cc<R>(1) + Mx::ex((v + cc<R>(93.81)) / cc<R>(7.488)));
   const R mss = Mx::rcp(cc<R>(1) + Mx::ex(-(v + cc<R>(mssV1)) / cc<R>(mssV2)));
@@ -88,6 +88,38 @@ CEP_D void gate_rates(const YV &y, const PA &P, Gates &g)
   const R ass = Mx::rcp(cc<R>(1) + Mx::ex(-(v - cc<R>(14.34)) / cc<R>(14.82)));
   const R ta = Mx::dv(cc<R>(1.0515), Mx::rcp(cc<R>(1.2089) * (cc<R>(1) + Mx::ex(-(v - cc<R>(18.4099)) / cc<R>(29.3814))))
                                         + Mx::dv(cc<R>(3.5), cc<R>(1) + Mx::ex((v + cc<R>(100)) / cc<R>(29.3814))));
+  const R jss = hss;
+  const R tj = cc<R>(2.038) + Mx::rcp(cc<R>(0.02136) * Mx::ex(-(v + cc<R>(100.6)) / cc<R>(8.281)) + cc<R>(0.3052) * Mx::ex((v + cc<R>(0.9941)) / cc<R>(38.45)));
+  const R assp = Mx::rcp(cc<R>(1) + Mx::ex(-(v - cc<R>(24.34)) / cc<R>(14.82)));
+  const R hssp = Mx::rcp(cc<R>(1) + Mx::ex((v + cc<R>(89.1)) / cc<R>(6.086)));
+  const R thsp = cc<R>(3) * ths;
+  const R tjp = cc<R>(1.46) * tj;
+  const R mLss = Mx::rcp(cc<R>(1) + Mx::ex(-(v + cc<R>(42.85)) / cc<R>(5.264)));
+  const R tmL = tm;
+  const R iss = Mx::rcp(cc<R>(1) + Mx::ex((v + cc<R>(43.94)) / cc<R>(5.711)));
+  const R tiF = cc<R>(4.562) + Mx::rcp(cc<R>(0.3933) * Mx::ex(-(v + cc<R>(100)) / cc<R>(100)) + cc<R>(0.08004) * Mx::ex((v + cc<R>(50)) / cc<R>(16.59)));
+  const R tiS = cc<R>(23.62) + Mx::rcp(cc<R>(0.001416) * Mx::ex(-(v + cc<R>(96.52)) / cc<R>(59.05)) + cc<R>(1.78e-08) * Mx::ex((v + cc<R>(114.1)) / cc<R>(8.079)));
+  const R dti_develop = cc<R>(1.354) + Mx::dv(cc<R>(0.0001), Mx::ex((v - cc<R>(167.4)) / cc<R>(15.89)) + Mx::ex(-(v - cc<R>(12.23)) / cc<R>(0.2154)));
+  const R dti_recover = cc<R>(1) - Mx::dv(cc<R>(0.5), cc<R>(1) + Mx::ex((v + cc<R>(70)) / cc<R>(20)));
+  const R tiFp = dti_develop * dti_recover * tiF;
+  const R tiSp = dti_develop * dti_recover * tiS;
+  auto gate = [&](int i, R inf, R tau) {
+    g.inf[i - G0] = (double)inf;
+    g.tau[i - G0] = (double)tau;
+  };
+  gate(10, mss, tm); gate(11, hss, thf); gate(12, hss, ths); gate(13, jss, tj); gate(14, hssp, thsp);
+  gate(15, jss, tjp); gate(16, mLss, tmL); gate(17, hLss, cc<R>(thL)); gate(18, hLssp, cc<R>(thLp));
+  gate(19, ass, ta); gate(20, iss, tiF); gate(21, iss, tiS); gate(22, assp, ta); gate(23, iss, tiFp);
+  gate(24, iss, tiSp);
+}
+
+template <typename R, typename PA, typename YV>
+CEP_D void gate_rates_ca(const YV &y, const PA &P, Gates &g)
+{
+  using Mx = M<R>;
+  using namespace ord;
+  const R v = (R)y[0];
+  const R cass = (R)y[2];
   const R dss = Mx::rcp(cc<R>(1) + Mx::ex(-(v + cc<R>(3.94)) / cc<R>(4.23)));
   const R td = cc<R>(0.6) + Mx::rcp(Mx::ex(cc<R>(-0.05) * (v + cc<R>(6))) + Mx::ex(cc<R>(0.09) * (v + cc<R>(14))));
   const R fss = Mx::rcp(cc<R>(1) + Mx::ex((v + cc<R>(19.58)) / cc<R>(3.696)));
@@ -104,36 +136,17 @@ CEP_D void gate_rates(const YV &y, const PA &P, Gates &g)
   const R txs1 = cc<R>(817.3) + Mx::rcp(cc<R>(0.0002326) * Mx::ex((v + cc<R>(48.28)) / cc<R>(17.8)) + cc<R>(0.001292) * Mx::ex(-(v + cc<R>(210)) / cc<R>(230)));
   const R xk1ss = Mx::rcp(cc<R>(1) + Mx::ex(-(v + cc<R>(2.5538 * ko + 144.59)) / cc<R>(1.5692 * ko + 3.8115)));
   const R txk1 = Mx::dv(cc<R>(122.2), Mx::ex(-(v + cc<R>(127.2)) / cc<R>(20.36)) + Mx::ex((v + cc<R>(236.8)) / cc<R>(69.33)));
-  const R jss = hss;
-  const R tj = cc<R>(2.038) + Mx::rcp(cc<R>(0.02136) * Mx::ex(-(v + cc<R>(100.6)) / cc<R>(8.281)) + cc<R>(0.3052) * Mx::ex((v + cc<R>(0.9941)) / cc<R>(38.45)));
-  const R assp = Mx::rcp(cc<R>(1) + Mx::ex(-(v - cc<R>(24.34)) / cc<R>(14.82)));
   const R tfcaf = cc<R>(7) + Mx::rcp(cc<R>(0.04) * Mx::ex(-(v - cc<R>(4)) / cc<R>(7)) + cc<R>(0.04) * Mx::ex((v - cc<R>(4)) / cc<R>(7)));
   const R tfcas = cc<R>(100) + Mx::rcp(cc<R>(0.00012) * Mx::ex(-v / cc<R>(3)) + cc<R>(0.00012) * Mx::ex(v / cc<R>(7)));
   const R tffp = cc<R>(2.5) * tff;
   const R xs2ss = xs1ss;
   const R txs2 = Mx::rcp(cc<R>(0.01) * Mx::ex((v - cc<R>(50)) / cc<R>(20)) + cc<R>(0.0193) * Mx::ex(-(v + cc<R>(66.54)) / cc<R>(31)));
-  const R hssp = Mx::rcp(cc<R>(1) + Mx::ex((v + cc<R>(89.1)) / cc<R>(6.086)));
-  const R thsp = cc<R>(3) * ths;
-  const R tjp = cc<R>(1.46) * tj;
-  const R mLss = Mx::rcp(cc<R>(1) + Mx::ex(-(v + cc<R>(42.85)) / cc<R>(5.264)));
-  const R tmL = tm;
   const R tfcafp = cc<R>(2.5) * tfcaf;
-  const R iss = Mx::rcp(cc<R>(1) + Mx::ex((v + cc<R>(43.94)) / cc<R>(5.711)));
-  const R tiF = cc<R>(4.562) + Mx::rcp(cc<R>(0.3933) * Mx::ex(-(v + cc<R>(100)) / cc<R>(100)) + cc<R>(0.08004) * Mx::ex((v + cc<R>(50)) / cc<R>(16.59)));
-  const R tiS = cc<R>(23.62) + Mx::rcp(cc<R>(0.001416) * Mx::ex(-(v + cc<R>(96.52)) / cc<R>(59.05)) + cc<R>(1.78e-08) * Mx::ex((v + cc<R>(114.1)) / cc<R>(8.079)));
-  const R dti_develop = cc<R>(1.354) + Mx::dv(cc<R>(0.0001), Mx::ex((v - cc<R>(167.4)) / cc<R>(15.89)) + Mx::ex(-(v - cc<R>(12.23)) / cc<R>(0.2154)));
-  const R dti_recover = cc<R>(1) - Mx::dv(cc<R>(0.5), cc<R>(1) + Mx::ex((v + cc<R>(70)) / cc<R>(20)));
-  const R tiFp = dti_develop * dti_recover * tiF;
-  const R tiSp = dti_develop * dti_recover * tiS;
-
   auto gate = [&](int i, R inf, R tau) {
     g.inf[i - G0] = (double)inf;
     g.tau[i - G0] = (double)tau;
   };
-  gate(10, mss, tm); gate(11, hss, thf); gate(12, hss, ths); gate(13, jss, tj); gate(14, hssp, thsp);
-  gate(15, jss, tjp); gate(16, mLss, tmL); gate(17, hLss, cc<R>(thL)); gate(18, hLssp, cc<R>(thLp));
-  gate(19, ass, ta); gate(20, iss, tiF); gate(21, iss, tiS); gate(22, assp, ta); gate(23, iss, tiFp);
-  gate(24, iss, tiSp); gate(25, dss, td); gate(26, fss, tff); gate(27, fss, tfs); gate(28, fcass, tfcaf);
+  gate(25, dss, td); gate(26, fss, tff); gate(27, fss, tfs); gate(28, fcass, tfcaf);
   gate(29, fcass, tfcas); gate(30, fcass, (R)P(P_TJCA)); gate(31, fss, tffp); gate(32, fcass, tfcafp);
   gate(34, xrss, txrf); gate(35, xrss, txrs); gate(36, xs1ss, txs1); gate(37, xs2ss, txs2);
   gate(38, xk1ss, txk1);
@@ -143,26 +156,88 @@ CEP_D void gate_rates(const YV &y, const PA &P, Gates &g)
   g.tau[33 - G0] = (double)km2n;     // a rate, not a time constant (see rl_step)
 }
 
-// The rest of computeRates (:802-989): stimulus, currents, fluxes, the Jrel relaxations (39, 40:
-// they depend on ICaL) and the rates f[0..9] of V, CaMKt and the concentrations.
-// tissue_stim < 0: the single-cell periodic stimulus; 0 / 1: stimulus off / on (the TISSUE build).
 template <typename R, typename PA, typename YV>
-CEP_D void current_rates(double t, const YV &y, double *f, const PA &P, Currents &cur, Gates &g, int tissue_stim = -1)
+CEP_D void gate_rates(const YV &y, const PA &P, Gates &g)
+{
+  gate_rates_na<R>(y, P, g);
+  gate_rates_ca<R>(y, P, g);
+}
+
+// The currents of computeRates in two groups that share no intermediate result but CaMK's
+// phosphorylated fraction and the GHK exponential (each group computes them): the Na and K
+// currents (nak_currents) and the Ca currents, pumps, exchangers and fluxes (ca_currents).
+// They hold the doubles the rates are formed from.
+struct NaKCurrents {
+  double INa, INaL, Ito, IKr, IKs, IK1, IKb, INab;
+};
+struct CaCurrents {
+  double ICaL, ICaNa, ICaK, INaCa_i, INaCa_ss, INaK, IpCa, ICab;
+  double JdiffK, JdiffNa, Jdiff, Jrel, Jup, Jtr, Bcass, Bcai, Bcajsr, f1;
+  double Jrel_inf, tau_rel, Jrel_infp, tau_relp;
+};
+
+// INa, INaL, Ito, IKr, IKs, IK1, IKb, INab (:802-989)
+template <typename R, typename PA, typename YV>
+CEP_D void nak_currents(const YV &y, const PA &P, NaKCurrents &o)
 {
   using Mx = M<R>;
   using namespace ord;   // shared physical constants (same values in both models)
   const R v = (R)y[0];
   const R camkt = (R)y[1], cass = (R)y[2], nai = (R)y[3], nass = (R)y[4], ki = (R)y[5];
   const R kss = (R)y[6], cansr = (R)y[7], cajsr = (R)y[8], cai = (R)y[9];
+  (void)nass; (void)kss; (void)cansr; (void)cajsr;
+  const R CaMKb = Mx::dv(cc<R>(CaMKo) * (cc<R>(1) - camkt), cc<R>(1) + Mx::dv(cc<R>(KmCaM), cass));
+  const R vfrt = (v * cc<R>(Far)) / cc<R>(Rgas * Temp);
+  const R e1 = Mx::ex(vfrt);
+  const R CaMKa = CaMKb + camkt;
+  const R fCaMKp = Mx::rcp(cc<R>(1) + Mx::dv(cc<R>(KmCaMK), CaMKa));
 
-  double Istim;
-  if (tissue_stim >= 0) {
-    Istim = tissue_stim ? P(P_STIM_AMP) : 0.0;
-  } else {
-    const double ts = t - stim_start, bcl = P(P_BCL);
-    Istim = (t >= stim_start && ts - floor(ts / bcl) * bcl <= stim_dur) ? P(P_STIM_AMP) : 0.0;
-  }
-  cur.Istim = Istim;
+  // ---- K currents
+  const R EK = cc<R>(RTF) * Mx::lg(Mx::dv(cc<R>(ko), ki));
+  const R AiF = Mx::rcp(cc<R>(1) + Mx::ex((v - cc<R>(213.6)) / cc<R>(151.2)));
+  const R AiS = cc<R>(1) - AiF;
+  const R a = (R)y[19], iF = (R)y[20], iS = (R)y[21], ap = (R)y[22], iFp = (R)y[23], iSp = (R)y[24];
+  const R ii = AiF * iF + AiS * iS;
+  const R ip = AiF * iFp + AiS * iSp;
+  const R Ito = (R)P(P_GTO) * (v - EK) * ((cc<R>(1) - fCaMKp) * a * ii + fCaMKp * ap * ip);
+  const R Axrf = Mx::rcp(cc<R>(1) + Mx::ex((v + cc<R>(54.81)) / cc<R>(38.21)));
+  const R Axrs = cc<R>(1) - Axrf;
+  const R xr = Axrf * (R)y[34] + Axrs * (R)y[35];
+  const R rkr = Mx::rcp(cc<R>(1) + Mx::ex((v + cc<R>(55)) / cc<R>(75))) * Mx::rcp(cc<R>(1) + Mx::ex((v - cc<R>(10)) / cc<R>(30)));
+  const R IKr = (R)P(P_GKR) * cc<R>(1.0) /* sqrt(ko / 5.4) */ * xr * rkr * (v - EK);
+  const R EKs = cc<R>(RTF) * Mx::lg(Mx::dv(cc<R>(ko + PKNa * nao), ki + cc<R>(PKNa) * nai));
+  const R KsCa = cc<R>(1) + Mx::dv(cc<R>(0.6), cc<R>(1) + Mx::pw(Mx::dv(cc<R>(3.8e-05), cai), cc<R>(1.4)));
+  const R xs1 = (R)y[36], xs2 = (R)y[37], xk1 = (R)y[38];
+  const R IKs = (R)P(P_GKS) * KsCa * xs1 * xs2 * (v - EKs);
+  const R rk1 = Mx::rcp(cc<R>(1) + Mx::ex(((v + cc<R>(105.8)) - cc<R>(2.6 * ko)) / cc<R>(9.493)));
+  const R IK1 = (R)P(P_GK1) * cc<R>(2.3237900077244502) /* sqrt(ko) */ * rk1 * xk1 * (v - EK);
+
+  const R IKb = cc<R>(GKb) * Mx::rcp(cc<R>(1) + Mx::ex(-(v - cc<R>(14.48)) / cc<R>(18.34))) * (v - EK);
+
+  // ---- Na currents
+  const R ENa = cc<R>(RTF) * Mx::lg(Mx::dv(cc<R>(nao), nai));
+  const R m = (R)y[10], hf = (R)y[11], hs = (R)y[12], jj = (R)y[13], hsp = (R)y[14], jp = (R)y[15];
+  const R mL = (R)y[16], hL = (R)y[17], hLp = (R)y[18];
+  const R hh = cc<R>(Ahf) * hf + cc<R>(Ahs) * hs;
+  const R hp = cc<R>(Ahf) * hf + cc<R>(Ahs) * hsp;
+  const R INa = (R)P(P_GNA) * (v - ENa) * (m * m * m) * ((cc<R>(1) - fCaMKp) * hh * jj + fCaMKp * hp * jp);
+  const R INaL = (R)P(P_GNAL) * (v - ENa) * mL * ((cc<R>(1) - fCaMKp) * hL + fCaMKp * hLp);
+
+  const R INab = ghk11<R>(cc<R>(PNab * Far) * (nai * e1 - cc<R>(nao)), vfrt, e1);
+  o.INa = (double)INa; o.INaL = (double)INaL; o.Ito = (double)Ito; o.IKr = (double)IKr;
+  o.IKs = (double)IKs; o.IK1 = (double)IK1; o.IKb = (double)IKb; o.INab = (double)INab;
+}
+
+// ICaL, ICaNa, ICaK, INaK, INaCa (both compartments), IpCa, ICab, the fluxes, buffers, the
+// Jrel relaxations and dCaMKt/dt (:802-989)
+template <typename R, typename PA, typename YV>
+CEP_D void ca_currents(const YV &y, const PA &P, CaCurrents &o)
+{
+  using Mx = M<R>;
+  using namespace ord;   // shared physical constants (same values in both models)
+  const R v = (R)y[0];
+  const R camkt = (R)y[1], cass = (R)y[2], nai = (R)y[3], nass = (R)y[4], ki = (R)y[5];
+  const R kss = (R)y[6], cansr = (R)y[7], cajsr = (R)y[8], cai = (R)y[9];
   const R jca = (R)y[30];
   const R CaMKb = Mx::dv(cc<R>(CaMKo) * (cc<R>(1) - camkt), cc<R>(1) + Mx::dv(cc<R>(KmCaM), cass));
 
@@ -195,26 +270,6 @@ CEP_D void current_rates(double t, const YV &y, double *f, const PA &P, Currents
   const R tau_relp_t = Mx::dv(cc<R>(btp), cc<R>(1) + Mx::dv(cc<R>(0.0123), cajsr));
   const R tau_relp = tau_relp_t < cc<R>(0.001) ? cc<R>(0.001) : tau_relp_t;
 
-  // ---- K currents
-  const R EK = cc<R>(RTF) * Mx::lg(Mx::dv(cc<R>(ko), ki));
-  const R AiF = Mx::rcp(cc<R>(1) + Mx::ex((v - cc<R>(213.6)) / cc<R>(151.2)));
-  const R AiS = cc<R>(1) - AiF;
-  const R a = (R)y[19], iF = (R)y[20], iS = (R)y[21], ap = (R)y[22], iFp = (R)y[23], iSp = (R)y[24];
-  const R ii = AiF * iF + AiS * iS;
-  const R ip = AiF * iFp + AiS * iSp;
-  const R Ito = (R)P(P_GTO) * (v - EK) * ((cc<R>(1) - fCaMKp) * a * ii + fCaMKp * ap * ip);
-  const R Axrf = Mx::rcp(cc<R>(1) + Mx::ex((v + cc<R>(54.81)) / cc<R>(38.21)));
-  const R Axrs = cc<R>(1) - Axrf;
-  const R xr = Axrf * (R)y[34] + Axrs * (R)y[35];
-  const R rkr = Mx::rcp(cc<R>(1) + Mx::ex((v + cc<R>(55)) / cc<R>(75))) * Mx::rcp(cc<R>(1) + Mx::ex((v - cc<R>(10)) / cc<R>(30)));
-  const R IKr = (R)P(P_GKR) * cc<R>(1.0) /* sqrt(ko / 5.4) */ * xr * rkr * (v - EK);
-  const R EKs = cc<R>(RTF) * Mx::lg(Mx::dv(cc<R>(ko + PKNa * nao), ki + cc<R>(PKNa) * nai));
-  const R KsCa = cc<R>(1) + Mx::dv(cc<R>(0.6), cc<R>(1) + Mx::pw(Mx::dv(cc<R>(3.8e-05), cai), cc<R>(1.4)));
-  const R xs1 = (R)y[36], xs2 = (R)y[37], xk1 = (R)y[38];
-  const R IKs = (R)P(P_GKS) * KsCa * xs1 * xs2 * (v - EKs);
-  const R rk1 = Mx::rcp(cc<R>(1) + Mx::ex(((v + cc<R>(105.8)) - cc<R>(2.6 * ko)) / cc<R>(9.493)));
-  const R IK1 = (R)P(P_GK1) * cc<R>(2.3237900077244502) /* sqrt(ko) */ * rk1 * xk1 * (v - EK);
-
   // ---- INaK (H scaled by the Dutta quirk enters P and b3)
   const R Hh = (R)P(P_H);
   const R Knao = cc<R>(Knao0) * Mx::ex((cc<R>(1.0 - delta) * v * cc<R>(Far)) / cc<R>(3.0 * Rgas * Temp));
@@ -243,19 +298,10 @@ CEP_D void current_rates(double t, const YV &y, double *f, const PA &P, Currents
   const R JnakNa = cc<R>(3) * (Mx::dv(x1, xsum) * a3 - Mx::dv(x2, xsum) * b3);
   const R JnakK = cc<R>(2) * (Mx::dv(x4, xsum) * B1 - Mx::dv(x3, xsum) * a1);
   const R INaK = cc<R>(Pnak) * (JnakNa + JnakK);
-  const R IKb = cc<R>(GKb) * Mx::rcp(cc<R>(1) + Mx::ex(-(v - cc<R>(14.48)) / cc<R>(18.34))) * (v - EK);
   const R JdiffK = (kss - ki) / cc<R>(2);
   const R PhiCaK = ghk11<R>(cc<R>(Far) * (cc<R>(0.75) * kss * e1 - cc<R>(0.75 * ko)), vfrt, e1);
   const R ICaK = (cc<R>(1) - fCaMKp) * cc<R>(PCaK) * PhiCaK * gnp + fCaMKp * cc<R>(PCaKp) * PhiCaK * gp;
 
-  // ---- Na currents
-  const R ENa = cc<R>(RTF) * Mx::lg(Mx::dv(cc<R>(nao), nai));
-  const R m = (R)y[10], hf = (R)y[11], hs = (R)y[12], jj = (R)y[13], hsp = (R)y[14], jp = (R)y[15];
-  const R mL = (R)y[16], hL = (R)y[17], hLp = (R)y[18];
-  const R hh = cc<R>(Ahf) * hf + cc<R>(Ahs) * hs;
-  const R hp = cc<R>(Ahf) * hf + cc<R>(Ahs) * hsp;
-  const R INa = (R)P(P_GNA) * (v - ENa) * (m * m * m) * ((cc<R>(1) - fCaMKp) * hh * jj + fCaMKp * hp * jp);
-  const R INaL = (R)P(P_GNAL) * (v - ENa) * mL * ((cc<R>(1) - fCaMKp) * hL + fCaMKp * hLp);
 
   // ---- Na/Ca exchange
   const R hna = Mx::ex((cc<R>(qna) * v * cc<R>(Far)) / cc<R>(Rgas * Temp));
@@ -289,7 +335,6 @@ CEP_D void current_rates(double t, const YV &y, double *f, const PA &P, Currents
   };
   const R INaCa_i = ncx(nai, cai, cc<R>(0.8));
   const R INaCa_ss = ncx(nass, cass, cc<R>(0.2));
-  const R INab = ghk11<R>(cc<R>(PNab * Far) * (nai * e1 - cc<R>(nao)), vfrt, e1);
   const R JdiffNa = (nass - nai) / cc<R>(2);
   const R PhiCaNa = ghk11<R>(cc<R>(Far) * (cc<R>(0.75) * nass * e1 - cc<R>(0.75 * nao)), vfrt, e1);
   const R ICaNa = (cc<R>(1) - fCaMKp) * cc<R>(PCaNa) * PhiCaNa * gnp + fCaMKp * cc<R>(PCaNap) * PhiCaNa * gp;
@@ -308,33 +353,72 @@ CEP_D void current_rates(double t, const YV &y, double *f, const PA &P, Currents
   const R Jtr = (cansr - cajsr) / cc<R>(100);
   const R Bcajsr = Mx::rcp(cc<R>(1) + Mx::dv(cc<R>(csqnmax * kmcsqn), (cc<R>(kmcsqn) + cajsr) * (cc<R>(kmcsqn) + cajsr)));
 
-  g.inf[39 - G0] = (double)Jrel_inf;
-  g.tau[39 - G0] = (double)tau_rel;
-  f[39] = ((double)Jrel_inf - y[39]) / (double)tau_rel;
-  g.inf[40 - G0] = (double)Jrel_infp;
-  g.tau[40 - G0] = (double)tau_relp;
-  f[40] = ((double)Jrel_infp - y[40]) / (double)tau_relp;
+  o.ICaL = (double)ICaL; o.ICaNa = (double)ICaNa; o.ICaK = (double)ICaK; o.INaCa_i = (double)INaCa_i;
+  o.INaCa_ss = (double)INaCa_ss; o.INaK = (double)INaK; o.IpCa = (double)IpCa; o.ICab = (double)ICab;
+  o.JdiffK = (double)JdiffK; o.JdiffNa = (double)JdiffNa; o.Jdiff = (double)Jdiff; o.Jrel = (double)Jrel;
+  o.Jup = (double)Jup; o.Jtr = (double)Jtr; o.Bcass = (double)Bcass; o.Bcai = (double)Bcai;
+  o.Bcajsr = (double)Bcajsr;
+  o.f1 = (double)(cc<R>(aCaMK) * CaMKb * (CaMKb + camkt) - cc<R>(bCaMK) * camkt);
+  o.Jrel_inf = (double)Jrel_inf; o.tau_rel = (double)tau_rel;
+  o.Jrel_infp = (double)Jrel_infp; o.tau_relp = (double)tau_relp;
+}
+
+// The rates f[0..9] of V, CaMKt and the concentrations, and the Jrel relaxations (39, 40),
+// from the two current groups.  tissue_stim < 0: the single-cell periodic stimulus; 0 / 1:
+// stimulus off / on (the TISSUE build).
+template <typename R, typename PA, typename YV>
+CEP_D void rates_from_currents(double t, const YV &y, double *f, const PA &P, Currents &cur, Gates &g,
+                               const NaKCurrents &N, const CaCurrents &C, int tissue_stim = -1)
+{
+  using namespace ord;
+  double Istim;
+  if (tissue_stim >= 0) {
+    Istim = tissue_stim ? P(P_STIM_AMP) : 0.0;
+  } else {
+    const double ts = t - stim_start, bcl = P(P_BCL);
+    Istim = (t >= stim_start && ts - floor(ts / bcl) * bcl <= stim_dur) ? P(P_STIM_AMP) : 0.0;
+  }
+  cur.Istim = Istim;
+  g.inf[39 - G0] = C.Jrel_inf;
+  g.tau[39 - G0] = C.tau_rel;
+  f[39] = (C.Jrel_inf - y[39]) / C.tau_rel;
+  g.inf[40 - G0] = C.Jrel_infp;
+  g.tau[40 - G0] = C.tau_relp;
+  f[40] = (C.Jrel_infp - y[40]) / C.tau_relp;
 
   // ---- CaMK, concentrations, voltage (in double from the R-precision currents)
-  f[1] = (double)(cc<R>(aCaMK) * CaMKb * (CaMKb + camkt) - cc<R>(bCaMK) * camkt);
+  f[1] = C.f1;
   const double dIstim = Istim;
-  f[5] = (-(((double)Ito + (double)IKr + (double)IKs + (double)IK1 + (double)IKb + dIstim) - 2.0 * (double)INaK) * cm * Acap) / (Far * vmyo)
-         + ((double)JdiffK * vss) / vmyo;
-  f[6] = (-(double)ICaK * cm * Acap) / (Far * vss) - (double)JdiffK;
-  f[3] = (-((double)INa + (double)INaL + 3.0 * (double)INaCa_i + 3.0 * (double)INaK + (double)INab) * Acap * cm) / (Far * vmyo)
-         + ((double)JdiffNa * vss) / vmyo;
-  f[4] = (-((double)ICaNa + 3.0 * (double)INaCa_ss) * cm * Acap) / (Far * vss) - (double)JdiffNa;
-  f[0] = -((double)INa + (double)INaL + (double)Ito + (double)ICaL + (double)ICaNa + (double)ICaK + (double)IKr + (double)IKs
-           + (double)IK1 + (double)INaCa_i + (double)INaCa_ss + (double)INaK + (double)INab + (double)IKb + (double)IpCa
-           + (double)ICab + dIstim);
-  f[2] = (double)Bcass * (((-((double)ICaL - 2.0 * (double)INaCa_ss) * cm * Acap) / (2.0 * Far * vss)
-                           + ((double)Jrel * vjsr) / vss) - (double)Jdiff);
-  f[9] = (double)Bcai * (((-(((double)IpCa + (double)ICab) - 2.0 * (double)INaCa_i) * cm * Acap) / (2.0 * Far * vmyo)
-                          - ((double)Jup * vnsr) / vmyo) + ((double)Jdiff * vss) / vmyo);
-  f[7] = (double)Jup - ((double)Jtr * vjsr) / vnsr;
-  f[8] = (double)Bcajsr * ((double)Jtr - (double)Jrel);
+  f[5] = (-((N.Ito + N.IKr + N.IKs + N.IK1 + N.IKb + dIstim) - 2.0 * C.INaK) * cm * Acap) / (Far * vmyo)
+         + (C.JdiffK * vss) / vmyo;
+  f[6] = (-C.ICaK * cm * Acap) / (Far * vss) - C.JdiffK;
+  f[3] = (-(N.INa + N.INaL + 3.0 * C.INaCa_i + 3.0 * C.INaK + N.INab) * Acap * cm) / (Far * vmyo)
+         + (C.JdiffNa * vss) / vmyo;
+  f[4] = (-(C.ICaNa + 3.0 * C.INaCa_ss) * cm * Acap) / (Far * vss) - C.JdiffNa;
+  f[0] = -(N.INa + N.INaL + N.Ito + C.ICaL + C.ICaNa + C.ICaK + N.IKr + N.IKs
+           + N.IK1 + C.INaCa_i + C.INaCa_ss + C.INaK + N.INab + N.IKb + C.IpCa
+           + C.ICab + dIstim);
+  f[2] = C.Bcass * (((-(C.ICaL - 2.0 * C.INaCa_ss) * cm * Acap) / (2.0 * Far * vss)
+                           + (C.Jrel * vjsr) / vss) - C.Jdiff);
+  f[9] = C.Bcai * (((-((C.IpCa + C.ICab) - 2.0 * C.INaCa_i) * cm * Acap) / (2.0 * Far * vmyo)
+                          - (C.Jup * vnsr) / vmyo) + (C.Jdiff * vss) / vmyo);
+  f[7] = C.Jup - (C.Jtr * vjsr) / vnsr;
+  f[8] = C.Bcajsr * (C.Jtr - C.Jrel);
 
-  cur.INa = INa; cur.INaL = INaL; cur.Ito = Ito; cur.ICaL = ICaL; cur.IKr = IKr; cur.IKs = IKs; cur.IK1 = IK1;
+  cur.INa = N.INa; cur.INaL = N.INaL; cur.Ito = N.Ito; cur.ICaL = C.ICaL; cur.IKr = N.IKr; cur.IKs = N.IKs;
+  cur.IK1 = N.IK1;
+}
+
+// The rest of computeRates (:802-989): stimulus, currents, fluxes, the Jrel relaxations (39, 40:
+// they depend on ICaL) and the rates f[0..9] of V, CaMKt and the concentrations.
+template <typename R, typename PA, typename YV>
+CEP_D void current_rates(double t, const YV &y, double *f, const PA &P, Currents &cur, Gates &g, int tissue_stim = -1)
+{
+  NaKCurrents N;
+  CaCurrents C;
+  nak_currents<R>(y, P, N);
+  ca_currents<R>(y, P, C);
+  rates_from_currents<R>(t, y, f, P, cur, g, N, C, tissue_stim);
 }
 
 // computeRates.  y, f: NS doubles.  g: steady states / time constants of the gates for the
@@ -356,19 +440,27 @@ CEP_D void rhs(double t, const YV &y, double *f, const PA &P, Currents &cur, Gat
 // two Jrel states and forward Euler for V, CaMKt and the concentrations.  rl_step is both,
 // with the rates and relaxations of the last rhs() call.
 template <typename R, typename YV>
-CEP_D void rl_gates(double dt, YV &y, const Gates &g)
+CEP_D void rl_gates_range(double dt, YV &y, const Gates &g, int lo, int hi)
 {
   using Mx = M<R>;
 #pragma unroll
   for (int i = G0; i <= 38; ++i) {
-    if (i == 33) continue;
+    if (i < lo || i > hi || i == 33) continue;
     const double inf = g.inf[i - G0];
     // fcafp: exp(-25 / tfcafp), the reference's use of the d gate's index in place of dt
     const double arg = (i == 32) ? -kFcafpNumerator / g.tau[i - G0] : -dt / g.tau[i - G0];
     y[i] = inf - (inf - y[i]) * (double)Mx::ex((R)arg);
   }
-  y[33] = g.inf[33 - G0] - (g.inf[33 - G0] - y[33]) * (double)Mx::ex((R)(-g.tau[33 - G0] * dt));
+  if (lo <= 33 && 33 <= hi)
+    y[33] = g.inf[33 - G0] - (g.inf[33 - G0] - y[33]) * (double)Mx::ex((R)(-g.tau[33 - G0] * dt));
 }
+// the gates of gate_rates_na (10-24) and of gate_rates_ca (25-38)
+template <typename R, typename YV>
+CEP_D void rl_gates_na(double dt, YV &y, const Gates &g) { rl_gates_range<R>(dt, y, g, G0, 24); }
+template <typename R, typename YV>
+CEP_D void rl_gates_ca(double dt, YV &y, const Gates &g) { rl_gates_range<R>(dt, y, g, 25, 38); }
+template <typename R, typename YV>
+CEP_D void rl_gates(double dt, YV &y, const Gates &g) { rl_gates_range<R>(dt, y, g, G0, 38); }
 
 template <typename R>
 CEP_D void rl_core(double dt, double *y, const double *f, const Gates &g)

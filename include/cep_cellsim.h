@@ -138,9 +138,11 @@ int cep_set_cells(cep_batch *b, int first, int count, const double *celltype, co
    t, Vm, Ca_i and INaL ICaL Ito IKr IKs IK1 at every dt_write (1 ms) point of the beat. */
 #define CEP_OPT_TIME_SERIES 4
 /* CEP_OPT_ORD11_WAVES (ORd 2011, Rush-Larsen): wavefronts per 64 cells.  0 (default) chooses:
-   two (gate kinetics on one, currents and features on the other, on two SIMDs) while twice the
-   wave count fits one wave per SIMD, else one; 1 or 2 forces it.  Results are the same either
-   way up to floating-point contraction.  No reference counterpart (a launch-shape choice). */
+   four (Na / Ito gates, Ca / K gates, Na and K currents, Ca currents with the rates and
+   features, on four SIMDs) while four times the wave count fits one wave per SIMD, else two
+   (gate kinetics / currents) while that fits, else one; 1, 2 or 4 forces it.  Results are the
+   same either way up to floating-point contraction.  No reference counterpart (a launch-shape
+   choice). */
 #define CEP_OPT_ORD11_WAVES 5
 int cep_set_model_option(cep_batch *b, int option, double value);
 

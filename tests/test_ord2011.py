@@ -240,11 +240,13 @@ def test_ord11_ead_flags_match_oracle_over_300_beats():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("waves", [2, 4])
 @pytest.mark.parametrize("prec", [cep.PREC_FP64, cep.PREC_MIXED])
-def test_ord11_two_wave_kernel_matches_one_wave(prec):
-    """The step split over two wavefronts (gate kinetics / currents, exchanged through LDS)
-    against the one-wave kernel on the same 70 cells (two workgroups, the second one partly
-    empty), over 3 beats in two launches.  The arithmetic per element is the same; the compiler
+def test_ord11_split_kernels_match_one_wave(prec, waves):
+    """The step split over two wavefronts (gate kinetics / currents) or four (Na / Ito gates,
+    Ca / K gates, Na and K currents, Ca currents and the rest), exchanged through LDS, against
+    the one-wave kernel on the same 70 cells (two workgroups, the second one partly empty),
+    over 3 beats in two launches.  The arithmetic per element is the same; the compiler
     may contract a multiply-add differently in the two kernels, which the adaptive step rule
     can turn into a different step now and then, so: step counts within 0.5 %, states and
     features within 1e-9 (FP64; measured identical) or 1e-4 (FP32 rates; measured 2e-6)."""
@@ -254,20 +256,20 @@ def test_ord11_two_wave_kernel_matches_one_wave(prec):
     conc = np.array([k[1] for k in keys])
     ic50 = np.vstack([QUINIDINE[k[0]] for k in keys])
     out = {}
-    for waves in (1, 2):
+    for w in (1, waves):
         with _gpu_batch(n, prec, conc, ic50, np.full(n, 2000.0), 3, 3) as b:
-            b.set_model_option(cep.OPT_ORD11_WAVES, waves)
+            b.set_model_option(cep.OPT_ORD11_WAVES, w)
             b.set_states(np.repeat(STEADY[None, :], n, axis=0))
             b.run_beats(1)
             b.run_beats(2)
-            out[waves] = (b.states(), b.biomarker_array(), b.status(), b.stats()[:, 0].copy())
+            out[w] = (b.states(), b.biomarker_array(), b.status(), b.stats()[:, 0].copy())
     s1, r1, st1, n1 = out[1]
-    s2, r2, st2, n2 = out[2]
+    s2, r2, st2, n2 = out[waves]
     assert (st1 == 0).all() and (st2 == 0).all()
     assert np.all(np.abs(n2 - n1) <= 0.005 * n1)
     tol = 1e-9 if prec == cep.PREC_FP64 else 1e-4
     np.testing.assert_allclose(s2, s1, rtol=tol, atol=1e-9)
     np.testing.assert_allclose(r2, r1, rtol=tol, atol=1e-6)
     same = np.mean(n1 == n2)
-    print(f"two-wave vs one-wave: {same:.0%} of cells with identical step counts, "
+    print(f"{waves}-wave vs one-wave: {same:.0%} of cells with identical step counts, "
           f"max |state diff| {np.abs(s2 - s1).max():.2e}")
