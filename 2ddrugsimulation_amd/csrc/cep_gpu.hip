@@ -570,11 +570,12 @@ struct TtBuffers {
   int *bmi, *iout, *status;
 };
 
-// FP32 rates: at most 256 registers, two waves per SIMD (measured on 200,000 cells: 0.60 s per
-// beat against 0.78 s with 320 registers and one wave, whose 3125 waves took four rounds of
-// the 1024 SIMDs).  FP64 keeps its registers: capped, it would spill 512 bytes per lane.
+// At most 256 registers, two waves per SIMD: 200,000 cells make 3125 waves, which with one wave
+// per SIMD took four rounds of the 1024 SIMDs.  Measured per beat: FP32 rates 0.60 s against
+// 0.78 s (320 registers, 220 bytes of spills per lane once capped), FP64 1.39 s against 1.64 s
+// (390 registers, 512 bytes of spills): the second wave hides more than the spills cost.
 template <typename R>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 2 : 1)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 tt_rl_kernel(TtBuffers B, RunCfg rc, int nbeats)
 {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
