@@ -207,3 +207,17 @@ def test_one_layer_of_hexahedra_steps_like_the_grid():
     assert np.abs(v3[:nl] - v2).max() <= 1e-6
     assert np.abs(v3[nl:] - v2).max() <= 1e-6
     assert v2[corner].max() > 0.0
+
+
+def test_3d_mesh_files_round_trip(tmp_path):
+    """Hexahedral (C3D8I) and tetrahedral (C3D4) meshes through the reference's file format
+    (mesh.cpp:93-122 element keywords): node order, element order and dimension survive."""
+    xyz, conn = T.box_hex_mesh(3, 2, 2, 0.1, 0.1, 0.1)
+    tets = T.tet_mesh_from_hex(xyz, conn)
+    for etype, c, npe in (("C3D8I", conn, 8), ("C3D4", tets, 4)):
+        p = str(tmp_path / f"{etype}.inp")
+        T.write_inp(p, xyz, c, targets=(1, 2, 3), etype=etype)
+        m = cep.Mesh.read_inp(p)
+        assert (m.nnodes, m.nelem, m.nodes_per_elem, m.dimension) == (len(xyz), len(c), npe, 3)
+        np.testing.assert_array_equal(m.nodes(), xyz)
+        np.testing.assert_array_equal(m.elements(), c)
