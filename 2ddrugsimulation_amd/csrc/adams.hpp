@@ -11,6 +11,13 @@
 #pragma once
 #include "common.hpp"
 
+// Index check of every difference column, coefficient slot and table entry.  No-op on the
+// device; a host build of this header (tests/adams_host_check.cpp) defines it to abort, and runs
+// the integrator under AddressSanitizer.
+#ifndef CEP_ADAMS_CHECK
+#define CEP_ADAMS_CHECK(cond) ((void)0)
+#endif
+
 namespace cep {
 namespace adams {
 
@@ -54,20 +61,36 @@ struct Lane {
   int ns, k, kold, isnold, start, phase1, nornd;
   long long nsteps = 0, nfe = 0;
 
-  CEP_D double &PHI(int l, int col) const { return B.phi[((size_t)col * N + l) * P + c]; }
-  CEP_D double &YY(int l) const { return B.vec[((size_t)0 * N + l) * P + c]; }
-  CEP_D double &WT(int l) const { return B.vec[((size_t)1 * N + l) * P + c]; }
-  CEP_D double &PP(int l) const { return B.vec[((size_t)2 * N + l) * P + c]; }
-  CEP_D double &YP(int l) const { return B.vec[((size_t)3 * N + l) * P + c]; }
-  CEP_D double &YPOUT(int l) const { return B.vec[((size_t)4 * N + l) * P + c]; }
-  CEP_D double &CO(int j) const { return B.coef[(size_t)j * P + c]; }
-  CEP_D double &psi(int i) const { return CO(C_PSI + i); }
-  CEP_D double &alpha(int i) const { return CO(C_ALPHA + i); }
-  CEP_D double &beta(int i) const { return CO(C_BETA + i); }
-  CEP_D double &sig(int i) const { return CO(C_SIG + i); }
-  CEP_D double &v(int i) const { return CO(C_V + i); }
-  CEP_D double &w(int i) const { return CO(C_W + i); }
-  CEP_D double &g(int i) const { return CO(C_G + i); }
+  CEP_D double &PHI(int l, int col) const
+  {
+    CEP_ADAMS_CHECK(l >= 0 && l < N && col >= 0 && col < COLS);
+    return B.phi[((size_t)col * N + l) * P + c];
+  }
+  CEP_D double &VEC(int which, int l) const
+  {
+    CEP_ADAMS_CHECK(l >= 0 && l < N);
+    return B.vec[((size_t)which * N + l) * P + c];
+  }
+  CEP_D double &YY(int l) const { return VEC(0, l); }
+  CEP_D double &WT(int l) const { return VEC(1, l); }
+  CEP_D double &PP(int l) const { return VEC(2, l); }
+  CEP_D double &YP(int l) const { return VEC(3, l); }
+  CEP_D double &YPOUT(int l) const { return VEC(4, l); }
+  // slot i of the coefficient array that starts at slot `base` and holds `len` entries
+  CEP_D double &CO(int base, int len, int i) const
+  {
+    CEP_ADAMS_CHECK(i >= 0 && i < len);
+    return B.coef[(size_t)(base + i) * P + c];
+  }
+  CEP_D double &psi(int i) const { return CO(C_PSI, C_ALPHA - C_PSI, i); }
+  CEP_D double &alpha(int i) const { return CO(C_ALPHA, C_BETA - C_ALPHA, i); }
+  CEP_D double &beta(int i) const { return CO(C_BETA, C_SIG - C_BETA, i); }
+  CEP_D double &sig(int i) const { return CO(C_SIG, C_V - C_SIG, i); }
+  CEP_D double &v(int i) const { return CO(C_V, C_W - C_V, i); }
+  CEP_D double &w(int i) const { return CO(C_W, C_G - C_W, i); }
+  CEP_D double &g(int i) const { return CO(C_G, C_N - C_G, i); }
+  CEP_D static double gstr(int i) { CEP_ADAMS_CHECK(i >= 0 && i < 13); return kGstr[i]; }
+  CEP_D static double two(int i) { CEP_ADAMS_CHECK(i >= 0 && i < 13); return kTwo[i]; }
 
   CEP_D void load()
   {
@@ -107,6 +130,7 @@ struct Lane {
     double gi[13], rho[13], wi[13];
     const double hi = xout - x;
     const int ki = kold + 1;
+    CEP_ADAMS_CHECK(ki >= 1 && ki <= 13);
     for (int i = 1; i <= ki; ++i) wi[i - 1] = 1.0 / (double)i;
     gi[0] = 1.0;
     rho[0] = 1.0;
@@ -277,10 +301,10 @@ struct Lane {
         if (0 <= km2) erkm1 = erkm1 + sq((PHI(l, k - 1) + ypl - ph0) / wl);
         erk = erk + sq((ypl - ph0) / wl);
       }
-      if (0 < km2) erkm2 = absh * sig(km1 - 1) * kGstr[km2 - 1] * sqrt(erkm2);
-      if (0 <= km2) erkm1 = absh * sig(k - 1) * kGstr[km1 - 1] * sqrt(erkm1);
+      if (0 < km2) erkm2 = absh * sig(km1 - 1) * gstr(km2 - 1) * sqrt(erkm2);
+      if (0 <= km2) erkm1 = absh * sig(k - 1) * gstr(km1 - 1) * sqrt(erkm1);
       err = absh * sqrt(erk) * (g(k - 1) - g(kp1 - 1));
-      erk = absh * sqrt(erk) * sig(kp1 - 1) * kGstr[k - 1];
+      erk = absh * sqrt(erk) * sig(kp1 - 1) * gstr(k - 1);
       knew = k;
       if (0 < km2) {
         if (dmax(erkm1, erkm2) <= erk) knew = km1;
@@ -349,7 +373,7 @@ struct Lane {
       erk = erkm1;
     } else if (kp1 <= ns) {
       for (int l = 0; l < N; ++l) erkp1 = erkp1 + sq(PHI(l, kp2 - 1) / WT(l));
-      erkp1 = absh * kGstr[kp1 - 1] * sqrt(erkp1);
+      erkp1 = absh * gstr(kp1 - 1) * sqrt(erkp1);
       if (k == 1) {
         if (erkp1 < 0.5 * erk) { k = kp1; erk = erkp1; }
       } else if (erkm1 <= dmin(erk, erkp1)) {
@@ -359,7 +383,7 @@ struct Lane {
       }
     }
     double hnew = h + h;
-    if (!phase1 && p5eps < erk * kTwo[k]) {
+    if (!phase1 && p5eps < erk * two(k)) {
       hnew = h;
       if (p5eps < erk) {
         temp2 = (double)(k + 1);

@@ -14,6 +14,7 @@
 #include "adams.hpp"
 #include "bdf_ord.hpp"
 #include "ord11_model.hpp"
+#include "ord11_host.hpp"
 #include "tt06_model.hpp"
 
 namespace cep {
@@ -1409,39 +1410,8 @@ static void tt_constants(const TtCellIn &in, double stim_scale, double *C)
   C[6] = p[TP_BCL]; C[8] = p[TP_STIM_AMP];
 }
 
-// ---- ORd 2011 (DrugSimulation/cellmodels/Ohara_Rudy_2011.cpp)
-struct O11CellIn {
-  double celltype, conc, bcl;
-  double ic50[14];
-};
-
-// initial STATES of ___initConsts (:494-679)
-static const double kO11Init[41] = {
-  -87, 0, 1e-4, 7, 7, 145, 145, 1.2, 1.2, 1e-4, 0, 1, 1, 1, 1, 1, 0, 1, 1, 0, 1, 1, 0, 1, 1,
-  0, 1, 1, 1, 1, 1, 1, 1, 0, 0, 0, 0, 0, 1, 0, 0 };
-
-// per-cell parameters: endocardial constants, the Dutta scaling of the reference (slots 96,
-// 98, 100, 81 of CONSTANTS = GNaL, Aff, tjca, H), then the Hill block of ___applyDrugEffect
-static void o11_params(const O11CellIn &in, bool dutta, double stim_scale, double *p)
-{
-  using namespace ord11;
-  auto block = [&](int k) {
-    const double ic = in.ic50[2 * k], h = in.ic50[2 * k + 1];
-    return (ic > 10E-14 && h > 10E-14) ? 1. / (1. + pow(in.conc / ic, h)) : 1.;
-  };
-  p[P_PCA] = PCa_b * block(0);
-  p[P_GK1] = GK1_b * block(1);
-  p[P_GKS] = GKs_b * block(2);
-  p[P_GNA] = GNa * block(3);
-  p[P_GNAL] = (dutta ? GNaL_b * 1.013 : GNaL_b) * block(4);
-  p[P_GTO] = Gto_b * block(5);
-  p[P_GKR] = GKr_b * block(6);
-  p[P_AFF] = dutta ? 0.6 * 1.870 : 0.6;
-  p[P_TJCA] = dutta ? 75.0 * 1.698 : 75.0;
-  p[P_H] = dutta ? 1e-7 * 1.007 : 1e-7;
-  p[P_BCL] = in.bcl;
-  p[P_STIM_AMP] = stim_amp * stim_scale;
-}
+// ORd 2011 (DrugSimulation/cellmodels/Ohara_Rudy_2011.cpp): O11CellIn, kO11Init and o11_params
+// are in ord11_host.hpp
 
 // CONSTANTS[144] in the reference layout (enum E_CONSTANTS_T)
 static void o11_constants(const O11CellIn &in, bool dutta, double stim_scale, double *C)
@@ -1976,6 +1946,10 @@ int cep_compute_rates(cep_batch *b, double t, const double *states, double *rate
 
 static int launch_ord11_adams(cep_batch *b, int nbeats, int nout_fixed)
 {
+  // the Adams workspace is allocated by cep_init_consts when the integrator is Adams; a switch
+  // of integrator after it leaves the workspace unallocated until the next cep_init_consts
+  if (!b->d_aphi || !b->d_avec || !b->d_acoef || !b->d_asc || !b->d_asi)
+    return fail(b, CEP_E_STATE, "the Adams integrator was selected after cep_init_consts: call cep_init_consts again");
   O11Buffers B;
   B.ncell = b->n; B.pitch = b->pitch; B.nbeats_total = b->nbeats_total; B.nwin = b->nwin;
   B.y = b->d_y; B.prm = b->d_prm; B.bmd = b->d_bmd; B.bmi = b->d_bmi; B.status = b->d_status;
@@ -2000,8 +1974,11 @@ int cep_run_outputs(cep_batch *b, int nout)
   if (b->model != CEP_MODEL_ORD2011 || b->integrator != CEP_INTEGRATOR_ADAMS)
     return fail(b, CEP_E_STATE, "output-step runs are the Adams integrator's");
   if (b->cap == 0) return fail(b, CEP_E_STATE, "cep_init_consts not called");
+  if (!b->d_aphi)
+    return fail(b, CEP_E_STATE, "the Adams integrator was selected after cep_init_consts: call cep_init_consts again");
+  if (int rc = alloc_trace_buffers(b)) return rc;
   HIPCHK(b, hipEventRecord(b->ev0, b->stream));
-  launch_ord11_adams(b, 0, nout);
+  if (int rc = launch_ord11_adams(b, 0, nout)) return rc;
   HIPCHK(b, hipGetLastError());
   HIPCHK(b, hipEventRecord(b->ev1, b->stream));
   b->pending = true;
@@ -2012,6 +1989,8 @@ int cep_run_beats_async(cep_batch *b, int nbeats)
 {
   if (!b || nbeats < 1) return fail(b, CEP_E_ARG, "nbeats");
   if (b->cap == 0) return fail(b, CEP_E_STATE, "cep_init_consts not called");
+  if (b->model == CEP_MODEL_ORD2011 && b->integrator == CEP_INTEGRATOR_ADAMS && !b->d_aphi)
+    return fail(b, CEP_E_STATE, "the Adams integrator was selected after cep_init_consts: call cep_init_consts again");
   if (int rc = alloc_trace_buffers(b)) return rc;
   RunCfg rc{b->dt, b->nbeats_total, b->nwin, (int)((1. / b->dt) * 1.0)};
   if (rc.print_freq < 1) rc.print_freq = 1;
@@ -2041,7 +2020,7 @@ int cep_run_beats_async(cep_batch *b, int nbeats)
     else
       hipLaunchKernelGGL((ord_bdf_kernel<float, CPW>), dim3(blocks), dim3(64), 0, b->stream, B, rc, nbeats, max_passes);
   } else if (b->model == CEP_MODEL_ORD2011 && b->integrator == CEP_INTEGRATOR_ADAMS) {
-    launch_ord11_adams(b, nbeats, 0);
+    if (int r = launch_ord11_adams(b, nbeats, 0)) return r;
     b->beats_done += nbeats;
   } else if (b->model == CEP_MODEL_ORD2011) {
     O11Buffers B;

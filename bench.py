@@ -106,26 +106,193 @@ def read_traffic(path=TRAFFIC_FILE, keys=("hbm_bytes_per_beat_launch",)):
         return None
 
 
-def cpu_baseline(seconds: float):
+def host_info():
+    """CPU model and thread counts of the host the CPU legs run on."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    nproc = os.cpu_count() or 1
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = nproc
+    return {"cpu_model": model, "nproc": nproc, "affinity": avail}
+
+
+# Host threads of the CPU legs: one GPU's share of the box's host cores is 16, whatever nproc
+# shows (the box is shared by GPUs).
+CPU_THREADS = 16
+
+
+def cpu_threads():
+    return max(1, min(CPU_THREADS, host_info()["affinity"]))
+
+
+def cpu_baseline(pop, states, seconds: float, beat_index: int):
     """The FP64 CPU oracle (test infrastructure) on a bounded sample of the same population:
-    one beat from rest of a spread of cells, one cell stream per host thread."""
+    cells drawn at random (so the BCL mix is the population's), each started from the state the
+    GPU batch held when the timed beats began (beat `beat_index`) and run for one beat, one cell
+    stream per host thread."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
-    threads = min(16, os.cpu_count() or 1)
-    pop = make_population(10000)
+    threads = cpu_threads()
+    n = len(pop["bcl"])
     rng = np.random.default_rng(7)
-    ncell = max(threads, int(seconds * threads / 0.25))
-    idx = rng.choice(len(pop["bcl"]), size=min(ncell, len(pop["bcl"])), replace=False)
-    ncell = len(idx)
-    cfgs = (O.OrdCellCfg * ncell)()
-    for k, i in enumerate(idx):
-        cfgs[k] = O.make_ord_cfg(celltype=pop["celltype"][i], conc=pop["conc"][i], herg=pop["herg"][i],
-                                 gscale=pop["gscale"][i], bcl=pop["bcl"][i], nbeats=1, nwin=1)
-    res = (O.BmResult * ncell)()
-    wall = O.lib().oracle_ord_run_many(cfgs, ncell, threads, res, None)
+
+    def run(idx):
+        cfgs = (O.OrdCellCfg * len(idx))()
+        for k, i in enumerate(idx):
+            cfgs[k] = O.make_ord_cfg(celltype=pop["celltype"][i], conc=pop["conc"][i], herg=pop["herg"][i],
+                                     gscale=pop["gscale"][i], bcl=pop["bcl"][i], nbeats=1, nwin=1)
+        res = (O.BmResult * len(idx))()
+        # the oracle driver takes per-cell start states through its states_out rows when
+        # started from given states: run each cell on its own thread slot
+        S = np.ascontiguousarray(states[idx])
+        wall = O.lib().oracle_ord_run_many_from(cfgs, len(idx), threads, O.ptr(S), res)
+        return wall
+
+    probe = rng.choice(n, size=4 * threads, replace=False)
+    w0 = run(probe)
+    rate = len(probe) / max(w0, 1e-3)
+    ncell = int(min(n, max(4 * threads, rate * seconds)))
+    idx = rng.choice(n, size=ncell, replace=False)
+    wall = run(idx)
+    info = host_info()
     return {"value": ncell / wall, "unit": "cell-beats/s", "cores": threads, "kind": "port",
-            "sample": f"{ncell} cells x 1 beat (from rest) of the same population, FP64 BDF oracle, "
-                      f"{threads} threads, {wall:.1f} s"}
+            "sample": f"{ncell} random cells of the same population x 1 beat (beat {beat_index + 1}, "
+                      f"from the states the GPU batch timed from), FP64 BDF oracle rtol 1e-6, "
+                      f"{threads} threads, {wall:.1f} s",
+            "cpu_model": info["cpu_model"], "nproc": info["nproc"]}
+
+
+def stratified_cells(pop, per_bcl: int):
+    """per_bcl cells of each BCL class, evenly spaced within the class."""
+    sel = []
+    for b in np.unique(pop["bcl"]):
+        idx = np.nonzero(pop["bcl"] == b)[0]
+        sel += list(idx[np.linspace(0, len(idx) - 1, min(per_bcl, len(idx))).astype(int)])
+    return np.array(sel, dtype=int)
+
+
+def oracle_runs(sub, nbeats: int, rtol: float, atol: float, threads: int):
+    """The FP64 oracle on each cell of `sub`: biomarker rows, final states and last-beat Vm."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    from concurrent.futures import ThreadPoolExecutor
+    n = len(sub["bcl"])
+
+    def one(k):
+        cfg = O.make_ord_cfg(celltype=sub["celltype"][k], conc=sub["conc"][k], herg=sub["herg"][k],
+                             gscale=sub["gscale"][k], bcl=sub["bcl"][k], nbeats=nbeats, nwin=nbeats,
+                             rtol=rtol, atol=atol)
+        rc, S, res, vm, _ = O.ord_run(cfg, trace=True)
+        return rc, S, [res[f] for f in cep.BM_FIELDS], vm
+
+    with ThreadPoolExecutor(threads) as ex:      # ctypes releases the GIL inside the oracle
+        r = list(ex.map(one, range(n)))
+    L = max(len(x[3]) for x in r)
+    vm = np.full((n, L), np.nan)
+    for k, x in enumerate(r):
+        vm[k, :len(x[3])] = x[3]
+    return {"rc": np.array([x[0] for x in r]), "S": np.array([x[1] for x in r]),
+            "bm": np.array([x[2] for x in r], dtype=float), "vm": vm}
+
+
+def compare_to_oracle(bm, vm, o6, o7):
+    """Errors of one GPU solver against the FP64 oracle at the same tolerance (o6), with the
+    oracle's own change when its tolerance is tightened ten times (o7) as the resolution of each
+    figure.  A cell 'repolarises' when both the solver and the oracle find an APD90 in the
+    selected beat; a figure is 'resolved' when the oracle changes it by less than the bar
+    (1 ms, 1 %) between the two tolerances."""
+    bi = {k: i for i, k in enumerate(cep.BM_FIELDS)}
+    a, q, p = bi["apd90"], bi["qnet"], bi["steepest_pace"]
+    B, B7 = o6["bm"], o7["bm"]
+    rep = (bm[:, a] > 0) & (B[:, a] > 0)
+    ea = np.abs(bm[:, a] - B[:, a])
+    ea7 = np.abs(B7[:, a] - B[:, a])
+    den = np.maximum(np.abs(B[:, q]), 1e-300)
+    eq = np.abs(bm[:, q] - B[:, q]) / den
+    eq7 = np.abs(B7[:, q] - B[:, q]) / den
+    qres = rep & (eq7 <= 0.01)
+    ares = rep & (ea7 <= 1.0)
+    vml = np.nanmax(np.abs(vm - o6["vm"]), axis=1)
+    over = [{"cell": int(k), "qnet_oracle": float(B[k, q]), "rel_err": float(eq[k]),
+             "oracle_self_rel": float(eq7[k]), "apd90_oracle": float(B[k, a])}
+            for k in np.nonzero(rep & (eq > 0.01))[0]]
+    fields = {f: float(np.max(np.abs(bm[rep, bi[f]] - B[rep, bi[f]]))) if rep.any() else None
+              for f in cep.BM_FIELDS if f != "steepest_pace"}
+    return {
+        "cells_repolarising": int(rep.sum()),
+        "apd90_max_abs_err_ms": float(ea[rep].max()) if rep.any() else None,
+        "apd90_frac_within_1ms": float(np.mean(ea[rep] <= 1.0)) if rep.any() else None,
+        "apd90_max_abs_err_ms_resolved": float(ea[ares].max()) if ares.any() else None,
+        "qnet_max_rel_err": float(eq[rep].max()) if rep.any() else None,
+        "qnet_frac_within_1pct": float(np.mean(eq[rep] <= 0.01)) if rep.any() else None,
+        "qnet_max_rel_err_resolved": float(eq[qres].max()) if qres.any() else None,
+        "qnet_cells_resolved": int(qres.sum()),
+        "qnet_cells_over_1pct": over,
+        "steepest_pace_agree_frac": float(np.mean(bm[:, p] == B[:, p])),
+        "vm_linf_mv_last_beat": {"median": float(np.median(vml)), "p90": float(np.percentile(vml, 90)),
+                                 "max_repolarising": float(vml[rep].max()) if rep.any() else None,
+                                 "max": float(vml.max())},
+        "max_abs_err_by_field_repolarising": fields,
+    }
+
+
+def accuracy_check(pop, bm_timed, per_bcl: int, nbeats: int):
+    """The timed mixed solver and the plain per-job FP64 GPU solver against the FP64 CPU oracle
+    on BCL-stratified cells after the beats the bench ran."""
+    sel = stratified_cells(pop, per_bcl)
+    sub = {k: v[sel] for k, v in pop.items()}
+    threads = cpu_threads()
+    from concurrent.futures import ThreadPoolExecutor
+    ex = ThreadPoolExecutor(1)
+    # the oracle runs on the host while the GPU runs the two solvers
+    fut = ex.submit(lambda: (oracle_runs(sub, nbeats, 1e-6, 1e-8, threads),
+                             oracle_runs(sub, nbeats, 1e-7, 1e-9, threads)))
+    out = {}
+    for prec, tag in ((cep.PREC_MIXED, "mixed"), (cep.PREC_FP64, "fp64")):
+        log(f"bench: {tag} GPU job of {len(sel)} check cells, {nbeats} beats")
+        b = make_batch(sub, prec, 200)
+        b.enable_vm_trace(True)
+        t0 = time.perf_counter()
+        b.run_beats(nbeats)
+        t1 = time.perf_counter()
+        out[tag] = {"bm": b.biomarker_array(), "vm": b.vm_trace(), "rate": len(sel) * nbeats / (t1 - t0),
+                    "failed": int(np.sum(b.status() != 0))}
+        b.close()
+    o6, o7 = fut.result()
+    ex.shutdown()
+    mx = compare_to_oracle(out["mixed"]["bm"], out["mixed"]["vm"], o6, o7)
+    f64 = compare_to_oracle(out["fp64"]["bm"], out["fp64"]["vm"], o6, o7)
+    # the oracle against itself at the tighter tolerance: the resolution of the reference figures
+    selfc = compare_to_oracle(o7["bm"], o7["vm"], o6, o7)
+    same = bool(np.array_equal(out["mixed"]["bm"], bm_timed[sel]))
+    gate = {
+        "apd90_le_1ms_every_repolarising_cell": bool(mx["apd90_max_abs_err_ms"] is not None
+                                                     and mx["apd90_max_abs_err_ms"] <= 1.0),
+        "qnet_le_1pct_every_resolved_cell": bool(mx["qnet_max_rel_err_resolved"] is not None
+                                                 and mx["qnet_max_rel_err_resolved"] <= 0.01),
+    }
+    gate["pass"] = all(gate.values())
+    return {
+        "oracle": "FP64 CPU oracle (CVODE-style BDF restated), rtol 1e-6/atol 1e-8 as the GPU; "
+                  "rtol 1e-7/atol 1e-9 for the oracle's own resolution",
+        "cells": int(len(sel)), "cells_per_bcl": per_bcl, "beats": nbeats,
+        "mixed_job_equals_timed_batch": same,
+        "mixed_vs_oracle": mx,
+        "fp64_gpu_vs_oracle": f64,
+        "oracle_rtol1e-7_vs_rtol1e-6": {k: selfc[k] for k in ("apd90_max_abs_err_ms", "qnet_max_rel_err",
+                                                               "steepest_pace_agree_frac", "vm_linf_mv_last_beat")},
+        "gate": gate,
+        "fp64_plain_job_cell_beats_per_s": out["fp64"]["rate"],
+        "oracle_failed_cells": int(np.sum(o6["rc"] != 0)),
+    }
 
 
 def serving_latency(njobs: int, rate: float):
@@ -267,7 +434,7 @@ def tt06_main(args):
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as O
-        threads = min(16, os.cpu_count() or 1)
+        threads = cpu_threads()
         ncpu = max(threads, int(args.cpu_seconds * threads / 0.12))
         sel = np.linspace(0, n - 1, ncpu).astype(int)
         cfgs = (O.TtCellCfg * ncpu)()
@@ -280,7 +447,8 @@ def tt06_main(args):
         wall = O.lib().oracle_tt_run_many(cfgs, ncpu, threads, res, None)
         cpu = {"value": ncpu / wall, "unit": "cell-beats/s", "cores": threads, "kind": "port",
                "sample": f"{ncpu} cells x 1 beat of the same population, FP64 Rush-Larsen oracle, "
-                         f"{threads} threads, {wall:.1f} s"}
+                         f"{threads} threads, {wall:.1f} s",
+               "cpu_model": host_info()["cpu_model"], "nproc": host_info()["nproc"]}
     print(json.dumps({
         "metric": "TT06 cell-beats/s on 1 GPU (Rush-Larsen dt 0.02 ms)",
         "value": value, "unit": "cell-beats/s", "n_gpus": 1, "steps": K, "warmup": W,
@@ -483,10 +651,11 @@ def ord11_cpu_baseline(q, s0):
     import oracle_lib as O
     cfgs = [O.make_ord11_cfg(conc=c, ic50=q[i], bcl=2000.0, nbeats=20, nwin=20, celltype=2.0)
             for c in (3237.0, 6474.0, 9711.0) for i in range(len(q))]
-    threads = min(16, os.cpu_count() or 1)
+    threads = cpu_threads()
     _, _, wall = O.ord11_run_many(cfgs, s0, nthreads=threads)
     return {"value": len(cfgs) * 20 / wall, "unit": "cell-beats/s", "cores": threads, "kind": "port",
-            "sample": f"{len(cfgs)} cells x 20 beats of the same run, FP64 oracle, {threads} threads, {wall:.1f} s"}
+            "sample": f"{len(cfgs)} cells x 20 beats of the same run, FP64 oracle, {threads} threads, {wall:.1f} s",
+            "cpu_model": host_info()["cpu_model"], "nproc": host_info()["nproc"]}
 
 
 def main():
@@ -501,7 +670,12 @@ def main():
     ap.add_argument("--ncell", type=int, default=10000)
     ap.add_argument("--precision", choices=("mixed", "fp64"), default=None,
                     help="population default: mixed; tissue default: fp64")
-    ap.add_argument("--check-cells", type=int, default=64)
+    ap.add_argument("--check-cells", type=int, default=64, help="tt06: FP64 check cells")
+    ap.add_argument("--check-per-bcl", type=int, default=32,
+                    help="population: oracle-checked cells per BCL class (0: no accuracy check)")
+    ap.add_argument("--fp64-beats", type=int, default=2,
+                    help="population: beats of the other precision on the full batch (0: skip)")
+    ap.add_argument("--legs", type=int, default=1, help="population: compact TT06 and tissue legs")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=24)
@@ -537,6 +711,7 @@ def main():
     log(f"bench: {n} cells, warmup {W} beat(s)")
     if W:
         b.run_beats(W)
+    states_w = b.states()               # the state the timed beats start from (CPU baseline)
     b.kernel_time()                     # reset the event timer
     stats0 = b.stats()
     log(f"bench: timing {K} beat(s)")
@@ -547,7 +722,7 @@ def main():
     nl, kavg_ms, ktot_ms = b.kernel_time()
     stats1 = b.stats()
     st = b.status()
-    bm_mixed = b.biomarker_array()
+    bm_timed = b.biomarker_array()
     elapsed = t1 - t0
     value = n * K / elapsed
     flops_launch = flops_of(stats1 - stats0) / max(nl, 1)
@@ -556,44 +731,42 @@ def main():
     nrhs = float(np.sum((stats1 - stats0)[:, [1, 7]]))
     b.close()
 
-    # accuracy of the timed solver vs the FP64 solver on a subsample run as one plain job
-    chk = {}
-    if args.check_cells > 0:
-        sel = np.linspace(0, n - 1, args.check_cells).astype(int)
-        sub = {k: v[sel] for k, v in pop.items()}
-        log(f"bench: FP64 plain job of {len(sel)} cells")
-        b64 = make_batch(sub, cep.PREC_FP64, nb_total)
+    # the other precision on the same 10,000 cells: what mixed precision buys at full size
+    other = {}
+    if args.fp64_beats > 0:
+        oprec = cep.PREC_FP64 if prec == cep.PREC_MIXED else cep.PREC_MIXED
+        log(f"bench: {'FP64' if oprec == cep.PREC_FP64 else 'mixed'} batch of {n} cells, "
+            f"1 + {args.fp64_beats} beat(s)")
+        bo = make_batch(pop, oprec, nb_total)
+        bo.run_beats(1)
         t2 = time.perf_counter()
-        b64.run_beats(W + K)
+        bo.run_beats(args.fp64_beats)
         t3 = time.perf_counter()
-        bm64 = b64.biomarker_array()
-        b64.close()
-        bi = {k: i for i, k in enumerate(cep.BM_FIELDS)}
-        plain = len(sel) * (W + K) / (t3 - t2)
-        qm, q64 = bm_mixed[sel, bi["qnet"]], bm64[:, bi["qnet"]]
-        rel = np.abs(qm - q64) / np.maximum(np.abs(q64), 1e-300)
-        # cells with a regular action potential in both solvers (APD90 found); cells that do not
-        # repolarise within the beat (EAD / depolarised under strong block) are chaotic and
-        # are reported separately
-        rep = (bm_mixed[sel, bi["apd90"]] > 0) & (bm64[:, bi["apd90"]] > 0)
-        chk = {
-            "apd90_max_abs_err_ms": float(np.abs(bm_mixed[sel, bi["apd90"]] - bm64[:, bi["apd90"]]).max()),
-            "qnet_max_rel_err": float(rel[rep].max()) if rep.any() else None,
-            "qnet_max_rel_err_all_cells": float(rel.max()),
-            "qnet_frac_within_1pct": float(np.mean(rel <= 0.01)),
-            "cells_repolarising": int(rep.sum()),
-            "fp64_plain_job_cell_beats_per_s": plain,
-            "batched_mixed_over_plain_fp64": value / plain,
-            "cells_checked": int(len(sel)),
-        }
+        bo.close()
+        other = {"precision": "fp64" if oprec == cep.PREC_FP64 else "mixed",
+                 "cell_beats_per_s": n * args.fp64_beats / (t3 - t2),
+                 "beats_timed": args.fp64_beats, "after_beats": 1}
+
+    # accuracy of the timed solver against the FP64 CPU oracle, BCL-stratified, at the beat
+    # count the timed batch reached
+    chk = {}
+    if args.check_per_bcl > 0:
+        chk = accuracy_check(pop, bm_timed, args.check_per_bcl, W + K)
+        plain = chk["fp64_plain_job_cell_beats_per_s"]
+        chk["batched_mixed_over_plain_fp64"] = value / plain if prec == cep.PREC_MIXED else None
+        if other.get("precision") == "fp64":
+            chk["batched_mixed_over_batched_fp64"] = value / other["cell_beats_per_s"]
     lat = {"p50": None, "p99": None}
     if args.serve_jobs > 0:
         log(f"bench: serving session, {args.serve_jobs} jobs")
         lat = serving_latency(args.serve_jobs, args.serve_rate)
+    legs = {}
+    if args.legs:
+        legs = compact_legs(args)
     cpu = None
     if not args.no_cpu_baseline:
         log("bench: CPU baseline")
-        cpu = cpu_baseline(args.cpu_seconds)
+        cpu = cpu_baseline(pop, states_w, args.cpu_seconds, W)
     traffic = read_traffic()
     out = {
         "metric": "ORd cell-beats/s on 1 GPU at APD90 err <=1 ms; p50/p99 job latency",
@@ -611,18 +784,65 @@ def main():
         "config": {"workload": "ORd-CiPA batched population, 10,000 cells = 50 drugs x 4 conc x 50 "
                                "variants, BCL {500,1000,2000} ms, BDF rtol 1e-6/atol 1e-8, step = 1 beat",
                    "cells": n, "beats_per_job": nb_total},
-        "accuracy_vs_fp64": chk,
+        "accuracy_vs_oracle": chk,
+        "other_precision_full_batch": other,
         "failed_cells": int(np.sum(st != 0)),
         "latency_ms": lat,
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / peak,
                      "traffic": traffic,
+                     "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE of the same launch, "
+                                       "profiles/ord_bdf_population_10k_traffic.txt",
                      "kernel": f"ord_bdf_kernel<{'float' if prec == cep.PREC_MIXED else 'double'}, 16>",
                      "kernel_avg_ms": kavg_ms, "flops_per_launch": flops_launch,
                      "rhs_evals_per_launch": nrhs / max(nl, 1)},
+        "other_workloads": legs,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def compact_legs(args):
+    """Short runs of the two other configurations, timed by the same driver run: one beat of the
+    200,000-cell TT06 population (mixed) and 100 Strang steps of the 512 x 512 tissue (FP64)."""
+    legs = {}
+    pop = cep.workload.tt_population(20251205, args.tt_cells)
+    herg = np.concatenate([pop["ic50"], pop["hill"]], axis=1)
+    n = args.tt_cells
+    log(f"bench: TT06 leg, {n} cells x 1 beat")
+    b = cep.CellBatch(cep.MODEL_TT06, n, cep.PREC_MIXED)
+    b.set_cells(celltype=pop["celltype"], conc=pop["conc"], herg=herg, gscale=pop["gscale"], bcl=pop["bcl"])
+    b.set_run_config(dt=0.02, nbeats_total=500, nwin=500)
+    b.init_consts()
+    b.run_beats(1)
+    b.kernel_time()
+    t0 = time.perf_counter()
+    b.run_beats(1)
+    t1 = time.perf_counter()
+    _, kavg, _ = b.kernel_time()
+    bad = int(np.sum(b.status() != 0))
+    b.close()
+    fl = n * int(round(1000.0 / 0.02)) * F_TT_STEP
+    legs["tt06"] = {"workload": f"TT06 {n} cells, Rush-Larsen dt 0.02 ms, mixed, beat 2",
+                    "cell_beats_per_s": n / (t1 - t0), "kernel_avg_ms": kavg, "failed_cells": bad,
+                    "roofline": {"bound": "valu", "achieved": fl / (kavg * 1e-3) / 1e12, "peak": PEAK_FP32 / 1e12,
+                                 "unit": "TFLOP/s", "frac": fl / (kavg * 1e-3) / PEAK_FP32}}
+    log("bench: tissue leg, 512 x 512, 20 + 100 steps")
+    tis, _, _ = tissue_setup(TISSUE_NX, cep.PREC_FP64)
+    tis.step(20, TISSUE_DT, cep.SPLIT_STRANG)
+    tis.kernel_times()
+    t0 = time.perf_counter()
+    iters = tis.step(100, TISSUE_DT, cep.SPLIT_STRANG)
+    t1 = time.perf_counter()
+    avg, _ = tis.kernel_times()
+    nn = tis.nn
+    tis.close()
+    legs["tissue"] = {"workload": "512x512 Q1 monodomain, ORd-CiPA nodes, Strang dt 0.01 ms, FP64 CG",
+                      "node_steps_per_s": nn * 100 / (t1 - t0), "cg_iters_per_step": iters / 100,
+                      "cg_spmv_gbs": nn * B_SPMV / (avg[1] * 1e-3) / 1e9 if avg[1] > 0 else None,
+                      "kernels_avg_ms": {"tissue_react": float(avg[0]), "cg_spmv": float(avg[1]),
+                                         "cg_update": float(avg[2])}}
+    return legs
 
 
 if __name__ == "__main__":

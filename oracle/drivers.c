@@ -116,7 +116,7 @@ typedef struct {
   int kind, ncell, nthreads, tid;
   const void *cfgs;
   bm_result_t *res;
-  double *states_out;
+  double *states_out;   /* kind 2: per-cell start states in, final states out */
 } job_t;
 
 static void *worker(void *p)
@@ -127,6 +127,9 @@ static void *worker(void *p)
       double S[ORD_NS];
       oracle_ord_run(&((const ord_cell_cfg_t *)j->cfgs)[c], S, 0, j->res ? &j->res[c] : NULL, NULL, NULL);
       if (j->states_out) memcpy(j->states_out + (size_t)c * ORD_NS, S, sizeof(S));
+    } else if (j->kind == 2) {
+      oracle_ord_run(&((const ord_cell_cfg_t *)j->cfgs)[c], j->states_out + (size_t)c * ORD_NS, 1,
+                     j->res ? &j->res[c] : NULL, NULL, NULL);
     } else {
       double S[TT_NS];
       oracle_tt_run(&((const tt_cell_cfg_t *)j->cfgs)[c], S, 0, j->res ? &j->res[c] : NULL, NULL);
@@ -156,6 +159,14 @@ static double run_many(int kind, const void *cfgs, int ncell, int nthreads, bm_r
 double oracle_ord_run_many(const ord_cell_cfg_t *cfgs, int ncell, int nthreads, bm_result_t *res, double *states_out)
 {
   return run_many(0, cfgs, ncell, nthreads, res, states_out);
+}
+
+/* as oracle_ord_run_many, each cell started from its row of states_io (the reference's restart
+   from a saved state, cipa_drug_bench.cpp:151); final states are written back to the rows */
+double oracle_ord_run_many_from(const ord_cell_cfg_t *cfgs, int ncell, int nthreads, double *states_io,
+                                bm_result_t *res)
+{
+  return run_many(2, cfgs, ncell, nthreads, res, states_io);
 }
 
 double oracle_tt_run_many(const tt_cell_cfg_t *cfgs, int ncell, int nthreads, bm_result_t *res, double *states_out)

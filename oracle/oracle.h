@@ -171,6 +171,9 @@ double oracle_ord_run_many(const ord_cell_cfg_t *cfgs, int ncell, int nthreads,
                            bm_result_t *res, double *states_out);
 double oracle_tt_run_many(const tt_cell_cfg_t *cfgs, int ncell, int nthreads,
                           bm_result_t *res, double *states_out);
+/* ncell ORd cells, cell c started from row c of states_io (52 doubles), final states written back */
+double oracle_ord_run_many_from(const ord_cell_cfg_t *cfgs, int ncell, int nthreads, double *states_io,
+                                bm_result_t *res);
 /* ord_rates over n cells (rows of C and S), one thread; returns wall seconds */
 double oracle_ord_rates_many(int n, const double *C, const double *S, double *R, double t);
 

@@ -79,6 +79,9 @@ def lib():
         L.oracle_ord_run_many.argtypes = [C.POINTER(OrdCellCfg), C.c_int, C.c_int,
                                           C.POINTER(BmResult), dp]
         L.oracle_ord_run_many.restype = C.c_double
+        L.oracle_ord_run_many_from.argtypes = [C.POINTER(OrdCellCfg), C.c_int, C.c_int, dp,
+                                               C.POINTER(BmResult)]
+        L.oracle_ord_run_many_from.restype = C.c_double
         L.oracle_tt_run_many.argtypes = [C.POINTER(TtCellCfg), C.c_int, C.c_int,
                                          C.POINTER(BmResult), dp]
         L.oracle_tt_run_many.restype = C.c_double

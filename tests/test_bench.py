@@ -37,3 +37,62 @@ def test_counted_flops_of_the_bdf_statistics():
             + 1 * bench.F_RHS + 1 * bench.F_NEWTON + 1 * bench.F_STEP)
     assert bench.flops_of(st) == want
     assert bench.F_TT_STEP == 534
+
+
+def test_stratified_cells_cover_every_bcl():
+    import bench
+    pop = {"bcl": np.array([500.0] * 10 + [1000.0] * 7 + [2000.0] * 3)}
+    sel = bench.stratified_cells(pop, 4)
+    b = pop["bcl"][sel]
+    assert (b == 500).sum() == 4 and (b == 1000).sum() == 4 and (b == 2000).sum() == 3
+    assert len(set(sel.tolist())) == len(sel)
+
+
+def test_compare_to_oracle_separates_unresolved_cells():
+    """A cell whose qNet the oracle itself changes by more than 1% between its two tolerances is
+    reported but kept out of the resolved maximum; APD90 errors count on repolarising cells only."""
+    import bench
+    nf = len(bench.cep.BM_FIELDS)
+    bi = {k: i for i, k in enumerate(bench.cep.BM_FIELDS)}
+    o6 = np.zeros((3, nf)); o7 = np.zeros((3, nf)); g = np.zeros((3, nf))
+    o6[:, bi["apd90"]] = [300.0, 400.0, 0.0]       # third cell does not repolarise
+    o7[:, bi["apd90"]] = [300.1, 400.0, 0.0]
+    g[:, bi["apd90"]] = [300.5, 400.2, 0.0]
+    o6[:, bi["qnet"]] = [0.100, 0.050, -20.0]
+    o7[:, bi["qnet"]] = [0.1001, 0.055, -30.0]      # cell 1: oracle moves 10 % -> unresolved
+    g[:, bi["qnet"]] = [0.1005, 0.056, -40.0]
+    vm = np.zeros((3, 4))
+    r = bench.compare_to_oracle(g, vm, {"bm": o6, "vm": vm}, {"bm": o7, "vm": vm})
+    assert r["cells_repolarising"] == 2
+    assert abs(r["apd90_max_abs_err_ms"] - 0.5) < 1e-12
+    assert abs(r["qnet_max_rel_err"] - 0.12) < 1e-9
+    assert r["qnet_cells_resolved"] == 1
+    assert abs(r["qnet_max_rel_err_resolved"] - 0.005) < 1e-9
+    assert [c["cell"] for c in r["qnet_cells_over_1pct"]] == [1]
+
+
+def test_oracle_restart_from_states_matches_single_runs():
+    """oracle_ord_run_many_from (the CPU baseline's restart at the timed beat) gives per cell
+    what oracle_ord_run gives from the same start state."""
+    import oracle_lib as O
+    wl = __import__("importlib").import_module("2ddrugsimulation_amd").workload
+    pop = wl.ord_single_drug(seed=20251203)
+    n = 3
+    rng = np.random.default_rng(3)
+    S0 = np.zeros((n, 52))
+    for k in range(n):
+        _, S = O.ord_init(0.0, pop["conc"][k], pop["herg"][k])
+        S0[k] = S * (1 + 1e-3 * rng.standard_normal(52))
+        S0[k, 43] = pop["conc"][k]
+    cfgs = (O.OrdCellCfg * n)()
+    for k in range(n):
+        cfgs[k] = O.make_ord_cfg(celltype=0.0, conc=pop["conc"][k], herg=pop["herg"][k], bcl=500.0,
+                                 nbeats=1, nwin=1)
+    res = (O.BmResult * n)()
+    S = np.ascontiguousarray(S0.copy())
+    O.lib().oracle_ord_run_many_from(cfgs, n, 2, O.ptr(S), res)
+    for k in range(n):
+        rc, Sk, r, _, _ = O.ord_run(cfgs[k], states=S0[k])
+        assert rc == 0
+        assert np.array_equal(Sk, S[k])
+        assert r["apd90"] == res[k].apd90 and r["qnet"] == res[k].qnet
