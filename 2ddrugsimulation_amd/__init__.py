@@ -25,6 +25,10 @@ OPT_TIME_POINT = 2
 OPT_INTEGRATOR = 3
 OPT_TIME_SERIES = 4
 OPT_ORD11_WAVES = 5
+OPT_DRIVER_RULES = 6
+OPT_V_APD90 = 7
+OPT_V_APD50 = 8
+OPT_FIRST_OUTPUT = 9
 INTEGRATOR_RUSH_LARSEN = 0
 INTEGRATOR_ADAMS = 1
 PREC_FP64 = 0
@@ -38,6 +42,11 @@ NBM = len(BM_FIELDS)
 BM11_FIELDS = ("qnet_ap", "qnet4_ap", "inal_auc_ap", "ical_auc_ap", "qnet_cl", "qnet4_cl",
                "inal_auc_cl", "ical_auc_cl", "vm_peak", "vm_valley", "dvmdt_repol", "t_peak",
                "vm_repol30", "vm_repol50", "vm_repol90", "steepest_pace")
+# records of the other reference drivers' rules (cep_get_driver_rules)
+DR_FIELDS = ("dh_apd", "dh_t_dep", "dh_t_rep", "dh_nrep", "dh_qnet", "dh_inal_auc", "dh_ical_auc",
+             "bp_qnet", "bp_max_dvmdt", "bp_vm_peak", "bp_vm_dia", "bp_apd90", "bp_apd50", "bp_ca_peak",
+             "bp_ca_dia", "bp_cad90", "bp_cad50")
+DR_TT06_FIELDS = ("apd", "mean_cai")
 STATUS_NAMES = {0: "ok", 1: "conv_fail", 2: "err_fail", 3: "nonfinite", 4: "max_steps"}
 
 EXPORTED = (
@@ -47,7 +56,7 @@ EXPORTED = (
     "cep_run_beats", "cep_run_beats_async", "cep_sync", "cep_get_states", "cep_set_states",
     "cep_get_biomarkers", "cep_get_status", "cep_get_stats", "cep_enable_vm_trace",
     "cep_get_vm_trace", "cep_get_time_series", "cep_get_dvmdt_repol_history",
-    "cep_kernel_time", "cep_query", "cep_batch_device_bytes",
+    "cep_kernel_time", "cep_query", "cep_batch_device_bytes", "cep_get_driver_rules", "cep_get_qinward",
     "cep_tissue_create", "cep_tissue_create_model", "cep_tissue_destroy", "cep_tissue_last_error", "cep_tissue_set_cells",
     "cep_tissue_set_conductivity", "cep_tissue_set_stimulus", "cep_tissue_step", "cep_tissue_diffuse",
     "cep_tissue_get_vm", "cep_tissue_get_states", "cep_tissue_set_states", "cep_tissue_time",
@@ -113,6 +122,8 @@ def library(path: str | None = None):
     L.cep_get_dvmdt_repol_history.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int), dp,
                                               C.POINTER(C.c_int)]
     L.cep_kernel_time.argtypes = [C.c_void_p, dp, dp]
+    L.cep_get_driver_rules.argtypes = [C.c_void_p, dp]
+    L.cep_get_qinward.argtypes = [C.c_void_p, C.POINTER(C.c_int), dp]
     L.cep_query.argtypes = [C.c_void_p]
     L.cep_batch_device_bytes.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int]
     L.cep_batch_device_bytes.restype = C.c_longlong
@@ -237,6 +248,30 @@ class CellBatch:
 
     def init_consts(self):
         self._chk(self.L.cep_init_consts(self.h))
+
+    def set_driver_rules(self, on=True, v_apd90=None, v_apd50=None, first_output=None):
+        """Also evaluate the biomarker rules of the drugs_herg / benchpatch drivers (ORd-CiPA) or
+        of population_tn2006 (TT06); call before init_consts."""
+        self.set_model_option(OPT_DRIVER_RULES, 1.0 if on else 0.0)
+        if v_apd90 is not None:
+            self.set_model_option(OPT_V_APD90, v_apd90)
+        if v_apd50 is not None:
+            self.set_model_option(OPT_V_APD50, v_apd50)
+        if first_output is not None:
+            self.set_model_option(OPT_FIRST_OUTPUT, first_output)
+
+    def driver_rules(self) -> dict:
+        names = DR_TT06_FIELDS if self.model == MODEL_TT06 else DR_FIELDS
+        out = np.zeros((self.n, len(names)))
+        self._chk(self.L.cep_get_driver_rules(self.h, _p(out)))
+        return {k: out[:, i].copy() for i, k in enumerate(names)}
+
+    def qinward(self, control) -> np.ndarray:
+        """qinward of each cell against control[i], its concentration-0 cell (-1: NaN), on the device."""
+        ctl = np.ascontiguousarray(control, dtype=np.int32)
+        out = np.zeros(self.n)
+        self._chk(self.L.cep_get_qinward(self.h, ctl.ctypes.data_as(C.POINTER(C.c_int)), _p(out)))
+        return out
 
     # ---- CellML arrays
     def constants(self, cell: int) -> np.ndarray:

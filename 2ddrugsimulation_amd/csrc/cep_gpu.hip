@@ -1,6 +1,7 @@
 // cep_gpu.hip -- kernels and C ABI of the cardiac cell-population engine (gfx950).
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <stddef.h>
 #include <stdio.h>
 #include <string.h>
 
@@ -13,6 +14,7 @@
 #include "../../include/cep_tissue.h"
 #include "adams.hpp"
 #include "bdf_ord.hpp"
+#include "driver_rules.hpp"
 #include "ord11_model.hpp"
 #include "ord11_host.hpp"
 #include "tt06_model.hpp"
@@ -84,6 +86,9 @@ __global__ void __launch_bounds__(64) tt_rates_kernel(int n, int pitch, double t
 struct RunCfg {
   double dt;
   int nbeats_total, nwin, print_freq;
+  double t_first;             // first output time (ORd-CiPA; dt unless set)
+  double v_apd90, v_apd50;    // thresholds of the driver rules
+  double *dr;                 // driver-rule state (DR_SN rows), or null: rules off
 };
 
 CEP_D void bm_config(Biomarker &bm, double bcl, double stim_start, const RunCfg &rc, int cap)
@@ -161,6 +166,8 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
   __shared__ int s_oiout[CPW], s_opend[CPW], s_ofail[CPW];
   __shared__ unsigned short s_jq[QCAP];           // Jacobian tasks: (slot << 6) | column slot
   __shared__ double s_map[CPW][NMAP];             // each cell's attempt map for the wave sweep
+  __shared__ DriverRules s_dr[CPW];               // driver rules of the output lanes (rc.dr set)
+  __shared__ double s_acc[CPW][3];                // output point's STATES[qnet, INaL_AUC, ICaL_AUC]
 
   const int lane = threadIdx.x;
   const int cell0 = blockIdx.x * CPW;
@@ -188,7 +195,7 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
     } else if (ph == PH_FRESH) {
       // CVodeInit at t0 = 0 with the current states
       for (int i = 0; i < N; ++i) L.ZN(0, i) = L.Y(i);
-      m.tn = 0.0; m.tout = rc.dt; m.q = 1; m.qprime = 1; m.qwait = 2; m.etamax = ETAMX1;
+      m.tn = 0.0; m.tout = rc.t_first; m.q = 1; m.qprime = 1; m.qwait = 2; m.etamax = ETAMX1;
       m.h = 0.0; m.hprime = 0.0; m.hscale = 0.0; m.eta = 1.0; m.nst = m.nstlp = m.nstlj = 0;
       m.crate = 1.0; m.gamrat = 1.0; m.gammap = 0.0; m.saved_tq5 = 0.0; m.jcur = 0; m.iout = 0;
       m.nfe = m.nje = m.nsetups = m.nni = m.ncfn = m.netf = 0;
@@ -212,8 +219,13 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
     Biomarker &bm = s_bm[slot];
     bm_config(bm, B.prm[ord::OP_BCL * B.pitch + c], B.prm[ord::OP_STIM_START * B.pitch + c], rc, B.cap);
     const int iout0 = B.si[11 * B.pitch + c];
-    if (B.si[4 * B.pitch + c] == PH_FRESH) bm.begin(B.y[c], B.y[9 * B.pitch + c]);
+    const bool fresh = B.si[4 * B.pitch + c] == PH_FRESH;
+    if (fresh) bm.begin(B.y[c], B.y[9 * B.pitch + c]);
     else bm.load(B.bmd, B.bmi, B.pitch, c);
+    if (rc.dr) {
+      if (fresh) s_dr[slot].begin(B.y[c]);
+      else s_dr[slot].load(rc.dr, B.pitch, c);
+    }
     trace_start = iout0 + (nbeats - 1) * bm.beat_steps;
     s_opend[slot] = 0;
     s_ofail[slot] = 0;
@@ -468,6 +480,9 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
       for (int i = 0; i < N; ++i) y[i] = LV[i];
       v0 = y[0];
       ca0 = y[9];
+      if (req == RQ_OUT && rc.dr) {
+        s_acc[slot][0] = y[49]; s_acc[slot][1] = y[50]; s_acc[slot][2] = y[51];
+      }
       ord::rhs<R>(treq, y, LV, P, &cur);          // f overwrites the argument in LDS
     }
     const LVec &f = LV;
@@ -535,6 +550,18 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
                         B.ct + c, B.cv + c, pp, tv, ires);
       if (B.vtrace && iout > trace_start && iout - trace_start <= B.tcap)
         B.vtrace[(iout - trace_start - 1) * pp + c] = v0;
+      if (rc.dr) {
+        // benchpatch takes RATES[V] at the next output time: the same currents, the stimulus there
+        const PrmAcc P{B.prm + c, pp};
+        const double bcl = P(ord::OP_BCL);
+        auto istim = [&](double tt) {
+          return ord::stimulus(tt, P(ord::OP_STIM_START), P(ord::OP_STIM_END), P(ord::OP_STIM_AMP), bcl,
+                               P(ord::OP_STIM_DUR));
+        };
+        s_dr[slot].sample(iout, treq, v0, ca0, s_acc[slot][0], s_acc[slot][1], s_acc[slot][2],
+                          f[0] + istim(treq) - istim(treq + rc.dt), bcl, rc.dt, rc.nbeats_total, rc.v_apd90,
+                          rc.v_apd50);
+      }
       s_opend[slot] = 0;
     } else if (req == RQ_JAC) {
       Lane<R> Lo{B, s_m[jown], V, (unsigned)jown, (unsigned)(cell0 + jown), pp};
@@ -561,7 +588,10 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
     if (m.phase == PH_DONE && m.status == ST_OK) m.phase = PH_STEP_DONE;
     L.store();
   }
-  if (is_out) s_bm[slot].store(B.bmd, B.bmi, B.pitch, c);
+  if (is_out) {
+    s_bm[slot].store(B.bmd, B.bmi, B.pitch, c);
+    if (rc.dr) s_dr[slot].store(rc.dr, B.pitch, c);
+  }
 }
 
 // ---- TT06: fixed-dt Rush-Larsen + biomarkers, one cell per lane
@@ -593,6 +623,13 @@ tt_rl_kernel(TtBuffers B, RunCfg rc, int nbeats)
   double *ct = B.ct + c, *cv = B.cv + c;
   const int steps = nbeats * bm.beat_steps;
   const int trace_start = iout + steps - bm.beat_steps;
+  // population_tn2006's rules (rc.dr set): crossing APD and the last beat's mean Ca_i
+  TnRules tn{y[0], 0.0, 0.0, 0.0, 0.0, 0.0};
+  const double tn_last = (rc.nbeats_total - 1) * P(tt::TP_BCL);
+  if (rc.dr && iout > 0) {
+    double *t6 = &tn.v_prev;
+    for (int k = 0; k < 6; ++k) t6[k] = rc.dr[(size_t)k * B.pitch + c];
+  }
   for (int s = 0; s < steps; ++s) {
     tt::Currents cur;
     double dvdt;
@@ -601,6 +638,7 @@ tt_rl_kernel(TtBuffers B, RunCfg rc, int nbeats)
     iout++;
     bm.sample(iout, tcurr, y[0], y[3], dvdt, cur.ICaL + cur.Ito + cur.IKr + cur.IKs + cur.IK1, 0.0,
               cur.ICaL, ct, cv, B.pitch);
+    if (rc.dr) tn.sample(tcurr, y[0], y[3], tn_last, rc.v_apd90);
     if (B.vtrace && iout > trace_start && iout - trace_start <= B.tcap)
       B.vtrace[(size_t)(iout - trace_start - 1) * B.pitch + c] = y[0];
   }
@@ -611,6 +649,10 @@ tt_rl_kernel(TtBuffers B, RunCfg rc, int nbeats)
   B.iout[c] = iout;
   B.tcur[c] = tcurr;
   bm.store(B.bmd, B.bmi, B.pitch, c);
+  if (rc.dr) {
+    const double *t6 = &tn.v_prev;
+    for (int k = 0; k < 6; ++k) rc.dr[(size_t)k * B.pitch + c] = t6[k];
+  }
 }
 
 // ---- ORd 2011: the DrugSimulation loop (drug_sim.cpp:306-420) -- adaptive Rush-Larsen steps
@@ -1482,6 +1524,11 @@ struct cep_batch {
   int o11_waves = 0;             // CEP_OPT_ORD11_WAVES: 0 automatic, 1, 2 or 4 wavefronts per 64 cells
   double *d_aphi = nullptr, *d_avec = nullptr, *d_acoef = nullptr, *d_asc = nullptr;
   int *d_asi = nullptr;
+  // biomarker rules of the drugs_herg / benchpatch / population_tn2006 drivers
+  bool dr_on = false;
+  double v_apd90 = -40.0, v_apd50 = -40.0, t_first = 0.0;   // t_first 0: the first output at dt
+  double *d_dr = nullptr, *d_qin = nullptr;
+  int *d_qctl = nullptr;
   // device memory
   double *d_y = nullptr, *d_prm = nullptr, *d_bmd = nullptr, *d_ct = nullptr, *d_cv = nullptr;
   double *d_zn = nullptr, *d_ewt = nullptr, *d_acor = nullptr, *d_ft = nullptr, *d_jd = nullptr;
@@ -1748,6 +1795,20 @@ int cep_set_run_config(cep_batch *b, double dt, double rtol, double atol, int ma
 int cep_set_model_option(cep_batch *b, int option, double value)
 {
   if (!b) return CEP_E_ARG;
+  if (option == CEP_OPT_DRIVER_RULES || option == CEP_OPT_V_APD90) {
+    if (b->model != CEP_MODEL_ORD_CIPA && b->model != CEP_MODEL_TT06)
+      return fail(b, CEP_E_ARG, "driver rules are evaluated for ORd-CiPA and TT06");
+    if (option == CEP_OPT_DRIVER_RULES) b->dr_on = value != 0.0;
+    else b->v_apd90 = value;
+    return 0;
+  }
+  if (option == CEP_OPT_V_APD50 || option == CEP_OPT_FIRST_OUTPUT) {
+    if (b->model != CEP_MODEL_ORD_CIPA) return fail(b, CEP_E_ARG, "an ORd-CiPA option");
+    if (option == CEP_OPT_V_APD50) b->v_apd50 = value;
+    else if (value >= 0.0) b->t_first = value;
+    else return fail(b, CEP_E_ARG, "first output time");
+    return 0;
+  }
   if (option == CEP_OPT_TIME_SERIES) {
     if (b->model != CEP_MODEL_ORD_CIPA) return fail(b, CEP_E_ARG, "time series are recorded for ORd-CiPA");
     b->ts_on = value != 0.0;
@@ -1793,6 +1854,9 @@ static double max_bcl(const cep_batch *b)
 
 static int alloc_trace_buffers(cep_batch *b)
 {
+  if (b->dr_on && !b->d_dr) {
+    if (dalloc(b, &b->d_dr, (size_t)b->pitch * DR_SN)) return CEP_E_HIP;
+  }
   if (b->model == CEP_MODEL_ORD2011) {   // no Ca trace; the Adams path may keep a Vm trace
     b->cap = 1;
     const int beat_steps = (int)llround(max_bcl(b) / b->dt);
@@ -1834,6 +1898,7 @@ static int reset_integrators(cep_batch *b)
   HIPCHK(b, hipMemsetAsync(b->d_status, 0, P * sizeof(int), b->stream));
   HIPCHK(b, hipMemsetAsync(b->d_bmd, 0, P * BM_TOTAL_D * sizeof(double), b->stream));
   HIPCHK(b, hipMemsetAsync(b->d_bmi, 0, P * BM_I_N * sizeof(int), b->stream));
+  if (b->d_dr) HIPCHK(b, hipMemsetAsync(b->d_dr, 0, P * DR_SN * sizeof(double), b->stream));
   if (b->model == CEP_MODEL_ORD_CIPA) {
     HIPCHK(b, hipMemsetAsync(b->d_si, 0, P * ordbdf::SI_N * sizeof(int), b->stream));   // phase = FRESH
     // the integrator keeps Nordsieck rows above the current order at zero
@@ -1992,7 +2057,10 @@ int cep_run_beats_async(cep_batch *b, int nbeats)
   if (b->model == CEP_MODEL_ORD2011 && b->integrator == CEP_INTEGRATOR_ADAMS && !b->d_aphi)
     return fail(b, CEP_E_STATE, "the Adams integrator was selected after cep_init_consts: call cep_init_consts again");
   if (int rc = alloc_trace_buffers(b)) return rc;
-  RunCfg rc{b->dt, b->nbeats_total, b->nwin, (int)((1. / b->dt) * 1.0)};
+  if (b->dr_on && (b->model == CEP_MODEL_ORD_CIPA || b->model == CEP_MODEL_TT06) && b->nbeats_total < 2)
+    return fail(b, CEP_E_ARG, "the driver rules read beats n-2 and n-1: nbeats_total >= 2");
+  RunCfg rc{b->dt, b->nbeats_total, b->nwin, (int)((1. / b->dt) * 1.0), b->t_first > 0.0 ? b->t_first : b->dt,
+            b->v_apd90, b->v_apd50, b->dr_on ? b->d_dr : nullptr};
   if (rc.print_freq < 1) rc.print_freq = 1;
   HIPCHK(b, hipEventRecord(b->ev0, b->stream));
   if (b->model == CEP_MODEL_ORD_CIPA) {
@@ -2114,7 +2182,8 @@ long long cep_batch_device_bytes(int model, int ncells, double max_bcl, double d
   const long long cap = beat_steps / (print_freq > 0 ? print_freq : 1) + 4;
   d += 2 * P * cap;
   if (vm_trace) d += P * beat_steps;
-  const int nalloc = ord ? 14 : 10;
+  d += P * DR_SN;                                             // driver-rule state, if enabled
+  const int nalloc = ord ? 15 : 11;
   return d * 8 + i4 * 4 + nalloc * pad;
 }
 
@@ -2277,6 +2346,73 @@ int cep_get_dvmdt_repol_history(const cep_batch *b, int cell, int max_n, int *pa
   const int m = cnt < max_n ? cnt : max_n;
   if (paces) if (int rc = get_column(b, b->d_tsp, sizeof(int), cell, m, paces)) return rc;
   if (values) if (int rc = get_column(b, b->d_tsv, 8, cell, m, values)) return rc;
+  return 0;
+}
+
+int cep_get_driver_rules(const cep_batch *b, double *out)
+{
+  if (!b || !out) return CEP_E_ARG;
+  if (!b->d_dr) return fail(const_cast<cep_batch *>(b), CEP_E_STATE, "CEP_OPT_DRIVER_RULES was not on");
+  const size_t P = b->pitch;
+  std::vector<double> h(P * DR_SN);
+  HIPCHK(const_cast<cep_batch *>(b), hipMemcpyAsync(h.data(), b->d_dr, h.size() * sizeof(double),
+                                                  hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(const_cast<cep_batch *>(b), hipStreamSynchronize(b->stream));
+  for (int i = 0; i < b->n; ++i) {
+    double st[DR_SN];
+    for (int k = 0; k < DR_SN; ++k) st[k] = h[k * P + i];
+    if (b->model == CEP_MODEL_ORD_CIPA) {
+      DriverRules r;
+      memcpy(&r, st, sizeof(r));
+      r.record(out + (size_t)i * CEP_NDR);
+    } else {
+      TnRules r;
+      memcpy(&r, st, sizeof(r));
+      r.record(out + (size_t)i * CEP_NDR_TT06);
+    }
+  }
+  return 0;
+}
+
+}  // extern "C"
+
+// qinward of each cell against its control cell (drugs_herg.cpp:309-315), from the AUC windows
+__global__ void __launch_bounds__(64) qinward_kernel(int n, int pitch, const DriverRules *__restrict__ dummy,
+                                                    const double *__restrict__ dr, const int *__restrict__ ctl,
+                                                    double *__restrict__ out)
+{
+  (void)dummy;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const int k = ctl[c];
+  if (k < 0 || k >= n) { out[c] = __builtin_nan(""); return; }
+  auto row = [&](int slot, int cell) { return dr[(size_t)slot * pitch + cell]; };
+  constexpr int N0 = offsetof(DriverRules, dh_n0) / 8, N1 = offsetof(DriverRules, dh_n1) / 8;
+  constexpr int C0 = offsetof(DriverRules, dh_c0) / 8, C1 = offsetof(DriverRules, dh_c1) / 8;
+  const double inal_d = row(N1, c) - row(N0, c), ical_d = row(C1, c) - row(C0, c);
+  const double inal_c = row(N1, k) - row(N0, k), ical_c = row(C1, k) - row(C0, k);
+  out[c] = ((inal_d / inal_c) + (ical_d / ical_c)) * 0.5;
+}
+
+extern "C" {
+
+int cep_get_qinward(cep_batch *b, const int *control, double *out)
+{
+  if (!b || !control || !out) return CEP_E_ARG;
+  if (b->model != CEP_MODEL_ORD_CIPA) return fail(b, CEP_E_ARG, "qinward is an ORd-CiPA output");
+  if (!b->d_dr) return fail(b, CEP_E_STATE, "CEP_OPT_DRIVER_RULES was not on");
+  for (int i = 0; i < b->n; ++i)
+    if (control[i] >= b->n) return fail(b, CEP_E_ARG, "control cell index");
+  if (!b->d_qin) {
+    if (dalloc(b, &b->d_qin, (size_t)b->pitch) || dalloc(b, &b->d_qctl, (size_t)b->pitch)) return CEP_E_HIP;
+  }
+  HIPCHK(b, hipMemcpyAsync(b->d_qctl, control, b->n * sizeof(int), hipMemcpyHostToDevice, b->stream));
+  const int blocks = (b->n + 63) / 64;
+  hipLaunchKernelGGL(qinward_kernel, dim3(blocks), dim3(64), 0, b->stream, b->n, b->pitch, nullptr, b->d_dr,
+                     b->d_qctl, b->d_qin);
+  HIPCHK(b, hipGetLastError());
+  HIPCHK(b, hipMemcpyAsync(out, b->d_qin, b->n * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(b, hipStreamSynchronize(b->stream));
   return 0;
 }
 

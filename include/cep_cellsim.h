@@ -144,6 +144,23 @@ int cep_set_cells(cep_batch *b, int first, int count, const double *celltype, co
    same either way up to floating-point contraction.  No reference counterpart (a launch-shape
    choice). */
 #define CEP_OPT_ORD11_WAVES 5
+/* CEP_OPT_DRIVER_RULES (ORd-CiPA, TT06): nonzero also evaluates, at every output point, the
+   biomarker rules of the reference's other drivers (read with cep_get_driver_rules).  Enable before
+   cep_init_consts; nbeats_total of cep_set_run_config is the drivers' num_pace1 (>= 2).
+   ORd-CiPA: CEP7Cell/modules/drugs_herg.cpp:221-315 (APD from crossings of a fixed v_apd90, qnet
+   and AUCs as differences of STATES[qnet, INaL_AUC, ICaL_AUC] between outputs (BCL/dt)(n-2)+BCL
+   and (BCL/dt)(n-1)+BCL) and CEPPatch_Population_Drugs/benchpatch.cpp:320-432 (crossing APD90 and
+   APD50, Ca diastole, the previous beat's Ca peak, CaD90/CaD50 from threshold crossings, max dV/dt
+   and Vm peak of the last cycle, the final V).  TT06: CEP7Cell/modules/population_tn2006.cpp:
+   143-191 (crossing APD, mean Ca_i of the last beat).
+   CEP_OPT_V_APD90 (ORd-CiPA, TT06), CEP_OPT_V_APD50 (ORd-CiPA): the thresholds (mV; default -40,
+   the reference's default v_apd90, CEP7Cell/commons/helper.cpp:21).
+   CEP_OPT_FIRST_OUTPUT (ORd-CiPA): time of the first output point (ms; 0 = dt); benchpatch starts
+   its outputs at 0.01 (benchpatch.cpp:297). */
+#define CEP_OPT_DRIVER_RULES 6
+#define CEP_OPT_V_APD90 7
+#define CEP_OPT_V_APD50 8
+#define CEP_OPT_FIRST_OUTPUT 9
 int cep_set_model_option(cep_batch *b, int option, double value);
 
 /* Integration / biomarker configuration shared by the batch.
@@ -206,6 +223,34 @@ int cep_get_time_series(const cep_batch *b, int cell, int max_n, double *t, doub
 /* dvmdt_repol of every eligible pace of the window, in pace order (the reference's
    max_dvdt_repol file, cipa_drug_bench.cpp:262): *n entries, the first max_n copied. */
 int cep_get_dvmdt_repol_history(const cep_batch *b, int cell, int max_n, int *paces, double *values, int *n);
+
+/* driver-rule records (CEP_OPT_DRIVER_RULES on): ORd-CiPA n x CEP_NDR, TT06 n x CEP_NDR_TT06 */
+#define CEP_DR_DH_APD 0        /* drugs_herg: apd of the last repolarisation through v_apd90 */
+#define CEP_DR_DH_T_DEP 1      /*   its t_depolarize, t_repolarize, and the count of crossings */
+#define CEP_DR_DH_T_REP 2
+#define CEP_DR_DH_NREP 3
+#define CEP_DR_DH_QNET 4       /*   (qnet_curr - qnet_prev) / 1000 (drugs_herg.cpp:306) */
+#define CEP_DR_DH_INAL_AUC 5   /*   INaL_AUC and ICaL_AUC over the same window */
+#define CEP_DR_DH_ICAL_AUC 6
+#define CEP_DR_BP_QNET 7       /* benchpatch: qnet_tot, max_dvmdt, vm_peak, vm_dia, apd90, apd50, */
+#define CEP_DR_BP_MAX_DVMDT 8  /*   ca_peak, ca_dia, cad90, cad50 (benchpatch.cpp:420-432) */
+#define CEP_DR_BP_VM_PEAK 9
+#define CEP_DR_BP_VM_DIA 10
+#define CEP_DR_BP_APD90 11
+#define CEP_DR_BP_APD50 12
+#define CEP_DR_BP_CA_PEAK 13
+#define CEP_DR_BP_CA_DIA 14
+#define CEP_DR_BP_CAD90 15
+#define CEP_DR_BP_CAD50 16
+#define CEP_NDR 17
+#define CEP_DR_TN_APD 0        /* population_tn2006: ap_duration, mean_ca_i */
+#define CEP_DR_TN_MEAN_CAI 1
+#define CEP_NDR_TT06 2
+int cep_get_driver_rules(const cep_batch *b, double *out);
+/* qinward of every cell against control[i], the index of its concentration-0 cell in the same
+   batch (drugs_herg.cpp:309-315): ((INaL_d / INaL_c) + (ICaL_d / ICaL_c)) / 2 over the drugs_herg
+   AUC windows, computed on the device; control[i] < 0 gives NaN.  ORd-CiPA, driver rules on. */
+int cep_get_qinward(cep_batch *b, const int *control, double *out);
 
 /* average duration (ms) of the integration kernel over the launches since the last
    call, timed with HIP events on the batch stream; returns launches counted */

@@ -34,6 +34,20 @@ static const int ord_gslot[7] = { 45, 153, 156, 159, 160, 161, 154 };
 int oracle_ord_run(const ord_cell_cfg_t *cfg, double *states_io, int use_states_in,
                    bm_result_t *res, double *vm_trace, bdf_stats_t *st)
 {
+  return oracle_ord_run_rules(cfg, states_io, use_states_in, res, vm_trace, st, NULL);
+}
+
+/* stimulus of the single-cell drivers (ohara_rudy_cipa_v1_2017.cpp:950-951), for the RATES[V]
+   that benchpatch evaluates one output step ahead */
+static double ord_istim(const double *C, double t)
+{
+  const double start = C[12], end = C[13], amp = C[14], period = C[15], dur = C[16];
+  return (t >= start && t <= end && (t - start) - floor((t - start) / period) * period <= dur) ? amp : 0.0;
+}
+
+int oracle_ord_run_rules(const ord_cell_cfg_t *cfg, double *states_io, int use_states_in,
+                         bm_result_t *res, double *vm_trace, bdf_stats_t *st, double *rules)
+{
   ord_ctx_t ctx;
   double S[ORD_NS], R[ORD_NS];
   ord_init_consts(ctx.C, S, cfg->celltype);
@@ -52,7 +66,10 @@ int oracle_ord_run(const ord_cell_cfg_t *cfg, double *states_io, int use_states_
   /* Ca trace sampled every dt_write = 1 ms (reference default, CEP7Cell/commons/helper.cpp:13) */
   bm_state_t *b = bm_create(ctx.C[15], cfg->dt, ctx.C[12], cfg->nbeats, cfg->nwin, (int)((1. / cfg->dt) * 1.0));
   bm_begin(b, S[0], S[9]);
-  double tnext = cfg->dt;
+  dr_state_t dr;
+  dr_init(&dr, ctx.C[15], cfg->dt, cfg->nbeats, cfg->v_apd90, cfg->v_apd50);
+  dr_begin(&dr, S[0]);
+  double tnext = cfg->t_first > 0.0 ? cfg->t_first : cfg->dt;
   int iout = 0, rc = 0;
   while (iout < imax) {
     rc = bdf_advance(m, tnext, S);
@@ -64,9 +81,12 @@ int oracle_ord_run(const ord_cell_cfg_t *cfg, double *states_io, int use_states_
     const double *A = ctx.A;
     bm_sample(b, iout, tcurr, S[0], S[9], R[0],
               A[60] + A[83] + A[66] + A[90] + A[96] + A[98], A[60], A[83]);
+    if (rules)
+      dr_sample(&dr, iout, tcurr, S[0], S[9], &S[49], R[0] + ord_istim(ctx.C, tcurr) - ord_istim(ctx.C, tnext));
     if (vm_trace && iout > (cfg->nbeats - 1) * beat_steps) vm_trace[iout - 1 - (cfg->nbeats - 1) * beat_steps] = S[0];
   }
   if (res) bm_result(b, res);
+  if (rules) dr_result(&dr, rules);
   if (st) bdf_get_stats(m, st);
   memcpy(states_io, S, sizeof(S));
   bm_free(b);
@@ -79,6 +99,12 @@ static const int tt_gslot[10] = { 16, 13, 14, 15, 20, 18, 17, 19, 30, 32 };
 
 int oracle_tt_run(const tt_cell_cfg_t *cfg, double *states_io, int use_states_in,
                   bm_result_t *res, double *vm_trace)
+{
+  return oracle_tt_run_rules(cfg, states_io, use_states_in, res, vm_trace, -40.0, NULL);
+}
+
+int oracle_tt_run_rules(const tt_cell_cfg_t *cfg, double *states_io, int use_states_in,
+                        bm_result_t *res, double *vm_trace, double v_apd90, double *rules)
 {
   double C[TT_NC], S[TT_NS], R[TT_NS], A[TT_NA];
   tt_init_consts(C, S, cfg->celltype);
@@ -95,6 +121,8 @@ int oracle_tt_run(const tt_cell_cfg_t *cfg, double *states_io, int use_states_in
   const int beat_steps = (int)(C[6] / cfg->dt);
   bm_state_t *b = bm_create(C[6], cfg->dt, C[5], cfg->nbeats, cfg->nwin, (int)((1. / cfg->dt) * 1.0));
   bm_begin(b, S[0], S[3]);
+  tn_state_t tn;
+  tn_init(&tn, C[6], cfg->nbeats, v_apd90, S[0]);
   double tcurr = 0.0;
   int iout = 0;
   while (iout < imax) {
@@ -102,10 +130,12 @@ int oracle_tt_run(const tt_cell_cfg_t *cfg, double *states_io, int use_states_in
     tt_rush_larsen(cfg->dt, C, R, S, A);
     tcurr += cfg->dt;
     iout++;
+    if (rules) tn_sample(&tn, tcurr, S[0], S[3]);
     bm_sample(b, iout, tcurr, S[0], S[3], R[0], A[52] + A[54] + A[48] + A[49] + A[47], 0.0, A[52]);
     if (vm_trace && iout > (cfg->nbeats - 1) * beat_steps) vm_trace[iout - 1 - (cfg->nbeats - 1) * beat_steps] = S[0];
   }
   if (res) bm_result(b, res);
+  if (rules) tn_result(&tn, rules);
   memcpy(states_io, S, sizeof(S));
   bm_free(b);
   return 0;

@@ -142,12 +142,58 @@ void bm_sample(bm_state_t *b, int iout, double t, double v, double ca, double dv
                double qsum, double inal, double ical);
 void bm_result(const bm_state_t *b, bm_result_t *out);
 
+/* ---------------- biomarker rules of the drugs_herg, benchpatch and population_tn2006 drivers
+   (driver_rules.c); record layout shared with the library's cep_get_driver_rules */
+#define DR_DH_APD 0
+#define DR_DH_T_DEP 1
+#define DR_DH_T_REP 2
+#define DR_DH_NREP 3
+#define DR_DH_QNET 4
+#define DR_DH_INAL_AUC 5
+#define DR_DH_ICAL_AUC 6
+#define DR_BP_QNET 7
+#define DR_BP_MAX_DVMDT 8
+#define DR_BP_VM_PEAK 9
+#define DR_BP_VM_DIA 10
+#define DR_BP_APD90 11
+#define DR_BP_APD50 12
+#define DR_BP_CA_PEAK 13
+#define DR_BP_CA_DIA 14
+#define DR_BP_CAD90 15
+#define DR_BP_CAD50 16
+#define DR_N 17
+#define DR_TN_APD 0
+#define DR_TN_MEAN_CAI 1
+typedef struct {
+  double bcl, dt, v_apd90, v_apd50, v_prev;
+  int nbeats, imax, w_prev, w_curr, w_cadia;
+  double dh_t_dep, dh_t_rep, dh_apd, dh_q0, dh_n0, dh_c0, dh_q1, dh_n1, dh_c1;
+  int dh_nrep;
+  double bp_t_dep, bp_t_rep, bp_apd90, bp_apd50, bp_q0, bp_q1, bp_ca_dia, bp_ca_peak, bp_ca_amp;
+  double bp_t1_cad90, bp_cad90, bp_t1_cad50, bp_cad50, bp_max_dvmdt, bp_vm_peak, bp_vm_dia;
+  int bp_ca90_inc, bp_cad90_found, bp_ca50_inc, bp_cad50_found;
+} dr_state_t;
+void dr_init(dr_state_t *s, double bcl, double dt, int nbeats, double v_apd90, double v_apd50);
+void dr_begin(dr_state_t *s, double v0);
+void dr_sample(dr_state_t *s, int k, double t, double v, double ca, const double *acc, double dvdt_next);
+void dr_result(const dr_state_t *s, double *out);
+double dr_qinward(double inal_drug, double ical_drug, double inal_ctl, double ical_ctl);
+typedef struct {
+  double t_last, v_apd90, v_prev, t_dep, t_rep, apd, sum_ca;
+  int m;
+} tn_state_t;
+void tn_init(tn_state_t *s, double bcl, int nbeats, double v_apd90, double v0);
+void tn_sample(tn_state_t *s, double tcurr, double v, double ca);
+void tn_result(const tn_state_t *s, double *out);
+
 /* ---------------- drivers */
 typedef struct {
   double celltype, conc, bcl, stim_amp_scale, dt, rtol, atol;
   double herg[19];
   double gscale[7];   /* GNa, GNaL, PCa, GKr, GKs, GK1, Gto */
   int nbeats, nwin, maxord;
+  double v_apd90, v_apd50;   /* thresholds of the driver rules */
+  double t_first;            /* first output time (0: dt) */
 } ord_cell_cfg_t;
 
 /* Runs one ORd-CiPA cell through nbeats beats with the BDF, output every dt,
@@ -155,6 +201,9 @@ typedef struct {
    optional last-beat Vm trace (vm_trace[nsamp], nsamp = bcl/dt) and stats. */
 int oracle_ord_run(const ord_cell_cfg_t *cfg, double *states_io, int use_states_in,
                    bm_result_t *res, double *vm_trace, bdf_stats_t *st);
+/* the same run, also evaluating the driver rules (rules: DR_N doubles) */
+int oracle_ord_run_rules(const ord_cell_cfg_t *cfg, double *states_io, int use_states_in,
+                         bm_result_t *res, double *vm_trace, bdf_stats_t *st, double *rules);
 
 typedef struct {
   double bcl, dt, stim_start_unused;
@@ -165,6 +214,9 @@ typedef struct {
 } tt_cell_cfg_t;
 int oracle_tt_run(const tt_cell_cfg_t *cfg, double *states_io, int use_states_in,
                   bm_result_t *res, double *vm_trace);
+/* the same run with population_tn2006's rules at threshold v_apd90 (rules: 2 doubles) */
+int oracle_tt_run_rules(const tt_cell_cfg_t *cfg, double *states_io, int use_states_in,
+                        bm_result_t *res, double *vm_trace, double v_apd90, double *rules);
 
 /* threads: run ncell independent ORd cells round-robin over nthreads host threads */
 double oracle_ord_run_many(const ord_cell_cfg_t *cfgs, int ncell, int nthreads,

@@ -41,7 +41,31 @@ class OrdCellCfg(C.Structure):
     _fields_ = [("celltype", C.c_double), ("conc", C.c_double), ("bcl", C.c_double),
                 ("stim_amp_scale", C.c_double), ("dt", C.c_double), ("rtol", C.c_double),
                 ("atol", C.c_double), ("herg", C.c_double * 19), ("gscale", C.c_double * 7),
-                ("nbeats", C.c_int), ("nwin", C.c_int), ("maxord", C.c_int)]
+                ("nbeats", C.c_int), ("nwin", C.c_int), ("maxord", C.c_int),
+                ("v_apd90", C.c_double), ("v_apd50", C.c_double), ("t_first", C.c_double)]
+
+
+DR_FIELDS = ("dh_apd", "dh_t_dep", "dh_t_rep", "dh_nrep", "dh_qnet", "dh_inal_auc", "dh_ical_auc",
+             "bp_qnet", "bp_max_dvmdt", "bp_vm_peak", "bp_vm_dia", "bp_apd90", "bp_apd50", "bp_ca_peak",
+             "bp_ca_dia", "bp_cad90", "bp_cad50")
+DR_N = len(DR_FIELDS)
+
+
+class DrState(C.Structure):
+    _fields_ = [("bcl", C.c_double), ("dt", C.c_double), ("v_apd90", C.c_double), ("v_apd50", C.c_double),
+                ("v_prev", C.c_double), ("nbeats", C.c_int), ("imax", C.c_int), ("w_prev", C.c_int),
+                ("w_curr", C.c_int), ("w_cadia", C.c_int)] + [
+        (n, C.c_double) for n in ("dh_t_dep", "dh_t_rep", "dh_apd", "dh_q0", "dh_n0", "dh_c0", "dh_q1",
+                                  "dh_n1", "dh_c1")] + [("dh_nrep", C.c_int)] + [
+        (n, C.c_double) for n in ("bp_t_dep", "bp_t_rep", "bp_apd90", "bp_apd50", "bp_q0", "bp_q1",
+                                  "bp_ca_dia", "bp_ca_peak", "bp_ca_amp", "bp_t1_cad90", "bp_cad90",
+                                  "bp_t1_cad50", "bp_cad50", "bp_max_dvmdt", "bp_vm_peak", "bp_vm_dia")] + [
+        (n, C.c_int) for n in ("bp_ca90_inc", "bp_cad90_found", "bp_ca50_inc", "bp_cad50_found")]
+
+
+class TnState(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("t_last", "v_apd90", "v_prev", "t_dep", "t_rep", "apd", "sum_ca")] + [
+        ("m", C.c_int)]
 
 
 class TtCellCfg(C.Structure):
@@ -82,6 +106,19 @@ def lib():
         L.oracle_ord_run_many_from.argtypes = [C.POINTER(OrdCellCfg), C.c_int, C.c_int, dp,
                                                C.POINTER(BmResult)]
         L.oracle_ord_run_many_from.restype = C.c_double
+        L.oracle_ord_run_rules.argtypes = [C.POINTER(OrdCellCfg), dp, C.c_int, C.POINTER(BmResult), dp,
+                                           C.POINTER(BdfStats), dp]
+        L.oracle_tt_run_rules.argtypes = [C.POINTER(TtCellCfg), dp, C.c_int, C.POINTER(BmResult), dp,
+                                          C.c_double, dp]
+        L.dr_init.argtypes = [C.POINTER(DrState), C.c_double, C.c_double, C.c_int, C.c_double, C.c_double]
+        L.dr_begin.argtypes = [C.POINTER(DrState), C.c_double]
+        L.dr_sample.argtypes = [C.POINTER(DrState), C.c_int, C.c_double, C.c_double, C.c_double, dp, C.c_double]
+        L.dr_result.argtypes = [C.POINTER(DrState), dp]
+        L.dr_qinward.argtypes = [C.c_double] * 4
+        L.dr_qinward.restype = C.c_double
+        L.tn_init.argtypes = [C.POINTER(TnState), C.c_double, C.c_int, C.c_double, C.c_double]
+        L.tn_sample.argtypes = [C.POINTER(TnState), C.c_double, C.c_double, C.c_double]
+        L.tn_result.argtypes = [C.POINTER(TnState), dp]
         L.oracle_tt_run_many.argtypes = [C.POINTER(TtCellCfg), C.c_int, C.c_int,
                                          C.POINTER(BmResult), dp]
         L.oracle_tt_run_many.restype = C.c_double
@@ -161,8 +198,10 @@ def tt_rates(t, Cc, S, celltype):
 
 
 def make_ord_cfg(celltype=0, conc=0.0, herg=None, gscale=None, bcl=1000.0, nbeats=1, nwin=None,
-                 dt=0.1, rtol=1e-6, atol=1e-8, stim_amp_scale=1.0, maxord=5):
+                 dt=0.1, rtol=1e-6, atol=1e-8, stim_amp_scale=1.0, maxord=5, v_apd90=-40.0, v_apd50=-40.0,
+                 t_first=0.0):
     c = OrdCellCfg()
+    c.v_apd90 = v_apd90; c.v_apd50 = v_apd50; c.t_first = t_first
     c.celltype = celltype; c.conc = conc; c.bcl = bcl; c.stim_amp_scale = stim_amp_scale
     c.dt = dt; c.rtol = rtol; c.atol = atol; c.nbeats = nbeats
     c.nwin = nbeats if nwin is None else nwin; c.maxord = maxord
@@ -183,6 +222,26 @@ def ord_run(cfg: OrdCellCfg, states=None, trace=False):
     rc = lib().oracle_ord_run(C.byref(cfg), ptr(S), 0 if states is None else 1, C.byref(res),
                               ptr(vm) if trace else None, C.byref(st))
     return rc, S, res.as_dict(), vm, {n: getattr(st, n) for n, _ in st._fields_}
+
+
+def ord_run_rules(cfg: OrdCellCfg, states=None, trace=False):
+    """oracle_ord_run plus the driver rules: (rc, final states, biomarkers, Vm trace, rules dict)."""
+    S = np.zeros(52) if states is None else np.array(states, dtype=np.float64)
+    res = BmResult(); st = BdfStats()
+    vm = np.zeros(int(cfg.bcl / cfg.dt)) if trace else None
+    rules = np.zeros(DR_N)
+    rc = lib().oracle_ord_run_rules(C.byref(cfg), ptr(S), 0 if states is None else 1, C.byref(res),
+                                    ptr(vm) if trace else None, C.byref(st), ptr(rules))
+    return rc, S, res.as_dict(), vm, dict(zip(DR_FIELDS, rules))
+
+
+def tt_run_rules(cfg, v_apd90=-40.0):
+    """oracle_tt_run plus population_tn2006's rules: (final states, biomarkers, (apd, mean Ca_i))."""
+    S = np.zeros(19)
+    res = BmResult()
+    rules = np.zeros(2)
+    lib().oracle_tt_run_rules(C.byref(cfg), ptr(S), 0, C.byref(res), None, float(v_apd90), ptr(rules))
+    return S, res.as_dict(), rules
 
 
 def make_tt_cfg(celltype=0, conc=0.0, ic50=(0, 0, 0), hill=(0, 0, 0), gscale=None, bcl=1000.0,
