@@ -29,6 +29,7 @@ OPT_DRIVER_RULES = 6
 OPT_V_APD90 = 7
 OPT_V_APD50 = 8
 OPT_FIRST_OUTPUT = 9
+OPT_KT = 10
 INTEGRATOR_RUSH_LARSEN = 0
 INTEGRATOR_ADAMS = 1
 PREC_FP64 = 0
@@ -57,6 +58,7 @@ EXPORTED = (
     "cep_get_biomarkers", "cep_get_status", "cep_get_stats", "cep_enable_vm_trace",
     "cep_get_vm_trace", "cep_get_time_series", "cep_get_dvmdt_repol_history",
     "cep_kernel_time", "cep_query", "cep_batch_device_bytes", "cep_get_driver_rules", "cep_get_qinward",
+    "cep_integrate_bdf", "cep_step_rl",
     "cep_tissue_create", "cep_tissue_create_model", "cep_tissue_destroy", "cep_tissue_last_error", "cep_tissue_set_cells",
     "cep_tissue_set_conductivity", "cep_tissue_set_stimulus", "cep_tissue_step", "cep_tissue_diffuse",
     "cep_tissue_get_vm", "cep_tissue_get_states", "cep_tissue_set_states", "cep_tissue_time",
@@ -123,6 +125,8 @@ def library(path: str | None = None):
                                               C.POINTER(C.c_int)]
     L.cep_kernel_time.argtypes = [C.c_void_p, dp, dp]
     L.cep_get_driver_rules.argtypes = [C.c_void_p, dp]
+    L.cep_integrate_bdf.argtypes = [C.c_void_p, C.c_double]
+    L.cep_step_rl.argtypes = [C.c_void_p, C.c_double, C.c_double]
     L.cep_get_qinward.argtypes = [C.c_void_p, C.POINTER(C.c_int), dp]
     L.cep_query.argtypes = [C.c_void_p]
     L.cep_batch_device_bytes.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int]
@@ -300,8 +304,16 @@ class CellBatch:
         self._chk(self.L.cep_run_beats(self.h, nbeats))
 
     def run_outputs(self, nout: int):
-        """Adams integrator: advance every cell by nout output points (blocking)."""
+        """ORd-CiPA BDF / ORd 2011 Adams: advance every cell by nout output points (blocking)."""
         self._chk(self.L.cep_run_outputs(self.h, nout))
+
+    def integrate(self, tout: float):
+        """ORd-CiPA: CVode(tout) for every cell -- integrate to the output point tout."""
+        self._chk(self.L.cep_integrate_bdf(self.h, float(tout)))
+
+    def step_rl(self, t: float, dt: float):
+        """One fixed-dt step of every cell: computeRates(t) then solveAnalytical(dt)."""
+        self._chk(self.L.cep_step_rl(self.h, float(t), float(dt)))
 
     def run_beats_async(self, nbeats: int):
         self._chk(self.L.cep_run_beats_async(self.h, nbeats))

@@ -149,7 +149,15 @@ CEP_D void wave_sweeps(const ordbdf::Buffers &B, const ordbdf::BlockBufs &V, ord
   }
 }
 
-template <typename R, int CPW>
+// What a launch advances: LAUNCH_BEATS `n` beats of each cell's BCL, LAUNCH_OUTPUTS `n` further
+// output points, LAUNCH_TO_OUTPUT up to the absolute output point `n` (cells already there take
+// no step).  A template parameter rather than a run-time argument: the beat launch, the headline
+// one, keeps the code of a kernel that reads no launch argument besides n (run-time forms of this
+// choice gave FP64 builds whose Newton iterations failed from the first step; the three forms
+// are compared bitwise by tests/test_adapter.py).
+enum { LAUNCH_BEATS = 0, LAUNCH_OUTPUTS = 1, LAUNCH_TO_OUTPUT = 2 };
+
+template <typename R, int CPW, int MODE>
 __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunCfg rc, const int nbeats,
                                                      const long long max_passes)
 {
@@ -213,7 +221,12 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
     m.sw_dq = 0;
     m.sw_q1 = m.q;
     const int beat_steps = (int)(B.prm[ord::OP_BCL * B.pitch + c] / rc.dt);
-    m.iout_end = m.iout + nbeats * beat_steps;
+    if constexpr (MODE == LAUNCH_BEATS) {
+      m.iout_end = m.iout + nbeats * beat_steps;
+    } else {
+      m.iout_end = MODE == LAUNCH_OUTPUTS ? m.iout + nbeats : nbeats;
+      if (m.iout >= m.iout_end && ph != PH_INIT_F) { ph = PH_DONE; m.phase = ph; }
+    }
   }
   if (is_out) {
     Biomarker &bm = s_bm[slot];
@@ -226,7 +239,11 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
       if (fresh) s_dr[slot].begin(B.y[c]);
       else s_dr[slot].load(rc.dr, B.pitch, c);
     }
-    trace_start = iout0 + (nbeats - 1) * bm.beat_steps;
+    if constexpr (MODE == LAUNCH_BEATS) {
+      trace_start = iout0 + (nbeats - 1) * bm.beat_steps;
+    } else {                                       // the trace holds the launch's last beat_steps outputs
+      trace_start = (MODE == LAUNCH_OUTPUTS ? iout0 + nbeats : nbeats) - bm.beat_steps;
+    }
     s_opend[slot] = 0;
     s_ofail[slot] = 0;
   }
@@ -1311,6 +1328,7 @@ struct OrdCellIn {
   double celltype, conc, bcl;
   double herg[19];
   double gs[7];
+  double kt;            // drug trapping rate Kt, CONSTANTS[100] (0 in the reference's ___initConsts)
 };
 
 // per-cell parameters (celltype-derived constants :749-803, Hill block :826-842,
@@ -1344,7 +1362,7 @@ static void ord_params(const OrdCellIn &in, double stim_scale, double *p)
   const double Kmax = hg[14], Ku = hg[15], n = hg[16], halfmax = hg[17];
   p[OP_KU] = Ku;
   p[OP_BIND] = (Kmax * Ku * exp(n * log(D))) / (exp(n * log(D)) + halfmax);
-  p[OP_KT] = 0.0;
+  p[OP_KT] = in.kt;
   p[OP_VHALF] = hg[18];
   p[OP_STIM_START] = 3;
   p[OP_STIM_END] = 100000000000000000.0;
@@ -1388,7 +1406,7 @@ static void ord_constants(const OrdCellIn &in, double stim_scale, double *C)
   C[0] = in.celltype;
   C[12] = p[OP_STIM_START]; C[13] = p[OP_STIM_END]; C[14] = p[OP_STIM_AMP]; C[15] = p[OP_BCL]; C[16] = p[OP_STIM_DUR];
   C[45] = p[OP_GNA];
-  C[96] = in.herg[14]; C[97] = in.herg[15]; C[98] = in.herg[16]; C[99] = in.herg[17]; C[100] = 0; C[101] = in.herg[18];
+  C[96] = in.herg[14]; C[97] = in.herg[15]; C[98] = in.herg[16]; C[99] = in.herg[17]; C[100] = in.kt; C[101] = in.herg[18];
   C[149] = frt; C[150] = p[OP_CMDNMAX]; C[151] = Ahs; C[152] = thLp; C[153] = p[OP_GNAL]; C[154] = p[OP_GTO];
   C[155] = 0.6; C[156] = p[OP_PCA]; C[157] = 75; C[158] = 0; C[159] = p[OP_GKR]; C[160] = p[OP_GKS];
   C[161] = p[OP_GK1]; C[162] = vcell; C[163] = p[OP_GKB]; C[164] = 0; C[165] = 0; C[166] = a_rel; C[167] = btp;
@@ -1809,6 +1827,12 @@ int cep_set_model_option(cep_batch *b, int option, double value)
     else return fail(b, CEP_E_ARG, "first output time");
     return 0;
   }
+  if (option == CEP_OPT_KT) {
+    if (b->model != CEP_MODEL_ORD_CIPA) return fail(b, CEP_E_ARG, "Kt is an ORd-CiPA constant");
+    if (!(value >= 0.0)) return fail(b, CEP_E_ARG, "Kt must be >= 0");
+    for (auto &c : b->ord_in) c.kt = value;
+    return 0;
+  }
   if (option == CEP_OPT_TIME_SERIES) {
     if (b->model != CEP_MODEL_ORD_CIPA) return fail(b, CEP_E_ARG, "time series are recorded for ORd-CiPA");
     b->ts_on = value != 0.0;
@@ -2033,17 +2057,92 @@ static int launch_ord11_adams(cep_batch *b, int nbeats, int nout_fixed)
   return 0;
 }
 
+// the ORd-CiPA BDF launch: nbeats beats of each cell's BCL, or nout_rel further output points,
+// or up to the absolute output point iout_abs (exactly one of the three is positive)
+static void launch_ord_bdf(cep_batch *b, const RunCfg &rc, int nbeats, int nout_rel, int iout_abs)
+{
+  Buffers B;
+  B.ncell = b->n; B.pitch = b->pitch; B.y = b->d_y; B.zn = b->d_zn; B.ewt = b->d_ewt;
+  B.acor = b->d_acor; B.ft = b->d_ft; B.jd = b->d_jd; B.fac = b->d_fac; B.sd = b->d_sd;
+  B.si = b->d_si; B.prm = b->d_prm; B.bmd = b->d_bmd; B.bmi = b->d_bmi; B.ct = b->d_ct;
+  B.cv = b->d_cv; B.vtrace = b->vtrace_on ? b->d_vtrace : nullptr; B.tcap = b->tcap;
+  B.stats = b->d_stats; B.status = b->d_status; B.dt = b->dt; B.rtol = b->rtol; B.atol = b->atol;
+  B.uround_fd = b->prec == CEP_PREC_FP64 ? ordbdf::UROUND : 1.1920928955078125e-07;
+  B.maxord = b->maxord; B.max_steps = 500; B.cap = b->cap;
+  B.ts = TsView{};
+  if (b->ts_on && b->d_ts) {
+    const size_t sl = (size_t)b->pitch * 2 * b->ts_cap;   // one field, both slots
+    B.ts.t = b->d_ts; B.ts.vm = b->d_ts + sl; B.ts.ca = b->d_ts + 2 * sl; B.ts.ires = b->d_ts + 3 * sl;
+    B.ts.rp = b->d_tsp; B.ts.rv = b->d_tsv; B.ts.cap = b->ts_cap; B.ts.nwin = b->ts_nwin;
+  }
+  // safety net only: a step takes one pass when Newton converges at once, a few more on
+  // failures; cells still running after this many passes are flagged CEP_ST_MAX_STEPS
+  const long long nout = nbeats > 0 ? (long long)nbeats * (long long)(max_bcl(b) / b->dt)
+                                    : nout_rel > 0 ? nout_rel : iout_abs;
+  const long long max_passes = 64LL * nout + 100000;
+  constexpr int CPW = ordbdf::CPB;
+  const int blocks = (b->n + CPW - 1) / CPW;
+  const dim3 g(blocks), w(64);
+  const bool f64 = b->prec == CEP_PREC_FP64;
+  if (nbeats > 0) {
+    if (f64) hipLaunchKernelGGL((ord_bdf_kernel<double, CPW, LAUNCH_BEATS>), g, w, 0, b->stream, B, rc, nbeats, max_passes);
+    else hipLaunchKernelGGL((ord_bdf_kernel<float, CPW, LAUNCH_BEATS>), g, w, 0, b->stream, B, rc, nbeats, max_passes);
+  } else if (nout_rel > 0) {
+    if (f64) hipLaunchKernelGGL((ord_bdf_kernel<double, CPW, LAUNCH_OUTPUTS>), g, w, 0, b->stream, B, rc, nout_rel, max_passes);
+    else hipLaunchKernelGGL((ord_bdf_kernel<float, CPW, LAUNCH_OUTPUTS>), g, w, 0, b->stream, B, rc, nout_rel, max_passes);
+  } else {
+    if (f64) hipLaunchKernelGGL((ord_bdf_kernel<double, CPW, LAUNCH_TO_OUTPUT>), g, w, 0, b->stream, B, rc, iout_abs, max_passes);
+    else hipLaunchKernelGGL((ord_bdf_kernel<float, CPW, LAUNCH_TO_OUTPUT>), g, w, 0, b->stream, B, rc, iout_abs, max_passes);
+  }
+}
+
+static RunCfg run_cfg(const cep_batch *b)
+{
+  RunCfg rc{b->dt, b->nbeats_total, b->nwin, (int)((1. / b->dt) * 1.0), b->t_first > 0.0 ? b->t_first : b->dt,
+            b->v_apd90, b->v_apd50, b->dr_on ? b->d_dr : nullptr};
+  if (rc.print_freq < 1) rc.print_freq = 1;
+  return rc;
+}
+
 int cep_run_outputs(cep_batch *b, int nout)
 {
   if (!b || nout < 1) return fail(b, CEP_E_ARG, "nout");
-  if (b->model != CEP_MODEL_ORD2011 || b->integrator != CEP_INTEGRATOR_ADAMS)
-    return fail(b, CEP_E_STATE, "output-step runs are the Adams integrator's");
   if (b->cap == 0) return fail(b, CEP_E_STATE, "cep_init_consts not called");
+  if (b->model == CEP_MODEL_ORD_CIPA) {
+    if (int rc = alloc_trace_buffers(b)) return rc;
+    HIPCHK(b, hipEventRecord(b->ev0, b->stream));
+    launch_ord_bdf(b, run_cfg(b), 0, nout, 0);
+    HIPCHK(b, hipGetLastError());
+    HIPCHK(b, hipEventRecord(b->ev1, b->stream));
+    b->pending = true;
+    return cep_sync(b);
+  }
+  if (b->model != CEP_MODEL_ORD2011 || b->integrator != CEP_INTEGRATOR_ADAMS)
+    return fail(b, CEP_E_STATE, "output-step runs are the BDF's (ORd-CiPA) and the Adams integrator's");
   if (!b->d_aphi)
     return fail(b, CEP_E_STATE, "the Adams integrator was selected after cep_init_consts: call cep_init_consts again");
   if (int rc = alloc_trace_buffers(b)) return rc;
   HIPCHK(b, hipEventRecord(b->ev0, b->stream));
   if (int rc = launch_ord11_adams(b, 0, nout)) return rc;
+  HIPCHK(b, hipGetLastError());
+  HIPCHK(b, hipEventRecord(b->ev1, b->stream));
+  b->pending = true;
+  return cep_sync(b);
+}
+
+int cep_integrate_bdf(cep_batch *b, double tout)
+{
+  if (!b) return CEP_E_ARG;
+  if (b->model != CEP_MODEL_ORD_CIPA) return fail(b, CEP_E_ARG, "the BDF integrates ORd-CiPA cells");
+  if (b->cap == 0) return fail(b, CEP_E_STATE, "cep_init_consts not called");
+  const double t1 = b->t_first > 0.0 ? b->t_first : b->dt;
+  const double k = (tout - t1) / b->dt;
+  const long long kr = llround(k);
+  if (!(tout >= t1) || fabs(k - (double)kr) > 1e-6 || kr + 1 > 2000000000LL)
+    return fail(b, CEP_E_ARG, "tout must be an output point t_first + k dt");
+  if (int rc = alloc_trace_buffers(b)) return rc;
+  HIPCHK(b, hipEventRecord(b->ev0, b->stream));
+  launch_ord_bdf(b, run_cfg(b), 0, 0, (int)(kr + 1));
   HIPCHK(b, hipGetLastError());
   HIPCHK(b, hipEventRecord(b->ev1, b->stream));
   b->pending = true;
@@ -2059,34 +2158,10 @@ int cep_run_beats_async(cep_batch *b, int nbeats)
   if (int rc = alloc_trace_buffers(b)) return rc;
   if (b->dr_on && (b->model == CEP_MODEL_ORD_CIPA || b->model == CEP_MODEL_TT06) && b->nbeats_total < 2)
     return fail(b, CEP_E_ARG, "the driver rules read beats n-2 and n-1: nbeats_total >= 2");
-  RunCfg rc{b->dt, b->nbeats_total, b->nwin, (int)((1. / b->dt) * 1.0), b->t_first > 0.0 ? b->t_first : b->dt,
-            b->v_apd90, b->v_apd50, b->dr_on ? b->d_dr : nullptr};
-  if (rc.print_freq < 1) rc.print_freq = 1;
+  const RunCfg rc = run_cfg(b);
   HIPCHK(b, hipEventRecord(b->ev0, b->stream));
   if (b->model == CEP_MODEL_ORD_CIPA) {
-    Buffers B;
-    B.ncell = b->n; B.pitch = b->pitch; B.y = b->d_y; B.zn = b->d_zn; B.ewt = b->d_ewt;
-    B.acor = b->d_acor; B.ft = b->d_ft; B.jd = b->d_jd; B.fac = b->d_fac; B.sd = b->d_sd;
-    B.si = b->d_si; B.prm = b->d_prm; B.bmd = b->d_bmd; B.bmi = b->d_bmi; B.ct = b->d_ct;
-    B.cv = b->d_cv; B.vtrace = b->vtrace_on ? b->d_vtrace : nullptr; B.tcap = b->tcap;
-    B.stats = b->d_stats; B.status = b->d_status; B.dt = b->dt; B.rtol = b->rtol; B.atol = b->atol;
-    B.uround_fd = b->prec == CEP_PREC_FP64 ? ordbdf::UROUND : 1.1920928955078125e-07;
-    B.maxord = b->maxord; B.max_steps = 500; B.cap = b->cap;
-    B.ts = TsView{};
-    if (b->ts_on && b->d_ts) {
-      const size_t sl = (size_t)b->pitch * 2 * b->ts_cap;   // one field, both slots
-      B.ts.t = b->d_ts; B.ts.vm = b->d_ts + sl; B.ts.ca = b->d_ts + 2 * sl; B.ts.ires = b->d_ts + 3 * sl;
-      B.ts.rp = b->d_tsp; B.ts.rv = b->d_tsv; B.ts.cap = b->ts_cap; B.ts.nwin = b->ts_nwin;
-    }
-    // safety net only: a step takes one pass when Newton converges at once, a few more on
-    // failures; cells still running after this many passes are flagged CEP_ST_MAX_STEPS
-    const long long max_passes = 64LL * nbeats * (long long)(max_bcl(b) / b->dt) + 100000;
-    constexpr int CPW = ordbdf::CPB;
-    const int blocks = (b->n + CPW - 1) / CPW;
-    if (b->prec == CEP_PREC_FP64)
-      hipLaunchKernelGGL((ord_bdf_kernel<double, CPW>), dim3(blocks), dim3(64), 0, b->stream, B, rc, nbeats, max_passes);
-    else
-      hipLaunchKernelGGL((ord_bdf_kernel<float, CPW>), dim3(blocks), dim3(64), 0, b->stream, B, rc, nbeats, max_passes);
+    launch_ord_bdf(b, rc, nbeats, 0, 0);
   } else if (b->model == CEP_MODEL_ORD2011 && b->integrator == CEP_INTEGRATOR_ADAMS) {
     if (int r = launch_ord11_adams(b, nbeats, 0)) return r;
     b->beats_done += nbeats;
@@ -2433,3 +2508,51 @@ int cep_kernel_time(cep_batch *b, double *avg_ms, double *total_ms)
 #include "tissue.hpp"
 
 #include "io.hpp"
+
+// ---- one fixed-dt step of every cell: computeRates(t) then solveAnalytical(dt), the step of the
+// reference's fixed-dt loops (population_tn2006.cpp:145-155 for TT06, drug_sim.cpp:330-336 for
+// ORd 2011).  ORd-CiPA's solveAnalytical is empty in the reference (:1113-1115); its step is the
+// tissue reaction scheme (Rush-Larsen gates, backward-Euler Markov block, forward Euler rest).
+template <typename R>
+__global__ void __launch_bounds__(256) tt_step_kernel(int n, int pitch, double t, double dt, double *__restrict__ y,
+                                                      const double *__restrict__ prm)
+{
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  double s[tt::NS];
+#pragma unroll
+  for (int i = 0; i < tt::NS; ++i) s[i] = y[(size_t)i * pitch + c];
+  const PrmAcc P{prm + c, pitch};
+  double dvdt;
+  tt::Currents cur;
+  tt::rush_larsen<R>(t, dt, s, P, dvdt, &cur);
+#pragma unroll
+  for (int i = 0; i < tt::NS; ++i) y[(size_t)i * pitch + c] = s[i];
+}
+
+extern "C" {
+
+int cep_step_rl(cep_batch *b, double t, double dt)
+{
+  if (!b || !(dt > 0) || !(t >= 0)) return fail(b, CEP_E_ARG, "step arguments");
+  if (b->cap == 0) return fail(b, CEP_E_STATE, "cep_init_consts not called");
+  const bool f64 = b->prec == CEP_PREC_FP64;
+  if (b->model == CEP_MODEL_TT06) {
+    const int blocks = (b->n + 255) / 256;
+    if (f64) hipLaunchKernelGGL(tt_step_kernel<double>, dim3(blocks), dim3(256), 0, b->stream, b->n, b->pitch, t, dt, b->d_y, b->d_prm);
+    else hipLaunchKernelGGL(tt_step_kernel<float>, dim3(blocks), dim3(256), 0, b->stream, b->n, b->pitch, t, dt, b->d_y, b->d_prm);
+  } else if (b->model == CEP_MODEL_ORD2011) {
+    const int blocks = (b->n + 63) / 64;
+    if (f64) hipLaunchKernelGGL(cep::tis::tissue_react11<double>, dim3(blocks), dim3(64), 0, b->stream, b->n, b->pitch, t, dt, -1, b->d_y, b->d_prm);
+    else hipLaunchKernelGGL(cep::tis::tissue_react11<float>, dim3(blocks), dim3(64), 0, b->stream, b->n, b->pitch, t, dt, -1, b->d_y, b->d_prm);
+  } else {
+    const int blocks = (b->n + 63) / 64;
+    if (f64) hipLaunchKernelGGL(cep::tis::tissue_react<double>, dim3(blocks), dim3(64), 0, b->stream, b->n, b->pitch, t, dt, b->d_y, b->d_prm);
+    else hipLaunchKernelGGL(cep::tis::tissue_react<float>, dim3(blocks), dim3(64), 0, b->stream, b->n, b->pitch, t, dt, b->d_y, b->d_prm);
+  }
+  HIPCHK(b, hipGetLastError());
+  HIPCHK(b, hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+}  // extern "C"

@@ -161,6 +161,10 @@ int cep_set_cells(cep_batch *b, int first, int count, const double *celltype, co
 #define CEP_OPT_V_APD90 7
 #define CEP_OPT_V_APD50 8
 #define CEP_OPT_FIRST_OUTPUT 9
+/* CEP_OPT_KT (ORd-CiPA): the drug trapping rate Kt of the Markov IKr block (1/ms), CONSTANTS[100]
+   of ohara_rudy_cipa_v1_2017.cpp (0 in its ___initConsts, :690, and in every reference driver);
+   applies to every cell of the batch; call before cep_init_consts. */
+#define CEP_OPT_KT 10
 int cep_set_model_option(cep_batch *b, int option, double value);
 
 /* Integration / biomarker configuration shared by the batch.
@@ -187,9 +191,22 @@ int cep_compute_rates(cep_batch *b, double t, const double *states, double *rate
    updated on the device).  Blocks until done.  The _async form returns after the launch on
    the batch stream; cep_sync waits for it. */
 int cep_run_beats(cep_batch *b, int nbeats);
-/* Adams integrator only: advance every cell by nout output points (the reference loop's
-   ode() calls, CEPCell_HACKATHON/modules/cipa_drug_shago_bench.cpp:132-140).  Blocks. */
+/* Advance every cell by nout output points of dt.  ORd-CiPA: the BDF (a CVode call per output
+   point, biomarkers sampled at each); ORd 2011 with the Adams integrator: the reference loop's
+   ode() calls (CEPCell_HACKATHON/modules/cipa_drug_shago_bench.cpp:132-140).  Blocks. */
 int cep_run_outputs(cep_batch *b, int nout);
+/* ORd-CiPA: CVode(cvode_mem, tout, ...) in CV_NORMAL mode for every cell (CEP7Cell/modules/
+   drugs_herg.cpp:221-229): integrate to the output point tout = t_first + k dt (t_first =
+   CEP_OPT_FIRST_OUTPUT, default dt), sampling the biomarkers at every output point on the way.
+   Cells already at or past tout are left as they are.  Blocks. */
+int cep_integrate_bdf(cep_batch *b, double tout);
+/* One fixed-dt step of every cell at time t: computeRates(t) then solveAnalytical(dt) (TT06,
+   ten_tusscher_model_2006_IK1Ko_endo_units.cpp:272-404 as population_tn2006.cpp:145-155 calls
+   them; ORd 2011, Ohara_Rudy_2011.cpp:742-1095 with the single-cell stimulus).  ORd-CiPA has an
+   empty solveAnalytical (ohara_rudy_cipa_v1_2017.cpp:1113-1115); its step here is Rush-Larsen for
+   the gates, backward Euler for the Markov IKr block and forward Euler for the rest.  No
+   biomarkers; the states stay on the device (cep_get_states).  Blocks. */
+int cep_step_rl(cep_batch *b, double t, double dt);
 int cep_run_beats_async(cep_batch *b, int nbeats);
 int cep_sync(cep_batch *b);
 /* Non-blocking completion test of the last cep_run_beats_async: 1 while it runs, 0 when
