@@ -34,6 +34,7 @@ INTEGRATOR_RUSH_LARSEN = 0
 INTEGRATOR_ADAMS = 1
 PREC_FP64 = 0
 PREC_MIXED = 1
+STATIC_RATES_MAX_CELLS = 2048
 
 BM_FIELDS = ("qnet", "inal_auc", "ical_auc", "dvmdt_repol", "dvmdt_max", "vm_peak", "vm_valley",
              "vm_dia", "apd90", "apd50", "ca_peak", "ca_valley", "ca_dia", "cad90", "cad50",
@@ -58,7 +59,7 @@ EXPORTED = (
     "cep_get_biomarkers", "cep_get_status", "cep_get_stats", "cep_enable_vm_trace",
     "cep_get_vm_trace", "cep_get_time_series", "cep_get_dvmdt_repol_history",
     "cep_kernel_time", "cep_query", "cep_batch_device_bytes", "cep_get_driver_rules", "cep_get_qinward",
-    "cep_integrate_bdf", "cep_step_rl",
+    "cep_integrate_bdf", "cep_step_rl", "cep_compute_rates_static",
     "cep_tissue_create", "cep_tissue_create_model", "cep_tissue_destroy", "cep_tissue_last_error", "cep_tissue_set_cells",
     "cep_tissue_set_conductivity", "cep_tissue_set_stimulus", "cep_tissue_step", "cep_tissue_diffuse",
     "cep_tissue_get_vm", "cep_tissue_get_states", "cep_tissue_set_states", "cep_tissue_time",
@@ -109,6 +110,7 @@ def library(path: str | None = None):
     L.cep_run_outputs.argtypes = [C.c_void_p, C.c_int]
     L.cep_get_constants.argtypes = [C.c_void_p, C.c_int, dp]
     L.cep_compute_rates.argtypes = [C.c_void_p, C.c_double, dp, dp, dp]
+    L.cep_compute_rates_static.argtypes = [C.c_void_p, C.c_double, dp, dp, dp]
     L.cep_run_beats.argtypes = [C.c_void_p, C.c_int]
     L.cep_run_beats_async.argtypes = [C.c_void_p, C.c_int]
     L.cep_sync.argtypes = [C.c_void_p]
@@ -288,6 +290,15 @@ class CellBatch:
         r = np.zeros_like(s)
         a = np.zeros((self.n, self.nalg)) if algebraic else None
         self._chk(self.L.cep_compute_rates(self.h, float(t), _p(s), _p(r), _p(a)))
+        return (r, a) if algebraic else r
+
+    def compute_rates_static(self, t: float, states: np.ndarray, algebraic: bool = False):
+        """compute_rates through the library's fixed device array (no device allocation; at most
+        STATIC_RATES_MAX_CELLS cells)."""
+        s = np.ascontiguousarray(states, dtype=np.float64).reshape(self.n, self.nstates)
+        r = np.zeros_like(s)
+        a = np.zeros((self.n, self.nalg)) if algebraic else None
+        self._chk(self.L.cep_compute_rates_static(self.h, float(t), _p(s), _p(r), _p(a)))
         return (r, a) if algebraic else r
 
     def states(self) -> np.ndarray:

@@ -2556,3 +2556,84 @@ int cep_step_rl(cep_batch *b, double t, double dt)
 }
 
 }  // extern "C"
+
+// ---- computeRates through a fixed device array (no allocation on the call path)
+//
+// The staging of cep_compute_rates_static -- the states, rates and ALGEBRAIC rows of up to
+// kStaticRatesCells cells of the widest model (ORd-CiPA: 52 + 52 + 200 doubles a cell), 4.98 MB --
+// lives in one file-scope device array.  Each GPU the library runs on holds its own copy until the
+// process ends.  Host code uses the array only through the device address hipGetSymbolAddress
+// gives for the current device (the array's name is a host-side stand-in there), looked up on
+// the first call on each device.  Every call on a GPU shares that GPU's array: calls that overlap
+// on one GPU (from several host threads or streams) race.
+namespace staticrates {
+constexpr int kStaticRatesCells = 2048;
+constexpr size_t kRowDoubles = (size_t)ord::NS + ord::NS + ORD_NALG;   // the widest model's row
+constexpr size_t kScratchDoubles = (size_t)kStaticRatesCells * kRowDoubles;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+static std::mutex mu;
+static double *dev_ptr[kMaxDevices];
+
+static int scratch_for_current_device(cep_batch *b, double **out)
+{
+  int dev = 0;
+  HIPCHK(b, hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(b, CEP_E_ARG, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(mu);
+  if (!dev_ptr[dev]) {
+    double *p = nullptr;
+    HIPCHK(b, hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(b, CEP_E_HIP, "hipGetSymbolAddress gave no address for the scratch array");
+    dev_ptr[dev] = p;
+  }
+  *out = dev_ptr[dev];
+  return 0;
+}
+
+__global__ void __launch_bounds__(256) fill_zero(double *__restrict__ p, size_t n)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+}  // namespace staticrates
+
+extern "C" {
+
+int cep_compute_rates_static(cep_batch *b, double t, const double *states, double *rates, double *algebraic)
+{
+  using namespace staticrates;
+  if (!b || !states || !rates) return fail(b, CEP_E_ARG, "null array");
+  if (b->n > kStaticRatesCells)
+    return fail(b, CEP_E_ARG, "cep_compute_rates_static takes at most 2048 cells (its fixed device array); "
+                              "use cep_compute_rates for larger batches");
+  double *base = nullptr;
+  if (int rc = scratch_for_current_device(b, &base)) return rc;
+  const size_t nsz = (size_t)b->n * b->ns, asz = (size_t)b->n * b->nalg;
+  double *dy = base, *df = base + nsz, *da = algebraic ? base + 2 * nsz : nullptr;
+  if (da) {   // the kernels write only the ALGEBRAIC slots they compute: the rest must read zero
+    hipLaunchKernelGGL(fill_zero, dim3((unsigned)((asz + 255) / 256)), dim3(256), 0, b->stream, da, asz);
+    HIPCHK(b, hipGetLastError());
+  }
+  HIPCHK(b, hipMemcpyAsync(dy, states, nsz * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  constexpr int kRatesBlock = 64;
+  const int blocks = (b->n + kRatesBlock - 1) / kRatesBlock;
+  const bool f64 = b->prec == CEP_PREC_FP64;
+  if (b->model == CEP_MODEL_ORD_CIPA) {
+    if (f64) hipLaunchKernelGGL(ord_rates_kernel<double>, dim3(blocks), dim3(kRatesBlock), 0, b->stream, b->n, b->pitch, t, dy, df, da, b->d_prm);
+    else hipLaunchKernelGGL(ord_rates_kernel<float>, dim3(blocks), dim3(kRatesBlock), 0, b->stream, b->n, b->pitch, t, dy, df, da, b->d_prm);
+  } else if (b->model == CEP_MODEL_ORD2011) {
+    if (f64) hipLaunchKernelGGL(ord11_rates_kernel<double>, dim3(blocks), dim3(kRatesBlock), 0, b->stream, b->n, b->pitch, t, dy, df, da, b->d_prm);
+    else hipLaunchKernelGGL(ord11_rates_kernel<float>, dim3(blocks), dim3(kRatesBlock), 0, b->stream, b->n, b->pitch, t, dy, df, da, b->d_prm);
+  } else {
+    if (f64) hipLaunchKernelGGL(tt_rates_kernel<double>, dim3(blocks), dim3(kRatesBlock), 0, b->stream, b->n, b->pitch, t, dy, df, da, b->d_prm);
+    else hipLaunchKernelGGL(tt_rates_kernel<float>, dim3(blocks), dim3(kRatesBlock), 0, b->stream, b->n, b->pitch, t, dy, df, da, b->d_prm);
+  }
+  HIPCHK(b, hipGetLastError());
+  HIPCHK(b, hipMemcpyAsync(rates, df, nsz * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  if (da) HIPCHK(b, hipMemcpyAsync(algebraic, da, asz * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(b, hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+}  // extern "C"

@@ -187,6 +187,15 @@ int cep_get_constants(const cep_batch *b, int cell, double *out);
    (reference ALGEBRAIC indices; may be null).  Evaluated on the device. */
 int cep_compute_rates(cep_batch *b, double t, const double *states, double *rates, double *algebraic);
 
+/* cep_compute_rates without any device allocation on the call: no counterpart in the reference.
+   For callers that cannot allocate device memory (code under HIP graph capture, tight serving
+   loops, harnesses that forbid hipMalloc).  The states / rates / ALGEBRAIC staging is a fixed
+   device array of the library (4.98 MB on each GPU it runs on, held until the process ends), so a
+   batch may have at most CEP_STATIC_RATES_MAX_CELLS cells (CEP_E_ARG beyond).  Every call on one
+   GPU shares that array: calls that overlap on one GPU race.  Same results as cep_compute_rates. */
+#define CEP_STATIC_RATES_MAX_CELLS 2048
+int cep_compute_rates_static(cep_batch *b, double t, const double *states, double *rates, double *algebraic);
+
 /* Advance every cell by nbeats beats of its own BCL (device-resident state, biomarkers
    updated on the device).  Blocks until done.  The _async form returns after the launch on
    the batch stream; cep_sync waits for it. */
