@@ -89,6 +89,24 @@ CEP_HD constexpr bool couples(int row, int g)
   }
 }
 
+CEP_HD constexpr bool is_simple_gate(int s) { return (s >= 10 && s <= 32) || (s >= 44 && s <= 46); }
+CEP_HD constexpr bool is_core_state(int s) { return (s >= 0 && s <= 9) || s == 47 || s == 48; }
+
+// whether the factorisation or the solve reads J(row, col): the FD Jacobian's structural pattern
+// as the structured LU uses it (gate rows: diagonal and V; nca: nca, jca, cass; the Markov block
+// with its V column; core and accumulator rows: the core columns and their coupled gates, nca
+// and O).  The finite-difference columns store only these entries.
+CEP_HD constexpr bool jac_read(int row, int col)
+{
+  if (is_simple_gate(row)) return col == row || col == 0;
+  if (row == 33) return col == 33 || col == 30 || col == 2;
+  if (row >= 34 && row <= 42) return (col >= 34 && col <= 42) || col == 0;
+  if (is_core_state(row) || (row >= 49 && row <= 51))
+    return is_core_state(col) || (is_simple_gate(col) && couples(row, col)) || (col == 33 && couples(row, 33)) ||
+           (col == 38 && couples(row, 38));
+  return false;
+}
+
 // entries fetch_couplings gathers for one row: 12 core columns (optional), the coupled simple
 // gates, nca and O
 CEP_HD constexpr int ncouplings(int row, bool with_core)

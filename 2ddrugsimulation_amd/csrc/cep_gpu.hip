@@ -586,7 +586,15 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
       double ft[N], fv[N];
       for (int i = 0; i < N; ++i) ft[i] = Lo.FT(i);
       for (int i = 0; i < N; ++i) fv[i] = f[i];
-      for (int i = 0; i < N; ++i) Lo.JD_lane((unsigned)jcol, i) = iinv * fv[i] - iinv * ft[i];
+      // only the entries the factorisation reads reach memory: the others get an offset past the
+      // block's buffer range, where the hardware's range check drops the store (no branches)
+      const int kst = jstate(jcol);
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        BElem e = Lo.JD_lane((unsigned)jcol, i);
+        e.voff += jac_read(i, kst) ? 0u : 0x40000000u;
+        e = iinv * fv[i] - iinv * ft[i];
+      }
     }
     // integrators that need a fresh Jacobian queue their 48 columns
     const unsigned long long pubm = __ballot(publish);
