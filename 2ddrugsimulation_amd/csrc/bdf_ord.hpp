@@ -674,6 +674,81 @@ struct Lane {
     return n;
   }
 
+  // number of coupling entries (simple gates, nca, O) of the core rows kCore[A0..A1)
+  template <int A0, int A1>
+  CEP_HD static constexpr int ncoupl_rows()
+  {
+    int n = 0;
+    for (int a = A0; a < A1; ++a) n += ncouplings(kCore[a], false);
+    return n;
+  }
+  // bk[a] += g * J(row, gate) * beta_gate + ... for the core rows kCore[A0..A1): every entry of the
+  // rows fetched in one batch, then the sums in the order of the rows
+  template <int A0, int A1>
+  CEP_D void core_couplings(double (&bk)[12], const double (&beta)[NG], double beta_n, double bO, double g) const
+  {
+    constexpr int NB = ncoupl_rows<A0, A1>();
+    static_assert(NB > 0 && NB <= NCPL_MAX, "coupling batch size");
+    double jr[NB];
+    int n = 0;
+#pragma unroll
+    for (int a = A0; a < A1; ++a) {
+      const int row = kCore[a];
+#pragma unroll
+      for (int k = 0; k < NG; ++k)
+        if (couples(row, kSimpleGates[k])) jr[n++] = J(row, kSimpleGates[k]);
+      if (couples(row, 33)) jr[n++] = J(row, 33);
+      if (couples(row, 38)) jr[n++] = J(row, 38);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    n = 0;
+#pragma unroll
+    for (int a = A0; a < A1; ++a) {
+      const int row = kCore[a];
+      double s = bk[a];
+#pragma unroll
+      for (int k = 0; k < NG; ++k)
+        if (couples(row, kSimpleGates[k])) s += g * jr[n++] * beta[k];
+      if (couples(row, 33)) s += g * jr[n++] * beta_n;
+      if (couples(row, 38)) s += g * jr[n++] * bO;
+      bk[a] = s;
+    }
+  }
+  // accumulator rows R0..R1-1 (qnet, INaL and ICaL AUC): x_r = b_r + g * J(r, :) x, the rows'
+  // entries fetched in one batch
+  template <int R0, int R1, typename BV>
+  CEP_D void acc_rows(const BV &b, double g) const
+  {
+    constexpr int NB = [] { int n = 0; for (int r = R0; r < R1; ++r) n += ncouplings(r, true); return n; }();
+    static_assert(NB <= NCPL_MAX, "accumulator batch size");
+    double jr[NB];
+    int n = 0;
+#pragma unroll
+    for (int r = R0; r < R1; ++r) {
+#pragma unroll
+      for (int a = 0; a < 12; ++a) jr[n++] = J(r, kCore[a]);
+#pragma unroll
+      for (int k = 0; k < NG; ++k)
+        if (couples(r, kSimpleGates[k])) jr[n++] = J(r, kSimpleGates[k]);
+      if (couples(r, 33)) jr[n++] = J(r, 33);
+      if (couples(r, 38)) jr[n++] = J(r, 38);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    n = 0;
+#pragma unroll
+    for (int r = R0; r < R1; ++r) {
+      double s = 0.0;
+#pragma unroll
+      for (int a = 0; a < 12; ++a) s += jr[n++] * b[kCore[a]];
+#pragma unroll
+      for (int k = 0; k < NG; ++k)
+        if (couples(r, kSimpleGates[k])) s += jr[n++] * b[kSimpleGates[k]];
+      if (couples(r, 33)) s += jr[n++] * b[33];
+      if (couples(r, 38)) s += jr[n++] * b[38];
+      b[r] += g * s;
+    }
+  }
+
   // solve M x = b in place (b: a lane vector, here the LDS staging vector), then CVODE's
   // 2/(1+gamrat) scaling.  Factors and Jacobian entries are fetched in batches.
   template <typename BV>
@@ -724,22 +799,15 @@ struct Lane {
         bm[i] /= d;
       }
     }
+    // the core rows' right-hand sides: the coupling entries of two rows per batch (one memory
+    // round trip each; CaMKt and the SR / Ca_i rows have none)
     double bk[12];
 #pragma unroll
-    for (int a = 0; a < 12; ++a) {
-      const int row = kCore[a];
-      double s = b[row];
-      double jr[NCPL_MAX];
-      const int nj = fetch_couplings(jr, row, false);
-      (void)nj;
-      int n = 0;
-#pragma unroll
-      for (int k = 0; k < NG; ++k)
-        if (couples(row, kSimpleGates[k])) s += g * jr[n++] * beta[k];
-      if (couples(row, 33)) s += g * jr[n++] * beta_n;
-      if (couples(row, 38)) s += g * jr[n++] * bm[4];
-      bk[a] = s;
-    }
+    for (int a = 0; a < 12; ++a) bk[a] = b[kCore[a]];
+    core_couplings<0, 3>(bk, beta, beta_n, bm[4], g);    // CaMKt, cass, nai
+    core_couplings<3, 5>(bk, beta, beta_n, bm[4], g);    // nass, ki
+    core_couplings<5, 10>(bk, beta, beta_n, bm[4], g);   // kss, cansr, cajsr, cai, Jrel np
+    core_couplings<10, 12>(bk, beta, beta_n, bm[4], g);  // Jrel p, V
     {
       // core LU: rows 1-11 of L in two batches, then U rows 11..0 in two
       double lo[38];
@@ -811,23 +879,9 @@ struct Lane {
 #pragma unroll
       for (int i = 0; i < 9; ++i) b[34 + i] = bm[i] - w[i] * xV;
     }
-    // accumulators: x = b + gamma * J(acc, :) x
-#pragma unroll
-    for (int r = 49; r <= 51; ++r) {
-      double jr[NCPL_MAX];
-      const int nj = fetch_couplings(jr, r, true);
-      (void)nj;
-      int n = 0;
-      double s = 0.0;
-#pragma unroll
-      for (int a = 0; a < 12; ++a) s += jr[n++] * b[kCore[a]];
-#pragma unroll
-      for (int k = 0; k < NG; ++k)
-        if (couples(r, kSimpleGates[k])) s += jr[n++] * b[kSimpleGates[k]];
-      if (couples(r, 33)) s += jr[n++] * b[33];
-      if (couples(r, 38)) s += jr[n++] * b[38];
-      b[r] += g * s;
-    }
+    // accumulators: x = b + gamma * J(acc, :) x, row 49 then rows 50 and 51 in one batch
+    acc_rows<49, 50>(b, g);
+    acc_rows<50, 52>(b, g);
     if (m.gamrat != 1.0) {
       const double sc = 2.0 / (1.0 + m.gamrat);
 #pragma unroll
