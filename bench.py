@@ -58,6 +58,7 @@ F_SETUP, F_NEWTON, F_STEP = 2700, 1450, 1100
 PEAK_FP32 = 157.3e12
 PEAK_FP64 = 78.6e12
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "ord_bdf_population_10k_traffic.txt")
+VALU_FILE = os.path.join(ROOT, "profiles", "cell_kernels_valu_counters.txt")
 
 
 def log(*a):
@@ -92,16 +93,37 @@ def flops_of(stats: np.ndarray) -> float:
     return float(np.sum((nfe + nout) * F_RHS + nsetups * F_SETUP + nni * F_NEWTON + nst * F_STEP))
 
 
+def read_profile(path):
+    """The `key = value` lines of a committed rocprofv3 summary ('#' lines are commentary)."""
+    kv = {}
+    for line in open(path):
+        if "=" in line and not line.startswith("#"):
+            k, v = line.split("=", 1)
+            kv[k.strip()] = v.split("#")[0].strip()
+    return kv
+
+
 def read_traffic(path=TRAFFIC_FILE, keys=("hbm_bytes_per_beat_launch",)):
     """HBM bytes per launch of a kernel from a committed rocprofv3 PMC summary (FETCH_SIZE +
     WRITE_SIZE, scaled by the copy-kernel calibration): the sum of the named entries, or None."""
     try:
-        kv = {}
-        for line in open(path):
-            if "=" in line and not line.startswith("#"):
-                k, v = line.split("=", 1)
-                kv[k.strip()] = v.split("#")[0].strip()
+        kv = read_profile(path)
         return float(sum(float(kv[k]) for k in keys))
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def read_valu(prefix: str, path=VALU_FILE):
+    """VALU-busy, active-lane, issued FP32/FP64 and transcendental figures of one cell kernel from
+    the committed rocprofv3 counter summary (profile-derived, not measured by this run), or None."""
+    try:
+        kv = read_profile(path)
+        out = {k: (float(kv[f"{prefix}_{k}"]) if f"{prefix}_{k}" in kv else None)
+               for k in ("valu_busy_pct", "valu_utilization_pct", "fp32_issued_tflops",
+                         "fp64_issued_tflops", "trans_frac")}
+        out["kernel"] = kv[f"{prefix}_kernel"]
+        out["source"] = "rocprofv3 SQ counters, profiles/" + os.path.basename(path)
+        return out
     except (OSError, KeyError, ValueError):
         return None
 
@@ -463,7 +485,8 @@ def tt06_main(args):
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None, "kernel": f"tt_rl_kernel<{'float' if prec == cep.PREC_MIXED else 'double'}>",
                      "kernel_avg_ms": kavg_ms, "flops_per_launch": flops_launch,
-                     "flops_note": "534 counted flops per cell-step; 63 exp, 4 log, 27 pow per step not counted"},
+                     "flops_note": "534 counted flops per cell-step; 63 exp, 4 log, 27 pow per step not counted",
+                     "valu_counters": read_valu("tt") if prec == cep.PREC_MIXED else None},
         "cpu_baseline": cpu,
     }), flush=True)
 
@@ -639,7 +662,8 @@ def ord11_main(args):
                        "kernel_avg_ms": pavg},
         "roofline": {"bound": "valu", "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
                      "frac": ach / peak, "traffic": None, "kernel": "ord11_rl_kernel (population)",
-                     "flops_per_step": f_step},
+                     "flops_per_step": f_step,
+                     "valu_counters": read_valu("o11") if prec == cep.PREC_FP64 else None},
         "cpu_baseline": None if args.no_cpu_baseline else ord11_cpu_baseline(q, s0),
     }
     print(json.dumps(out), flush=True)
@@ -791,11 +815,13 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / peak,
                      "traffic": traffic,
-                     "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE of the same launch, "
+                     "traffic_source": "profile-derived: committed rocprofv3 FETCH_SIZE/WRITE_SIZE "
+                                       "summary of this workload's launch, "
                                        "profiles/ord_bdf_population_10k_traffic.txt",
                      "kernel": f"ord_bdf_kernel<{'float' if prec == cep.PREC_MIXED else 'double'}, 16>",
                      "kernel_avg_ms": kavg_ms, "flops_per_launch": flops_launch,
-                     "rhs_evals_per_launch": nrhs / max(nl, 1)},
+                     "rhs_evals_per_launch": nrhs / max(nl, 1),
+                     "valu_counters": read_valu("bdf") if prec == cep.PREC_MIXED else None},
         "other_workloads": legs,
         "cpu_baseline": cpu,
     }
@@ -826,7 +852,8 @@ def compact_legs(args):
     legs["tt06"] = {"workload": f"TT06 {n} cells, Rush-Larsen dt 0.02 ms, mixed, beat 2",
                     "cell_beats_per_s": n / (t1 - t0), "kernel_avg_ms": kavg, "failed_cells": bad,
                     "roofline": {"bound": "valu", "achieved": fl / (kavg * 1e-3) / 1e12, "peak": PEAK_FP32 / 1e12,
-                                 "unit": "TFLOP/s", "frac": fl / (kavg * 1e-3) / PEAK_FP32}}
+                                 "unit": "TFLOP/s", "frac": fl / (kavg * 1e-3) / PEAK_FP32,
+                                 "valu_counters": read_valu("tt")}}
     log("bench: tissue leg, 512 x 512, 20 + 100 steps")
     tis, _, _ = tissue_setup(TISSUE_NX, cep.PREC_FP64)
     tis.step(20, TISSUE_DT, cep.SPLIT_STRANG)

@@ -96,3 +96,17 @@ def test_oracle_restart_from_states_matches_single_runs():
         assert rc == 0
         assert np.array_equal(Sk, S[k])
         assert r["apd90"] == res[k].apd90 and r["qnet"] == res[k].qnet
+
+
+def test_committed_counter_summaries_are_read():
+    """The roofline's profile-derived figures: HBM traffic of the BDF launch, and VALU-busy,
+    active-lane and issue figures of the three cell kernels, each consistent with its raw counters."""
+    import bench
+    assert bench.read_traffic() > 1e11
+    kv = bench.read_profile(bench.VALU_FILE)
+    for prefix in ("bdf", "tt", "o11"):
+        v = bench.read_valu(prefix)
+        assert v is not None and 0 < v["valu_busy_pct"] <= 100 and 0 < v["valu_utilization_pct"] <= 100
+        T = float(kv[f"{prefix}_T_ms"]) * 1e-3
+        fp64 = float(kv[f"{prefix}_sq_insts_valu_flops_fp64"]) * 64 / T / 1e12
+        assert abs(fp64 - v["fp64_issued_tflops"]) <= 0.01 * v["fp64_issued_tflops"]
