@@ -34,6 +34,14 @@ INTEGRATOR_RUSH_LARSEN = 0
 INTEGRATOR_ADAMS = 1
 PREC_FP64 = 0
 PREC_MIXED = 1
+# launch shapes (cep_set_launch_shape / cep_autotune); 0 is automatic everywhere
+SHAPE_WAVES_PER_EU = 0
+SHAPE_CELLS_PER_WAVE = 1
+SHAPE_BLOCK = 2
+SHAPE_GROUP_ORDER = 3
+SHAPE_FIELDS = ("waves_per_eu", "cells_per_wave", "block", "group_order")
+ORDER_STORAGE = 1
+ORDER_LONGEST_FIRST = 2
 STATIC_RATES_MAX_CELLS = 2048
 
 BM_FIELDS = ("qnet", "inal_auc", "ical_auc", "dvmdt_repol", "dvmdt_max", "vm_peak", "vm_valley",
@@ -60,6 +68,7 @@ EXPORTED = (
     "cep_get_vm_trace", "cep_get_time_series", "cep_get_dvmdt_repol_history",
     "cep_kernel_time", "cep_query", "cep_batch_device_bytes", "cep_get_driver_rules", "cep_get_qinward",
     "cep_integrate_bdf", "cep_step_rl", "cep_compute_rates_static",
+    "cep_set_launch_shape", "cep_get_launch_shape", "cep_autotune", "cep_tune_lookup",
     "cep_tissue_create", "cep_tissue_create_model", "cep_tissue_destroy", "cep_tissue_last_error", "cep_tissue_set_cells",
     "cep_tissue_set_conductivity", "cep_tissue_set_stimulus", "cep_tissue_step", "cep_tissue_diffuse",
     "cep_tissue_get_vm", "cep_tissue_get_states", "cep_tissue_set_states", "cep_tissue_time",
@@ -131,6 +140,10 @@ def library(path: str | None = None):
     L.cep_step_rl.argtypes = [C.c_void_p, C.c_double, C.c_double]
     L.cep_get_qinward.argtypes = [C.c_void_p, C.POINTER(C.c_int), dp]
     L.cep_query.argtypes = [C.c_void_p]
+    L.cep_set_launch_shape.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.cep_get_launch_shape.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
+    L.cep_autotune.argtypes = [C.c_void_p, C.c_int, C.c_char_p, dp, C.c_int, C.POINTER(C.c_int)]
+    L.cep_tune_lookup.argtypes = [C.c_void_p, C.c_char_p]
     L.cep_batch_device_bytes.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int]
     L.cep_batch_device_bytes.restype = C.c_longlong
     L.cep_tissue_create.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int]
@@ -397,6 +410,36 @@ class CellBatch:
         avg, tot = C.c_double(), C.c_double()
         n = self.L.cep_kernel_time(self.h, C.byref(avg), C.byref(tot))
         return n, avg.value, tot.value
+
+    def set_launch_shape(self, knob: int, value: int):
+        self._chk(self.L.cep_set_launch_shape(self.h, knob, int(value)))
+
+    def launch_shape(self) -> dict:
+        """The shape the next launch uses, automatic choices resolved."""
+        out = {}
+        for k, name in enumerate(SHAPE_FIELDS):
+            v = C.c_int()
+            self._chk(self.L.cep_get_launch_shape(self.h, k, C.byref(v)))
+            out[name] = v.value
+        return out
+
+    def autotune(self, trial_beats: int = 1, cache: str | None = None):
+        """Times every candidate launch shape on trial_beats beats of this batch (its device state
+        restored afterwards) and keeps the fastest: (candidates with their ms, chosen index)."""
+        rep = np.zeros((16, len(SHAPE_FIELDS) + 1))
+        nc = C.c_int()
+        best = self.L.cep_autotune(self.h, int(trial_beats), cache.encode() if cache else None, _p(rep), 16,
+                                   C.byref(nc))
+        self._chk(min(best, 0))
+        rows = [dict(zip(SHAPE_FIELDS + ("ms",), [int(x) for x in r[:-1]] + [float(r[-1])]))
+                for r in rep[:nc.value]]
+        return rows, best
+
+    def tune_lookup(self, cache: str) -> bool:
+        """Applies a cached choice to the shapes still automatic; True if the cache had one."""
+        rc = self.L.cep_tune_lookup(self.h, cache.encode())
+        self._chk(min(rc, 0))
+        return rc == 1
 
 
 class Mesh:

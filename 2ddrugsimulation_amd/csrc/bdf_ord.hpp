@@ -141,6 +141,7 @@ struct Buffers {
   double dt, rtol, atol, uround_fd;
   int maxord, max_steps;
   int cap;
+  const int *gorder;  // [groups] cell group of each workgroup (ordered beat launches), or null
 };
 
 // integrator scalars of one cell (LDS-resident during a launch)

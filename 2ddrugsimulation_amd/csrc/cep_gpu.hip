@@ -155,7 +155,11 @@ CEP_D void wave_sweeps(const ordbdf::Buffers &B, const ordbdf::BlockBufs &V, ord
 // one, keeps the code of a kernel that reads no launch argument besides n (run-time forms of this
 // choice gave FP64 builds whose Newton iterations failed from the first step; the three forms
 // are compared bitwise by tests/test_adapter.py).
-enum { LAUNCH_BEATS = 0, LAUNCH_OUTPUTS = 1, LAUNCH_TO_OUTPUT = 2 };
+// LAUNCH_BEATS_ORDERED: whole beats, with workgroup w running cell group B.gorder[w] (groups
+// dispatched longest beat first: the hardware dispatcher hands the next group to a CU as one
+// finishes, so the long groups no longer trail at the end of a launch of more groups than fit
+// at once)
+enum { LAUNCH_BEATS = 0, LAUNCH_OUTPUTS = 1, LAUNCH_TO_OUTPUT = 2, LAUNCH_BEATS_ORDERED = 3 };
 
 template <typename R, int CPW, int MODE>
 __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunCfg rc, const int nbeats,
@@ -177,15 +181,17 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
   __shared__ DriverRules s_dr[CPW];               // driver rules of the output lanes (rc.dr set)
   __shared__ double s_acc[CPW][3];                // output point's STATES[qnet, INaL_AUC, ICaL_AUC]
 
+  constexpr bool kBeats = MODE == LAUNCH_BEATS || MODE == LAUNCH_BEATS_ORDERED;
   const int lane = threadIdx.x;
-  const int cell0 = blockIdx.x * CPW;
+  const int grp = MODE == LAUNCH_BEATS_ORDERED ? B.gorder[blockIdx.x] : (int)blockIdx.x;
+  const int cell0 = grp * CPW;
   const bool is_int = lane < CPW && cell0 + lane < B.ncell;
   const bool is_out = lane >= CPW && lane < 2 * CPW && cell0 + lane - CPW < B.ncell;
   const int slot = lane < CPW ? lane : (lane < 2 * CPW ? lane - CPW : 0);
   const int c = cell0 + slot;
   const LVec LV{&s_v[0][lane]};                    // this lane's staging vector
   const double srur = sqrt(B.uround_fd);
-  const size_t blk0 = (size_t)blockIdx.x * CPB;     // first cell of this workgroup's block
+  const size_t blk0 = (size_t)grp * CPB;            // first cell of this workgroup's block
   const BlockBufs V0{make_rsrc(B.zn + blk0 * ((QMAX + 1) * N), CPB * (QMAX + 1) * N * 8),
                      make_rsrc(B.ewt + blk0 * N, CPB * N * 8), make_rsrc(B.acor + blk0 * N, CPB * N * 8),
                      make_rsrc(B.ft + blk0 * N, CPB * N * 8), make_rsrc(B.jd + blk0 * (NJC * N), CPB * NJC * N * 8),
@@ -221,7 +227,7 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
     m.sw_dq = 0;
     m.sw_q1 = m.q;
     const int beat_steps = (int)(B.prm[ord::OP_BCL * B.pitch + c] / rc.dt);
-    if constexpr (MODE == LAUNCH_BEATS) {
+    if constexpr (kBeats) {
       m.iout_end = m.iout + nbeats * beat_steps;
     } else {
       m.iout_end = MODE == LAUNCH_OUTPUTS ? m.iout + nbeats : nbeats;
@@ -239,7 +245,7 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
       if (fresh) s_dr[slot].begin(B.y[c]);
       else s_dr[slot].load(rc.dr, B.pitch, c);
     }
-    if constexpr (MODE == LAUNCH_BEATS) {
+    if constexpr (kBeats) {
       trace_start = iout0 + (nbeats - 1) * bm.beat_steps;
     } else {                                       // the trace holds the launch's last beat_steps outputs
       trace_start = (MODE == LAUNCH_OUTPUTS ? iout0 + nbeats : nbeats) - bm.beat_steps;
@@ -626,12 +632,14 @@ struct TtBuffers {
   int *bmi, *iout, *status;
 };
 
-// At most 256 registers, two waves per SIMD: 200,000 cells make 3125 waves, which with one wave
-// per SIMD took four rounds of the 1024 SIMDs.  Measured per beat: FP32 rates 0.60 s against
-// 0.78 s (320 registers, 220 bytes of spills per lane once capped), FP64 1.39 s against 1.64 s
-// (390 registers, 512 bytes of spills): the second wave hides more than the spills cost.
-template <typename R>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+// WPE: waves per SIMD the register allocation leaves room for (the register split).  WPE 2: at
+// most 256 registers, spills to scratch; 200,000 cells make 3125 waves, which with one wave per
+// SIMD took four rounds of the 1024 SIMDs.  Measured per beat: FP32 rates 0.60 s against 0.78 s
+// (320 registers, 220 bytes of spills per lane once capped), FP64 1.39 s against 1.64 s (390
+// registers, 512 bytes of spills): the second wave hides more than the spills cost.  WPE 1: all
+// 512 registers, no spills, for batches with no more waves than SIMDs.
+template <typename R, int WPE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE)))
 tt_rl_kernel(TtBuffers B, RunCfg rc, int nbeats)
 {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -702,11 +710,12 @@ struct O11Buffers {
   double time_point, max_dt;
 };
 
-// At most 256 registers, two waves per SIMD (the large-population kernel: 100,000 cells make
-// 1563 waves, more than the 1024 SIMDs).  Measured on 100,000 cells, FP64: 371 ms per beat
-// against 423 ms with 451 registers and one wave, despite 776 bytes of spills per lane.
-template <typename R>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+// WPE (the register split): 2 = at most 256 registers, two waves per SIMD (the large-population
+// kernel: 100,000 cells make 1563 waves, more than the 1024 SIMDs).  Measured on 100,000 cells,
+// FP64: 371 ms per beat against 423 ms with 451 registers and one wave, despite 776 bytes of
+// spills per lane.  1 = all 512 registers and no spills.
+template <typename R, int WPE>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE)))
 ord11_rl_kernel(O11Buffers B, int beats_end, long long max_steps)
 {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1548,6 +1557,10 @@ struct cep_batch {
   int beats_done = 0;
   int integrator = CEP_INTEGRATOR_RUSH_LARSEN;
   int o11_waves = 0;             // CEP_OPT_ORD11_WAVES: 0 automatic, 1, 2 or 4 wavefronts per 64 cells
+  // launch shapes (CEP_SHAPE_*, 0 = automatic), chosen by hand, by cep_autotune or from its cache
+  int shape_wpe = 0, shape_block = 0, shape_order = 0;
+  int *d_gorder = nullptr;       // ORd-CiPA: cell group of each workgroup, longest beat first
+  std::vector<int> h_gorder;
   double *d_aphi = nullptr, *d_avec = nullptr, *d_acoef = nullptr, *d_asc = nullptr;
   int *d_asi = nullptr;
   // biomarker rules of the drugs_herg / benchpatch / population_tn2006 drivers
@@ -1857,22 +1870,56 @@ int cep_set_model_option(cep_batch *b, int option, double value)
   return 0;
 }
 
+static int device_cus()
+{
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+      cus = 256;
+  }
+  return cus;
+}
+
 // ORd 2011 Rush-Larsen: wavefronts per 64 cells.  Four while four times the wave count still
 // fits one wave per SIMD (CUs x 4), else two while that fits, else one, whose waves already
 // fill the SIMDs.  CEP_OPT_ORD11_WAVES forces a choice.
 static int ord11_waves(const cep_batch *b)
 {
   if (b->o11_waves == 1 || b->o11_waves == 2 || b->o11_waves == 4) return b->o11_waves;
-  static int simds = 0;
-  if (simds == 0) {
-    int dev = 0, cus = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-      cus = 256;
-    simds = 4 * cus;
-  }
+  const int simds = 4 * device_cus();
   const int blocks = (b->n + 63) / 64;
   return 4 * blocks <= simds ? 4 : 2 * blocks <= simds ? 2 : 1;
+}
+
+// TT06 and the one-wave ORd 2011 kernel: waves per SIMD of the register allocation (1: 512
+// registers, no spills; 2: 256 registers, spills to scratch, a second wave hides latency).
+// Automatic: 2, the measured choice for populations of more waves than SIMDs.
+static int rl_waves_per_eu(const cep_batch *b) { return b->shape_wpe == 1 ? 1 : 2; }
+
+// TT06: threads per workgroup (one cell per thread).  Automatic: 256.
+static int tt_block(const cep_batch *b) { return b->shape_block ? b->shape_block : 256; }
+
+// ORd-CiPA whole-beat launches: the order in which the groups of CPB cells are dispatched.
+// Automatic: longest beat first when there are more groups than workgroups the device holds at
+// once (the dispatcher then refills CUs as groups end, and long groups started last would trail),
+// else storage order (every group starts at once; the order changes nothing).
+static int bdf_group_order(const cep_batch *b)
+{
+  if (b->shape_order == CEP_ORDER_STORAGE || b->shape_order == CEP_ORDER_LONGEST_FIRST) return b->shape_order;
+  static int resident[2] = {0, 0};
+  const int f64 = b->prec == CEP_PREC_FP64;
+  if (resident[f64] == 0) {
+    int per_cu = 0;
+    const hipError_t e = f64 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                                   &per_cu, ord_bdf_kernel<double, ordbdf::CPB, LAUNCH_BEATS>, 64, 0)
+                             : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                                   &per_cu, ord_bdf_kernel<float, ordbdf::CPB, LAUNCH_BEATS>, 64, 0);
+    resident[f64] = (e == hipSuccess && per_cu > 0 ? per_cu : 3) * device_cus();
+  }
+  const int groups = (b->n + ordbdf::CPB - 1) / ordbdf::CPB;
+  return groups > resident[f64] ? CEP_ORDER_LONGEST_FIRST : CEP_ORDER_STORAGE;
 }
 
 static double max_bcl(const cep_batch *b)
@@ -1986,8 +2033,22 @@ int cep_init_consts(cep_batch *b)
   if (int rc = alloc_trace_buffers(b)) return rc;
   HIPCHK(b, hipMemcpyAsync(b->d_prm, prm.data(), prm.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
   HIPCHK(b, hipMemcpyAsync(b->d_y, y.data(), y.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  if (b->model == CEP_MODEL_ORD_CIPA) {
+    // the groups of CPB cells by their longest beat, longest first (ties in storage order)
+    const int ng = (b->n + ordbdf::CPB - 1) / ordbdf::CPB;
+    std::vector<double> glen(ng, 0.0);
+    for (int i = 0; i < b->n; ++i) glen[i / ordbdf::CPB] = std::max(glen[i / ordbdf::CPB], b->ord_in[i].bcl);
+    b->h_gorder.resize(ng);
+    for (int g = 0; g < ng; ++g) b->h_gorder[g] = g;
+    std::stable_sort(b->h_gorder.begin(), b->h_gorder.end(), [&](int x, int y) { return glen[x] > glen[y]; });
+    if (!b->d_gorder)
+      if (int rc = dalloc(b, &b->d_gorder, (size_t)ng)) return rc;
+    HIPCHK(b, hipMemcpyAsync(b->d_gorder, b->h_gorder.data(), ng * sizeof(int), hipMemcpyHostToDevice, b->stream));
+  }
   if (int rc = reset_integrators(b)) return rc;
   HIPCHK(b, hipStreamSynchronize(b->stream));
+  if (const char *cache = getenv("CEP_TUNE_CACHE"))
+    if (*cache) (void)cep_tune_lookup(b, cache);
   return 0;
 }
 
@@ -2092,7 +2153,12 @@ static void launch_ord_bdf(cep_batch *b, const RunCfg &rc, int nbeats, int nout_
   const int blocks = (b->n + CPW - 1) / CPW;
   const dim3 g(blocks), w(64);
   const bool f64 = b->prec == CEP_PREC_FP64;
-  if (nbeats > 0) {
+  B.gorder = nullptr;
+  if (nbeats > 0 && bdf_group_order(b) == CEP_ORDER_LONGEST_FIRST) {
+    B.gorder = b->d_gorder;
+    if (f64) hipLaunchKernelGGL((ord_bdf_kernel<double, CPW, LAUNCH_BEATS_ORDERED>), g, w, 0, b->stream, B, rc, nbeats, max_passes);
+    else hipLaunchKernelGGL((ord_bdf_kernel<float, CPW, LAUNCH_BEATS_ORDERED>), g, w, 0, b->stream, B, rc, nbeats, max_passes);
+  } else if (nbeats > 0) {
     if (f64) hipLaunchKernelGGL((ord_bdf_kernel<double, CPW, LAUNCH_BEATS>), g, w, 0, b->stream, B, rc, nbeats, max_passes);
     else hipLaunchKernelGGL((ord_bdf_kernel<float, CPW, LAUNCH_BEATS>), g, w, 0, b->stream, B, rc, nbeats, max_passes);
   } else if (nout_rel > 0) {
@@ -2192,10 +2258,15 @@ int cep_run_beats_async(cep_batch *b, int nbeats)
         hipLaunchKernelGGL(ord11_rl2_kernel<double>, dim3(blocks), dim3(128), 0, b->stream, B, b->beats_done + nbeats, max_steps);
       else
         hipLaunchKernelGGL(ord11_rl2_kernel<float>, dim3(blocks), dim3(128), 0, b->stream, B, b->beats_done + nbeats, max_steps);
+    } else if (rl_waves_per_eu(b) == 1) {
+      if (b->prec == CEP_PREC_FP64)
+        hipLaunchKernelGGL((ord11_rl_kernel<double, 1>), dim3(blocks), dim3(64), 0, b->stream, B, b->beats_done + nbeats, max_steps);
+      else
+        hipLaunchKernelGGL((ord11_rl_kernel<float, 1>), dim3(blocks), dim3(64), 0, b->stream, B, b->beats_done + nbeats, max_steps);
     } else if (b->prec == CEP_PREC_FP64) {
-      hipLaunchKernelGGL(ord11_rl_kernel<double>, dim3(blocks), dim3(64), 0, b->stream, B, b->beats_done + nbeats, max_steps);
+      hipLaunchKernelGGL((ord11_rl_kernel<double, 2>), dim3(blocks), dim3(64), 0, b->stream, B, b->beats_done + nbeats, max_steps);
     } else {
-      hipLaunchKernelGGL(ord11_rl_kernel<float>, dim3(blocks), dim3(64), 0, b->stream, B, b->beats_done + nbeats, max_steps);
+      hipLaunchKernelGGL((ord11_rl_kernel<float, 2>), dim3(blocks), dim3(64), 0, b->stream, B, b->beats_done + nbeats, max_steps);
     }
     b->beats_done += nbeats;
   } else {
@@ -2204,11 +2275,16 @@ int cep_run_beats_async(cep_batch *b, int nbeats)
     B.prm = b->d_prm; B.bmd = b->d_bmd; B.ct = b->d_ct; B.cv = b->d_cv; B.tcur = b->d_tcur;
     B.vtrace = b->vtrace_on ? b->d_vtrace : nullptr; B.bmi = b->d_bmi; B.iout = b->d_iout;
     B.status = b->d_status;
-    const int blocks = (b->n + 255) / 256;
-    if (b->prec == CEP_PREC_FP64)
-      hipLaunchKernelGGL(tt_rl_kernel<double>, dim3(blocks), dim3(256), 0, b->stream, B, rc, nbeats);
-    else
-      hipLaunchKernelGGL(tt_rl_kernel<float>, dim3(blocks), dim3(256), 0, b->stream, B, rc, nbeats);
+    const int tb = tt_block(b);
+    const int blocks = (b->n + tb - 1) / tb;
+    const bool one = rl_waves_per_eu(b) == 1;
+    if (b->prec == CEP_PREC_FP64) {
+      if (one) hipLaunchKernelGGL((tt_rl_kernel<double, 1>), dim3(blocks), dim3(tb), 0, b->stream, B, rc, nbeats);
+      else hipLaunchKernelGGL((tt_rl_kernel<double, 2>), dim3(blocks), dim3(tb), 0, b->stream, B, rc, nbeats);
+    } else {
+      if (one) hipLaunchKernelGGL((tt_rl_kernel<float, 1>), dim3(blocks), dim3(tb), 0, b->stream, B, rc, nbeats);
+      else hipLaunchKernelGGL((tt_rl_kernel<float, 2>), dim3(blocks), dim3(tb), 0, b->stream, B, rc, nbeats);
+    }
   }
   HIPCHK(b, hipGetLastError());
   HIPCHK(b, hipEventRecord(b->ev1, b->stream));
@@ -2511,6 +2587,8 @@ int cep_kernel_time(cep_batch *b, double *avg_ms, double *total_ms)
 }
 
 }  // extern "C"
+
+#include "autotune.hpp"
 
 #include "mesh.hpp"
 #include "tissue.hpp"

@@ -282,6 +282,42 @@ int cep_get_qinward(cep_batch *b, const int *control, double *out);
    call, timed with HIP events on the batch stream; returns launches counted */
 int cep_kernel_time(cep_batch *b, double *avg_ms, double *total_ms);
 
+/* ---- launch shapes and the autotuner.  No reference counterpart: how the library maps cells
+   onto the GPU.  Every shape computes the same arithmetic; the ORd-CiPA group order and the
+   TT06 / ORd 2011 register split give bitwise the same results, the ORd 2011 wavefront split
+   the same up to floating-point contraction.  Value 0 is automatic everywhere. */
+#define CEP_SHAPE_WAVES_PER_EU 0   /* TT06, ORd 2011 one-wave kernel: 1 (all 512 registers, no
+                                      spills) or 2 (256 registers, spills to scratch, a second
+                                      wave per SIMD hides latency); automatic 2.  ORd-CiPA: 1. */
+#define CEP_SHAPE_CELLS_PER_WAVE 1 /* ORd 2011: 64, 32 or 16 (the step over 1, 2 or 4 wavefronts
+                                      exchanging through LDS; CEP_OPT_ORD11_WAVES); ORd-CiPA: 16
+                                      (16 integrator and 16 output lanes per wavefront); TT06: 64 */
+#define CEP_SHAPE_BLOCK 2          /* TT06: threads per workgroup, 64, 128 or 256 (automatic 256) */
+#define CEP_SHAPE_GROUP_ORDER 3    /* ORd-CiPA whole-beat launches: CEP_ORDER_STORAGE or
+                                      CEP_ORDER_LONGEST_FIRST (groups of 16 cells by their longest
+                                      BCL); automatic: longest first when the groups outnumber the
+                                      workgroups the device holds at once */
+#define CEP_NSHAPE 4
+#define CEP_ORDER_STORAGE 1
+#define CEP_ORDER_LONGEST_FIRST 2
+int cep_set_launch_shape(cep_batch *b, int knob, int value);
+/* the value the next launch uses (automatic resolved) */
+int cep_get_launch_shape(cep_batch *b, int knob, int *value);
+/* Times every candidate shape of the batch's model on trial_beats beats of its own cells from its
+   current state (HIP events on the batch stream), keeps the fastest, and restores the batch's
+   device state bitwise, so the tuning run leaves no trace in the results.  report (may be null):
+   per candidate CEP_NSHAPE shape values then its ms, (CEP_NSHAPE + 1) doubles, at most
+   max_candidates rows; *ncandidates gets the count.  cache_path (may be null): the choice is
+   stored there under the key (device architecture, CU count, model, precision, integrator, cell
+   count rounded up to a power of two), replacing an older entry.  Allocates a copy of the
+   batch's device memory for the duration of the call.  Returns the index of the chosen row. */
+int cep_autotune(cep_batch *b, int trial_beats, const char *cache_path, double *report, int max_candidates,
+                 int *ncandidates);
+/* Applies the cached choice for the batch's key to every shape still automatic: 1 if an entry was
+   found, 0 if not (or no file).  cep_init_consts does this itself when the environment variable
+   CEP_TUNE_CACHE names a cache file. */
+int cep_tune_lookup(cep_batch *b, const char *cache_path);
+
 #ifdef __cplusplus
 }
 #endif
