@@ -1895,11 +1895,19 @@ static int ord11_waves(const cep_batch *b)
 
 // TT06 and the one-wave ORd 2011 kernel: waves per SIMD of the register allocation (1: 512
 // registers, no spills; 2: 256 registers, spills to scratch, a second wave hides latency).
-// Automatic: 2, the measured choice for populations of more waves than SIMDs.
-static int rl_waves_per_eu(const cep_batch *b) { return b->shape_wpe == 1 ? 1 : 2; }
+// Automatic: 1 while the batch has no more waves than SIMDs (a second wave could not come), else
+// 2.  Measured one-beat trials (MI355X): TT06 4,000 cells 194 ms with 1 against 213 with 2,
+// 200,000 cells 780 against 631; ORd 2011 100,000 cells 210 against 186.
+static int rl_waves_per_eu(const cep_batch *b)
+{
+  if (b->shape_wpe == 1 || b->shape_wpe == 2) return b->shape_wpe;
+  return (b->n + 63) / 64 <= 4 * device_cus() ? 1 : 2;
+}
 
-// TT06: threads per workgroup (one cell per thread).  Automatic: 256.
-static int tt_block(const cep_batch *b) { return b->shape_block ? b->shape_block : 256; }
+// TT06: threads per workgroup (one cell per thread).  Automatic: 64 (one wave per workgroup, so
+// the dispatcher places waves one at a time): 616 ms against 631 with 256 on 200,000 cells, 197
+// against 213 on 4,000.
+static int tt_block(const cep_batch *b) { return b->shape_block ? b->shape_block : 64; }
 
 // ORd-CiPA whole-beat launches: the order in which the groups of CPB cells are dispatched.
 // Automatic: longest beat first when there are more groups than workgroups the device holds at

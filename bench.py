@@ -491,6 +491,57 @@ def tt06_main(args):
     }), flush=True)
 
 
+def tune_main(args):
+    """The launch-shape autotuner on each model at a small and a production cell count: every
+    candidate's time for one beat of the batch's own cells, and the choice.  A tuning cache
+    (--tune-cache) keeps the choices for later runs."""
+    G = os.path.join(ROOT, "tests", "golden")
+    cases = []
+
+    def ord_cipa(n):
+        pop = cep.workload.ord_population(20251204, 50, 4, max(1, n // 200))
+        order = np.argsort(pop["bcl"], kind="stable")
+        pop = {k: v[order][:n] for k, v in pop.items()}
+        b = make_batch(pop, cep.PREC_MIXED, 200)
+        return b, f"ORd-CiPA BDF, {n} cells, BCL 500/1000/2000 stored shortest first, mixed"
+
+    def tt06(n):
+        pop = cep.workload.tt_population(20251205, n)
+        b = cep.CellBatch(cep.MODEL_TT06, n, cep.PREC_MIXED)
+        b.set_cells(celltype=pop["celltype"], conc=pop["conc"],
+                    herg=np.concatenate([pop["ic50"], pop["hill"]], axis=1), gscale=pop["gscale"], bcl=pop["bcl"])
+        b.set_run_config(dt=0.02, nbeats_total=500, nwin=500)
+        b.init_consts()
+        return b, f"TT06 Rush-Larsen, {n} cells, mixed"
+
+    def ord11(n):
+        q = np.loadtxt(os.path.join(G, "quinidine_IC50_samples10.csv"), delimiter=",", skiprows=1)
+        s0 = np.loadtxt(os.path.join(G, "ord2011_output_orudy.dat"))
+        conc = np.array([(3237.0, 6474.0, 9711.0)[i % 3] for i in range(n)])
+        b = cep.CellBatch(cep.MODEL_ORD2011, n, cep.PREC_FP64)
+        b.set_cells(celltype=np.full(n, 2.0), conc=conc, herg=np.vstack([q[i % len(q)] for i in range(n)]),
+                    bcl=np.full(n, 2000.0))
+        b.set_run_config(dt=1.0, nbeats_total=1000, nwin=250)
+        b.init_consts()
+        b.set_states(np.repeat(s0[None, :], n, axis=0))
+        return b, f"ORd 2011 Rush-Larsen, {n} quinidine cells, FP64"
+
+    plan = [(ord_cipa, 10000), (ord_cipa, 40000), (tt06, 4000), (tt06, 200000), (ord11, 30), (ord11, 100000)]
+    for make, n in plan:
+        b, name = make(n)
+        default = b.launch_shape()
+        log(f"bench: tuning {name}")
+        rows, best = b.autotune(1, args.tune_cache)
+        chosen = b.launch_shape()
+        b.close()
+        dms = [r["ms"] for r in rows if all(r[k] == default[k] for k in cep.SHAPE_FIELDS)]
+        cases.append({"workload": name, "cells": n, "candidates": rows, "chosen": chosen, "default": default,
+                      "default_ms": dms[0] if dms else None, "chosen_ms": rows[best]["ms"],
+                      "speedup_vs_default": (dms[0] / rows[best]["ms"]) if dms else None})
+    print(json.dumps({"metric": "launch-shape autotuner: one-beat trial per candidate", "unit": "ms",
+                      "higher_is_better": False, "n_gpus": 1, "cases": cases}), flush=True)
+
+
 def serving_main(args):
     """Arrival-rate sweep of the serving loop: for each rate a fresh server and the same
     seeded jobs (ORd-CiPA or TT06, 1-400 cells, 2-4 beats), submitted with Poisson gaps."""
@@ -685,7 +736,7 @@ def ord11_cpu_baseline(q, s0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("population", "tissue", "ord2011", "serving", "tt06"),
+    ap.add_argument("--workload", choices=("population", "tissue", "ord2011", "serving", "tt06", "tune"),
                     default="population")
     ap.add_argument("--tt-cells", type=int, default=200000)
     ap.add_argument("--steps", type=int, default=None)
@@ -705,6 +756,7 @@ def main():
     ap.add_argument("--serve-jobs", type=int, default=24)
     ap.add_argument("--serve-rate", type=float, default=4.0)
     ap.add_argument("--serve-rates", default="1,2,4,8,16,32,64", help="arrival rates of --workload serving")
+    ap.add_argument("--tune-cache", default=None, help="--workload tune: tuning cache file to write")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "single-GPU build: run with --gpus 1 (cells shard trivially "
@@ -715,6 +767,9 @@ def main():
         return
     if args.workload == "serving":
         serving_main(args)
+        return
+    if args.workload == "tune":
+        tune_main(args)
         return
     if args.workload == "tt06":
         args.precision = args.precision or "mixed"

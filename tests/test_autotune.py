@@ -42,20 +42,21 @@ def _ord_batch(pop, prec):
 
 @pytest.mark.gpu
 def test_tt06_shapes_agree_and_tuning_is_invisible(tmp_path):
-    """TT06, 3000 cells: each of the four shapes (register split 1 or 2 waves per SIMD, 64- or
-    256-thread workgroups) gives bitwise one beat of the default shape; autotune restores the
+    """TT06, 3000 cells: the register split (1 or 2 waves per SIMD) and 128- or 256-thread
+    workgroups give bitwise one beat of the default shape; autotune restores the
     state, picks the fastest row of its report, writes one cache line per key (a second run
     replaces it) and a new batch of the same key finds it, by call and through CEP_TUNE_CACHE."""
     pop = cep.workload.tt_population(20251210, 3000)
     herg = np.concatenate([pop["ic50"], pop["hill"]], axis=1)
     prec = cep.PREC_MIXED
     with _tt_batch(pop, herg, prec) as b:
-        assert b.launch_shape() == {"waves_per_eu": 2, "cells_per_wave": 64, "block": 256, "group_order": 0}
+        # automatic: 47 waves, fewer than the SIMDs, so all 512 registers; one wave per workgroup
+        assert b.launch_shape() == {"waves_per_eu": 1, "cells_per_wave": 64, "block": 64, "group_order": 0}
         b.run_beats(1)
         ref = _snapshot(b)
     assert (ref[2] == 0).all()
     for wpe in (1, 2):
-        for block in (64, 128):
+        for block in (128, 256):
             with _tt_batch(pop, herg, prec) as b:
                 b.set_launch_shape(cep.SHAPE_WAVES_PER_EU, wpe)
                 b.set_launch_shape(cep.SHAPE_BLOCK, block)
