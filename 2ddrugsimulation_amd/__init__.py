@@ -34,6 +34,8 @@ INTEGRATOR_RUSH_LARSEN = 0
 INTEGRATOR_ADAMS = 1
 PREC_FP64 = 0
 PREC_MIXED = 1
+ST_OK = 0
+ST_IDLE = 5          # a parked cell (park_cells): no job in it, runs skip it
 # launch shapes (cep_set_launch_shape / cep_autotune); 0 is automatic everywhere
 SHAPE_WAVES_PER_EU = 0
 SHAPE_CELLS_PER_WAVE = 1
@@ -67,6 +69,7 @@ EXPORTED = (
     "cep_get_biomarkers", "cep_get_status", "cep_get_stats", "cep_enable_vm_trace",
     "cep_get_vm_trace", "cep_get_time_series", "cep_get_dvmdt_repol_history",
     "cep_kernel_time", "cep_query", "cep_batch_device_bytes", "cep_get_driver_rules", "cep_get_qinward",
+    "cep_restart_cells", "cep_park_cells",
     "cep_integrate_bdf", "cep_step_rl", "cep_compute_rates_static",
     "cep_set_launch_shape", "cep_get_launch_shape", "cep_autotune", "cep_tune_lookup",
     "cep_tissue_create", "cep_tissue_create_model", "cep_tissue_destroy", "cep_tissue_last_error", "cep_tissue_set_cells",
@@ -140,6 +143,8 @@ def library(path: str | None = None):
     L.cep_step_rl.argtypes = [C.c_void_p, C.c_double, C.c_double]
     L.cep_get_qinward.argtypes = [C.c_void_p, C.POINTER(C.c_int), dp]
     L.cep_query.argtypes = [C.c_void_p]
+    L.cep_restart_cells.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.c_int, C.c_int, C.c_int]
+    L.cep_park_cells.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.c_int]
     L.cep_set_launch_shape.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.cep_get_launch_shape.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
     L.cep_autotune.argtypes = [C.c_void_p, C.c_int, C.c_char_p, dp, C.c_int, C.POINTER(C.c_int)]
@@ -351,6 +356,25 @@ class CellBatch:
 
     def sync(self):
         self._chk(self.L.cep_sync(self.h))
+
+    # ---- cells restarted and parked one by one (a resident batch serving jobs)
+    @staticmethod
+    def _cells(idx):
+        a = np.ascontiguousarray(idx, dtype=np.int32)
+        return a, a.ctypes.data_as(C.POINTER(C.c_int))
+
+    def restart_cells(self, idx, nbeats_total: int, nwin: int | None = None):
+        """Restart the listed cells from their model's initial states with the parameters last
+        given by set_cells, a fresh integrator and biomarker machine, and a biomarker window of
+        the last nwin (default all) of nbeats_total beats from now; other cells are untouched."""
+        a, p = self._cells(idx)
+        nw = nbeats_total if nwin is None or nwin <= 0 else nwin
+        self._chk(self.L.cep_restart_cells(self.h, p, len(a), int(nbeats_total), int(nw)))
+
+    def park_cells(self, idx):
+        """Status ST_IDLE for the listed cells: runs skip them until they are restarted."""
+        a, p = self._cells(idx)
+        self._chk(self.L.cep_park_cells(self.h, p, len(a)))
 
     def enable_vm_trace(self, on=True):
         self._chk(self.L.cep_enable_vm_trace(self.h, 1 if on else 0))

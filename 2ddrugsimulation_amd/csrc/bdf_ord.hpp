@@ -142,6 +142,7 @@ struct Buffers {
   int maxord, max_steps;
   int cap;
   const int *gorder;  // [groups] cell group of each workgroup (ordered beat launches), or null
+  const int *win;     // per-cell first biomarker window beat, -1: the batch's window
 };
 
 // integrator scalars of one cell (LDS-resident during a launch)

@@ -91,14 +91,16 @@ struct RunCfg {
   double *dr;                 // driver-rule state (DR_SN rows), or null: rules off
 };
 
-CEP_D void bm_config(Biomarker &bm, double bcl, double stim_start, const RunCfg &rc, int cap)
+// win_cell: the cell's own first window beat (set when the cell was restarted on its own), or
+// -1 for the batch's window, the last nwin of nbeats_total beats
+CEP_D void bm_config(Biomarker &bm, double bcl, double stim_start, const RunCfg &rc, int cap, int win_cell)
 {
   bm.bcl = bcl;
   bm.dt = rc.dt;
   bm.stim_start = stim_start;
   bm.beat_steps = (int)(bcl / rc.dt);
   const int nwin = rc.nwin > rc.nbeats_total ? rc.nbeats_total : rc.nwin;
-  bm.win_start = rc.nbeats_total - nwin;
+  bm.win_start = win_cell >= 0 ? win_cell : rc.nbeats_total - nwin;
   bm.print_freq = rc.print_freq;
   bm.cap = cap;
 }
@@ -236,7 +238,8 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
   }
   if (is_out) {
     Biomarker &bm = s_bm[slot];
-    bm_config(bm, B.prm[ord::OP_BCL * B.pitch + c], B.prm[ord::OP_STIM_START * B.pitch + c], rc, B.cap);
+    bm_config(bm, B.prm[ord::OP_BCL * B.pitch + c], B.prm[ord::OP_STIM_START * B.pitch + c], rc, B.cap,
+              B.win[c]);
     const int iout0 = B.si[11 * B.pitch + c];
     const bool fresh = B.si[4 * B.pitch + c] == PH_FRESH;
     if (fresh) bm.begin(B.y[c], B.y[9 * B.pitch + c]);
@@ -630,6 +633,7 @@ struct TtBuffers {
   int ncell, pitch, cap, tcap;
   double *y, *prm, *bmd, *ct, *cv, *tcur, *vtrace;
   int *bmi, *iout, *status;
+  const int *win;     // per-cell first window beat, -1: the batch's window
 };
 
 // WPE: waves per SIMD the register allocation leaves room for (the register split).  WPE 2: at
@@ -643,13 +647,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE)))
 tt_rl_kernel(TtBuffers B, RunCfg rc, int nbeats)
 {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= B.ncell) return;
+  if (c >= B.ncell || B.status[c] == ST_IDLE) return;
   const PrmAcc P{B.prm + c, B.pitch};
   double y[tt::NS];
 #pragma unroll
   for (int i = 0; i < tt::NS; ++i) y[i] = B.y[(size_t)i * B.pitch + c];
   Biomarker bm;
-  bm_config(bm, P(tt::TP_BCL), P(tt::TP_STIM_START), rc, B.cap);
+  bm_config(bm, P(tt::TP_BCL), P(tt::TP_STIM_START), rc, B.cap, B.win[c]);
   int iout = B.iout[c];
   double tcurr = B.tcur[c];
   if (iout == 0) bm.begin(y[0], y[3]); else bm.load(B.bmd, B.bmi, B.pitch, c);
@@ -1574,6 +1578,13 @@ struct cep_batch {
   double *d_fac = nullptr, *d_sd = nullptr, *d_tcur = nullptr, *d_vtrace = nullptr;
   int *d_si = nullptr, *d_bmi = nullptr, *d_status = nullptr, *d_iout = nullptr;
   long long *d_stats = nullptr;
+  // cells restarted one by one (cep_restart_cells): each cell's first window beat (-1: the
+  // batch's) and the staging rows of a restart
+  int *d_win = nullptr;
+  double *d_stage = nullptr;
+  int *d_stage_idx = nullptr;
+  std::vector<double> h_stage;
+  std::vector<int> h_stage_idx;
   std::vector<std::pair<void *, size_t>> allocs;   // device blocks and their sizes
 };
 
@@ -1725,6 +1736,7 @@ static cep_batch *batch_create(int model, int ncells, int precision, bool integr
   rc |= dalloc(b, &b->d_bmd, P * BM_TOTAL_D);
   rc |= dalloc(b, &b->d_bmi, P * BM_I_N);
   rc |= dalloc(b, &b->d_status, P);
+  rc |= dalloc(b, &b->d_win, P);
   if (model == CEP_MODEL_ORD_CIPA && integrator) {
     rc |= dalloc(b, &b->d_zn, P * 6 * ordbdf::N);
     rc |= dalloc(b, &b->d_ewt, P * ordbdf::N);
@@ -1983,6 +1995,7 @@ static int reset_integrators(cep_batch *b)
 {
   const size_t P = b->pitch;
   HIPCHK(b, hipMemsetAsync(b->d_status, 0, P * sizeof(int), b->stream));
+  HIPCHK(b, hipMemsetAsync(b->d_win, 0xff, P * sizeof(int), b->stream));       // -1: the batch's window
   HIPCHK(b, hipMemsetAsync(b->d_bmd, 0, P * BM_TOTAL_D * sizeof(double), b->stream));
   HIPCHK(b, hipMemsetAsync(b->d_bmi, 0, P * BM_I_N * sizeof(int), b->stream));
   if (b->d_dr) HIPCHK(b, hipMemsetAsync(b->d_dr, 0, P * DR_SN * sizeof(double), b->stream));
@@ -2146,6 +2159,7 @@ static void launch_ord_bdf(cep_batch *b, const RunCfg &rc, int nbeats, int nout_
   B.stats = b->d_stats; B.status = b->d_status; B.dt = b->dt; B.rtol = b->rtol; B.atol = b->atol;
   B.uround_fd = b->prec == CEP_PREC_FP64 ? ordbdf::UROUND : 1.1920928955078125e-07;
   B.maxord = b->maxord; B.max_steps = 500; B.cap = b->cap;
+  B.win = b->d_win;
   B.ts = TsView{};
   if (b->ts_on && b->d_ts) {
     const size_t sl = (size_t)b->pitch * 2 * b->ts_cap;   // one field, both slots
@@ -2282,7 +2296,7 @@ int cep_run_beats_async(cep_batch *b, int nbeats)
     B.ncell = b->n; B.pitch = b->pitch; B.cap = b->cap; B.tcap = b->tcap; B.y = b->d_y;
     B.prm = b->d_prm; B.bmd = b->d_bmd; B.ct = b->d_ct; B.cv = b->d_cv; B.tcur = b->d_tcur;
     B.vtrace = b->vtrace_on ? b->d_vtrace : nullptr; B.bmi = b->d_bmi; B.iout = b->d_iout;
-    B.status = b->d_status;
+    B.status = b->d_status; B.win = b->d_win;
     const int tb = tt_block(b);
     const int blocks = (b->n + tb - 1) / tb;
     const bool one = rl_waves_per_eu(b) == 1;
@@ -2592,6 +2606,151 @@ int cep_kernel_time(cep_batch *b, double *avg_ms, double *total_ms)
   b->ktime_total = 0.0;
   b->klaunches = 0;
   return n;
+}
+
+}  // extern "C"
+
+// ---- cells restarted and parked one by one (a resident batch whose cells serve jobs)
+//
+// A restart gives each listed cell its packed row (states, then parameters), clears every
+// per-cell row of the integrator and the biomarker machine, and sets its status and its own
+// first window beat.  The integrator then starts as for a fresh batch (phase FRESH: CVodeInit
+// at t = 0 from the new states; TT06: output index 0), so a restarted cell computes exactly
+// what the same cell computes in a batch of its own.
+struct RestartArgs {
+  int m, pitch, ns, np, win;
+  const int *idx;          // [m] cells
+  const double *rows;      // [m][ns + np]
+  double *y, *prm, *bmd, *dr;
+  int *bmi, *status, *winv;
+  double *zn, *sd;         // ORd-CiPA BDF (zn in blocks of ordbdf::CPB cells), else null
+  int *si;
+  long long *stats;
+  double *tcur;            // TT06, else null
+  int *iout;
+};
+
+__global__ void __launch_bounds__(64) restart_cells_kernel(RestartArgs A)
+{
+  const int k = blockIdx.x;
+  const int c = A.idx[k];
+  const size_t P = A.pitch;
+  const double *r = A.rows + (size_t)k * (A.ns + A.np);
+  for (int j = threadIdx.x; j < A.ns; j += 64) A.y[j * P + c] = r[j];
+  for (int j = threadIdx.x; j < A.np; j += 64) A.prm[j * P + c] = r[A.ns + j];
+  for (int j = threadIdx.x; j < BM_TOTAL_D; j += 64) A.bmd[j * P + c] = 0.0;
+  for (int j = threadIdx.x; j < BM_I_N; j += 64) A.bmi[j * P + c] = 0;
+  if (A.dr)
+    for (int j = threadIdx.x; j < DR_SN; j += 64) A.dr[j * P + c] = 0.0;
+  if (A.zn) {
+    using namespace ordbdf;
+    double *blk = A.zn + (size_t)(c / CPB) * CPB * ((QMAX + 1) * N) + c % CPB;
+    for (int j = threadIdx.x; j < (QMAX + 1) * N; j += 64) blk[(size_t)j * CPB] = 0.0;
+    for (int j = threadIdx.x; j < SD_N; j += 64) A.sd[j * P + c] = 0.0;
+    for (int j = threadIdx.x; j < SI_N; j += 64) A.si[j * P + c] = 0;    // phase FRESH
+    for (int j = threadIdx.x; j < 8; j += 64) A.stats[j * P + c] = 0;
+  }
+  if (threadIdx.x == 0) {
+    if (A.tcur) { A.tcur[c] = 0.0; A.iout[c] = 0; }
+    A.status[c] = ST_OK;
+    A.winv[c] = A.win;
+  }
+}
+
+__global__ void __launch_bounds__(64) park_cells_kernel(int m, const int *__restrict__ idx, int *__restrict__ status)
+{
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < m) status[idx[k]] = ST_IDLE;
+}
+
+// the listed cells, checked (in range, no repeats) and copied to the staging index buffer
+static int stage_cells(cep_batch *b, const int *idx, int m)
+{
+  if (!idx || m < 1 || m > b->n) return fail(b, CEP_E_ARG, "cell list");
+  std::vector<char> seen(b->n, 0);
+  for (int k = 0; k < m; ++k) {
+    if (idx[k] < 0 || idx[k] >= b->n) return fail(b, CEP_E_ARG, "cell index out of range");
+    if (seen[idx[k]]++) return fail(b, CEP_E_ARG, "cell listed twice");
+  }
+  if (!b->d_stage_idx) {
+    // sized once for the whole batch: later restarts and parks reuse them
+    if (dalloc(b, &b->d_stage_idx, (size_t)b->n) ||
+        dalloc(b, &b->d_stage, (size_t)b->n * (size_t)(b->ns + b->np)))
+      return CEP_E_HIP;
+  }
+  b->h_stage_idx.assign(idx, idx + m);
+  HIPCHK(b, hipMemcpyAsync(b->d_stage_idx, b->h_stage_idx.data(), m * sizeof(int), hipMemcpyHostToDevice, b->stream));
+  return 0;
+}
+
+extern "C" {
+
+int cep_restart_cells(cep_batch *b, const int *idx, int m, int nbeats_total, int nwin)
+{
+  if (!b) return CEP_E_ARG;
+  if (b->model != CEP_MODEL_ORD_CIPA && b->model != CEP_MODEL_TT06)
+    return fail(b, CEP_E_ARG, "cells are restarted one by one in ORd-CiPA and TT06 batches");
+  if (b->cap == 0) return fail(b, CEP_E_STATE, "cep_init_consts not called");
+  if (b->pending) return fail(b, CEP_E_STATE, "a run is in flight: cep_sync or cep_query first");
+  if (b->dr_on) return fail(b, CEP_E_STATE, "the driver rules count beats for the whole batch");
+  if (nbeats_total < 1 || nwin < 1) return fail(b, CEP_E_ARG, "nbeats_total and nwin");
+  if (int rc = stage_cells(b, idx, m)) return rc;
+  const size_t row = (size_t)(b->ns + b->np);
+  b->h_stage.assign((size_t)m * row, 0.0);
+  // the Ca trace was sized at cep_init_consts for the batch's longest beat
+  const int print_freq = std::max(1, (int)((1. / b->dt) * 1.0));
+  double p[64];
+  for (int k = 0; k < m; ++k) {
+    const int i = idx[k];
+    double *y = &b->h_stage[(size_t)k * row], *q = y + b->ns;
+    double bcl;
+    if (b->model == CEP_MODEL_ORD_CIPA) {
+      host::ord_params(b->ord_in[i], b->stim_scale, p);
+      for (int s = 0; s < 49; ++s) y[s] = host::kOrdInitStates[s];
+      y[43] = b->ord_in[i].conc;
+      bcl = b->ord_in[i].bcl;
+    } else {
+      host::tt_params(b->tt_in[i], b->stim_scale, p);
+      const int ct = (int)b->tt_in[i].celltype;
+      for (int s = 0; s < tt::NS; ++s) y[s] = host::kTtInit[ct][s];
+      bcl = b->tt_in[i].bcl;
+    }
+    if ((int)(bcl / b->dt) / print_freq + 4 > b->cap)
+      return fail(b, CEP_E_ARG, "a restarted cell's BCL is longer than the batch's at cep_init_consts");
+    for (int j = 0; j < b->np; ++j) q[j] = p[j];
+  }
+  HIPCHK(b, hipMemcpyAsync(b->d_stage, b->h_stage.data(), b->h_stage.size() * sizeof(double), hipMemcpyHostToDevice,
+                           b->stream));
+  RestartArgs A{};
+  A.m = m; A.pitch = b->pitch; A.ns = b->ns; A.np = b->np;
+  A.win = nbeats_total - std::min(nwin, nbeats_total);
+  A.idx = b->d_stage_idx; A.rows = b->d_stage;
+  A.y = b->d_y; A.prm = b->d_prm; A.bmd = b->d_bmd; A.dr = b->d_dr; A.bmi = b->d_bmi;
+  A.status = b->d_status; A.winv = b->d_win;
+  if (b->model == CEP_MODEL_ORD_CIPA) {
+    A.zn = b->d_zn; A.sd = b->d_sd; A.si = b->d_si; A.stats = b->d_stats;
+  } else {
+    A.tcur = b->d_tcur; A.iout = b->d_iout;
+  }
+  hipLaunchKernelGGL(restart_cells_kernel, dim3(m), dim3(64), 0, b->stream, A);
+  HIPCHK(b, hipGetLastError());
+  // the staging vectors are reused by the next call
+  HIPCHK(b, hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int cep_park_cells(cep_batch *b, const int *idx, int m)
+{
+  if (!b) return CEP_E_ARG;
+  if (b->model != CEP_MODEL_ORD_CIPA && b->model != CEP_MODEL_TT06)
+    return fail(b, CEP_E_ARG, "cells are parked one by one in ORd-CiPA and TT06 batches");
+  if (b->pending) return fail(b, CEP_E_STATE, "a run is in flight: cep_sync or cep_query first");
+  if (int rc = stage_cells(b, idx, m)) return rc;
+  hipLaunchKernelGGL(park_cells_kernel, dim3((m + 63) / 64), dim3(64), 0, b->stream, m, b->d_stage_idx,
+                     b->d_status);
+  HIPCHK(b, hipGetLastError());
+  HIPCHK(b, hipStreamSynchronize(b->stream));
+  return 0;
 }
 
 }  // extern "C"

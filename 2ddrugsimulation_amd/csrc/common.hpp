@@ -44,7 +44,7 @@ struct M<float> {
   static CEP_D float sq(float a) { return __fsqrt_rn(a); }
 };
 
-// per-cell run status
-enum : int { ST_OK = 0, ST_CONV_FAIL = 1, ST_ERR_FAIL = 2, ST_NONFINITE = 3, ST_MAX_STEPS = 4 };
+// per-cell run status; ST_IDLE: a parked cell (no job in it), which every run skips
+enum : int { ST_OK = 0, ST_CONV_FAIL = 1, ST_ERR_FAIL = 2, ST_NONFINITE = 3, ST_MAX_STEPS = 4, ST_IDLE = 5 };
 
 }  // namespace cep

@@ -7,14 +7,15 @@ Workload: 10,000 ORd-CiPA endo cells = 50 seeded drugs x 4 concentrations x 50 v
 point.  A "step" is one beat of the whole population (every cell advances one beat of its
 own BCL); inputs and states are resident in HBM.  The timed solver is the mixed-precision
 one (FP32 gating/currents, FP64 state/Newton/LU); an FP64 run of a subsample, as one plain
-job, checks APD90 and qNet and gives the plain per-job FP64 rate.  A short serving session
-(Poisson arrivals of mixed ORd-CiPA / TT06 jobs) gives the p50/p99 latency.
+job, checks APD90 and qNet and gives the plain per-job FP64 rate.  A serving session
+(Poisson arrivals of mixed ORd-CiPA / TT06 jobs of 1-400 cells, 10-40 beats, BCL 500/1000/2000
+ms, continuously batched into resident batches at the cell level) gives the p50/p99 latency.
 
 `--workload ord2011` times the reference's own ORd 2011 run (quinidine, 30 cells x 1000
 beats), whose wall time the reference published, and a batched population of that model.
-`--workload serving` sweeps the Poisson arrival rate of the serving session (default 1, 2, 4,
-8, 16, 32 and 64 jobs/s, the same seeded jobs at every rate) and reports p50/p99 latency and served
-cell-beats/s per rate; the value is the highest served rate in cell-beats/s.
+`--workload serving` sweeps the Poisson arrival rate of the serving session (default 2, 8 and
+32 jobs/s, the same seeded jobs at every rate, 10-100 beats each) and reports p50/p99 latency and
+served cell-beats/s per rate; the value is the highest served rate in cell-beats/s.
 `--workload tt06` times the ten Tusscher 2006 population: 200,000 cells (endo/epi/M, Hill
 block of IKr/ICaL/INa, ten lognormal(0, 0.25) conductance scales), BCL 1000 ms, Rush-Larsen
 dt 0.02 ms, on-device APD90/CaTD; a step is one beat of every cell.
@@ -35,10 +36,11 @@ import time
 
 import numpy as np
 
-# The serving loop keeps one HIP stream per admitted job (up to 16); HIP maps streams onto
-# GPU_MAX_HW_QUEUES hardware queues (default 4, some of them taken by the runtime itself), and
-# a persistent beat kernel on a shared queue holds back the kernels queued behind it (measured:
-# 4 jobs' beats took twice as long as 2 with 4 queues; 8 ran side by side with 16 queues).
+# The serving loop keeps one HIP stream per resident batch (one per model and BCL: up to 6 here);
+# HIP maps streams onto GPU_MAX_HW_QUEUES hardware queues (default 4, some of them taken by the
+# runtime itself), and a persistent beat kernel on a shared queue holds back the kernels queued
+# behind it (measured: 4 batches' beats took twice as long as 2 with 4 queues; 8 ran side by
+# side with 16 queues).
 # Read by the HIP runtime when it initialises, so set before any HIP call.
 SERVE_HW_QUEUES = 24
 if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < SERVE_HW_QUEUES:
@@ -317,23 +319,28 @@ def accuracy_check(pop, bm_timed, per_bcl: int, nbeats: int):
     }
 
 
-def serving_latency(njobs: int, rate: float):
-    """Short serving session: Poisson arrivals of screening jobs (1-400 cells, ORd-CiPA or
-    TT06, 2-4 beats each) into the continuous-batching loop."""
-    jobs = serving.synthetic_jobs(20251206, njobs, beats=(2, 4))
-    srv = serving.Server(mem_budget=32 << 30, max_active=16)
-    return _serve(srv, jobs, njobs, rate)
+def serving_latency(njobs: int, rate: float, beats):
+    """Serving session: Poisson arrivals of screening jobs (1-400 cells, ORd-CiPA or TT06,
+    BCL 500/1000/2000 ms, `beats` beats each) into the cell-level continuous-batching loop."""
+    jobs = serving.synthetic_jobs(20251206, njobs, beats=beats)
+    srv = serving.Server(mem_budget=32 << 30)
+    return _serve(srv, jobs, njobs, rate, beats)
 
 
-def _serve(srv, jobs, njobs: int, rate: float):
-    fin, wall = serving.run_poisson(srv, jobs, rate, seed=1, timeout=300)
+def _serve(srv, jobs, njobs: int, rate: float, beats):
+    fin, wall = serving.run_poisson(srv, jobs, rate, seed=1, timeout=900)
+    launches = {f"model{k[0]}_bcl{int(k[2])}": p.launches for k, p in srv.pools.items()}
+    pool_cells = {f"model{k[0]}_bcl{int(k[2])}": p.capacity for k, p in srv.pools.items()}
+    srv.close()
     p = serving.latency_percentiles(fin)
     cb = sum(j.ncells * j.nbeats for j in fin)
     bad = int(sum(int(np.sum(j.status != 0)) for j in fin))
     return {"p50": p["p50"], "p99": p["p99"], "jobs": len(fin), "rate_jobs_per_s": rate,
-            "cell_beats_per_s": cb / wall, "failed_cells": bad,
-            "workload": f"{njobs} jobs, Poisson {rate}/s, cells log-uniform 1-400, 2-4 beats, "
-                        "ORd-CiPA or TT06 (p=0.5), mixed precision"}
+            "cell_beats_per_s": cb / wall, "failed_cells": bad, "wall_s": wall,
+            "launches_per_pool": launches, "cells_per_pool": pool_cells,
+            "workload": f"{njobs} jobs, Poisson {rate}/s, cells log-uniform 1-400, {beats[0]}-{beats[1]} beats, "
+                        "ORd-CiPA or TT06 (p=0.5), BCL 500/1000/2000 ms per job, mixed precision, "
+                        "one resident batch per model and BCL shared at the cell level"}
 
 
 # Tissue (monodomain) workload: 512 x 512 Q1 elements, 0.1 mm, one seeded drug at 2 x Cmax.
@@ -546,16 +553,17 @@ def serving_main(args):
     """Arrival-rate sweep of the serving loop: for each rate a fresh server and the same
     seeded jobs (ORd-CiPA or TT06, 1-400 cells, 2-4 beats), submitted with Poisson gaps."""
     rates = [float(r) for r in args.serve_rates.split(",")]
+    beats = tuple(int(x) for x in args.serve_sweep_beats.split(","))
     sweep = []
     for rate in rates:
         log(f"bench: serving at {rate} jobs/s, {args.serve_jobs} jobs")
-        jobs = serving.synthetic_jobs(20251206, args.serve_jobs, beats=(2, 4))
-        srv = serving.Server(mem_budget=32 << 30, max_active=16)
-        r = _serve(srv, jobs, args.serve_jobs, rate)
+        jobs = serving.synthetic_jobs(20251206, args.serve_jobs, beats=beats)
+        srv = serving.Server(mem_budget=32 << 30)
+        r = _serve(srv, jobs, args.serve_jobs, rate, beats)
         log(f"bench: rate {rate}: p50 {r['p50']:.0f} ms, p99 {r['p99']:.0f} ms, "
             f"{r['cell_beats_per_s']:.0f} cell-beats/s")
         sweep.append({k: r[k] for k in ("rate_jobs_per_s", "p50", "p99", "cell_beats_per_s", "jobs",
-                                        "failed_cells")})
+                                        "failed_cells", "wall_s", "launches_per_pool")})
     best = max(sweep, key=lambda e: e["cell_beats_per_s"])
     print(json.dumps({
         "metric": "ORd cell-beats/s on 1 GPU at APD90 err <=1 ms; p50/p99 job latency",
@@ -564,8 +572,9 @@ def serving_main(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "f32-rates/f64-state",
         "data": "synthetic (seeded screening jobs)",
         "config": {"workload": f"serving sweep: {args.serve_jobs} jobs per rate, Poisson arrivals, cells "
-                               "log-uniform 1-400, 2-4 beats, ORd-CiPA or TT06 (p=0.5), at most 16 "
-                               "jobs on the device"},
+                               f"log-uniform 1-400, {beats[0]}-{beats[1]} beats, ORd-CiPA or TT06 (p=0.5), "
+                               "BCL 500/1000/2000 ms per job, one resident batch per model and BCL "
+                               "shared at the cell level"},
         "latency_ms": {"p50": best["p50"], "p99": best["p99"], "rate_jobs_per_s": best["rate_jobs_per_s"]},
         "sweep": sweep,
     }), flush=True)
@@ -753,9 +762,11 @@ def main():
     ap.add_argument("--legs", type=int, default=1, help="population: compact TT06 and tissue legs")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--serve-jobs", type=int, default=24)
-    ap.add_argument("--serve-rate", type=float, default=4.0)
-    ap.add_argument("--serve-rates", default="1,2,4,8,16,32,64", help="arrival rates of --workload serving")
+    ap.add_argument("--serve-jobs", type=int, default=120)
+    ap.add_argument("--serve-rate", type=float, default=8.0)
+    ap.add_argument("--serve-beats", default="10,40", help="population: beats per serving job (low,high)")
+    ap.add_argument("--serve-rates", default="2,8,32", help="arrival rates of --workload serving")
+    ap.add_argument("--serve-sweep-beats", default="10,100", help="--workload serving: beats per job (low,high)")
     ap.add_argument("--tune-cache", default=None, help="--workload tune: tuning cache file to write")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -838,7 +849,8 @@ def main():
     lat = {"p50": None, "p99": None}
     if args.serve_jobs > 0:
         log(f"bench: serving session, {args.serve_jobs} jobs")
-        lat = serving_latency(args.serve_jobs, args.serve_rate)
+        lat = serving_latency(args.serve_jobs, args.serve_rate,
+                              tuple(int(x) for x in args.serve_beats.split(",")))
     legs = {}
     if args.legs:
         legs = compact_legs(args)

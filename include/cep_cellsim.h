@@ -45,6 +45,7 @@ extern "C" {
 #define CEP_ST_ERR_FAIL 2      /* error test failed MAXNEF times (CV_ERR_FAILURE) */
 #define CEP_ST_NONFINITE 3
 #define CEP_ST_MAX_STEPS 4     /* more than 500 steps between two output points (CV_TOO_MUCH_WORK) */
+#define CEP_ST_IDLE 5          /* parked by cep_park_cells: no job in the cell, runs skip it */
 
 /* biomarker record per cell (doubles), cipa_t of CEPCell_NEO/modules/commons.hpp:18-36 */
 #define CEP_BM_QNET 0          /* integral of INaL+ICaL+Ito+IKr+IKs+IK1 over the selected beat (uA/uF ms) */
@@ -227,6 +228,25 @@ int cep_query(cep_batch *b);
    traces for beats up to max_bcl at output step dt, optional Vm trace); used for
    memory-based admission of jobs.  No reference counterpart. */
 long long cep_batch_device_bytes(int model, int ncells, double max_bcl, double dt, int vm_trace);
+
+/* Cells restarted and parked one by one, so that one resident batch serves many jobs: a job
+   takes free cells at a beat boundary and gives them back when it ends.  No reference
+   counterpart (each reference driver builds its cells, runs them to the end and deletes
+   them: `new ohara_rudy_cipa_v1_2017()` ... `initConsts(type, conc, herg, drug)` per cell,
+   CEP7Cell/modules/drugs_herg.cpp:113-145); a restart is that sequence for the listed cells
+   inside a live batch.
+   cep_restart_cells: each of the m distinct cells idx[] takes the parameters last given by
+   cep_set_cells, its model's initial states, a fresh integrator (CVodeInit at t = 0) and
+   biomarker machine, status CEP_ST_OK, and its own biomarker window: the last nwin of
+   nbeats_total beats counted from this restart.  Every other cell is left as it is.  ORd-CiPA
+   (BDF) and TT06 batches, after cep_init_consts and between runs (CEP_E_STATE while a run
+   is in flight); a cell's BCL may not exceed the longest BCL of the batch at cep_init_consts
+   (the Ca trace is sized for it); not with the driver rules on (they count beats for the
+   whole batch).  Blocks until the restart is on the device.
+   cep_park_cells: status CEP_ST_IDLE for the listed cells; every run skips them until they
+   are restarted. */
+int cep_restart_cells(cep_batch *b, const int *idx, int m, int nbeats_total, int nwin);
+int cep_park_cells(cep_batch *b, const int *idx, int m);
 
 /* Current STATES (n x nstates); set_states also restarts the integrators at t = 0 from
    the given states (the reference's Is_Using_Output restart, cipa_drug_bench.cpp:151). */
