@@ -9,25 +9,28 @@
 //   * nodes (nx+1) x (ny+1), index iy*(nx+1)+ix; the states of the node cells are the SoA
 //     state planes of an ORd batch (plane 0 = Vm, read and written in place by the solver);
 //   * A = M/dt + K and B = M/dt assembled on the host as nine FP64 coefficient planes
-//     (stencil slot s = (dy+1)*3 + (dx+1)), zero outside the grid;
-//   * one CG iteration = three kernels: SpMV q = A p on 32x32 node tiles with the p tile and
-//     its halo staged in LDS, fused with the block partial of p.q; the update x += a p,
-//     r -= a q, z = D^-1 r fused with the partials of r.z and z.z; the direction p = z + b p.
-//     Scalars (a, b, the stop test) are reduced on the device by the last block to finish
-//     (fixed summation order: results do not depend on scheduling), so iterations run
+//     (stencil slot s = (dy+1)*3 + (dx+1)), zero outside the grid, rows padded to a multiple
+//     of the tile width so that a tile row is whole 128 B lines;
+//   * one CG iteration = two kernels on the grid: the SpMV q = A p on 32x32 node tiles with
+//     the p tile and its halo staged in LDS, which also forms the direction p = z + b p_prev
+//     while staging and runs the previous iteration's stop test, fused with the block partial
+//     of p.q; then the vector update x += a p, r -= a q, z = D^-1 r fused with the partials of
+//     r.z and z.z.  (A mesh in CSR keeps a separate direction kernel.)  The scalars a, b and
+//     the stop test are re-reduced from the per-workgroup partials by every workgroup of the
+//     next kernel in one fixed order (results do not depend on scheduling), so iterations run
 //     without host round trips; the host checks the stop flag every few iterations.
-// Algorithmic HBM bytes per CG iteration and node: SpMV 9*8 (A) + 8 (p) + 8 (q) = 88;
-// update 8*5 read (x, p, r, q, dinv) + 8*3 written = 64; direction 16 read + 8 written =
-// 24; total 176 B/node/iteration.
+// Algorithmic HBM bytes per CG iteration and node: SpMV 9*8 (A) + 8 (z) + 8 (p_prev) read,
+// 8 (p) + 8 (q) written = 104; update 8*5 read (x, p, r, q, dinv) + 8*3 written = 64; total
+// 168 B/node/iteration.
 #pragma once
 
 namespace cep {
 namespace tis {
 
-// node tile per workgroup (16 waves).  Every workgroup re-reduces the previous kernel's
-// per-workgroup partials, so that traffic grows with the square of the workgroup count:
-// 32 x 32 tiles (257 workgroups at 513^2 nodes) against 16 x 16 (1028) took a CG update from
-// 14.4 to 12.6 us and a Strang step from 0.422 to 0.402 ms.
+// SpMV node tile per workgroup (16 waves).  The next kernels re-reduce the per-workgroup
+// partials of the previous one (every workgroup, the same fixed order), a cost that grows with
+// the square of the workgroup count: 32 x 32 tiles (257 workgroups at 513^2 nodes) against
+// 16 x 16 (1028) took a CG update from 14.4 to 12.6 us and a Strang step from 0.422 to 0.402 ms.
 constexpr int TX = 32, TY = 32, TB = TX * TY;
 
 struct Scal {            // device-resident CG state
@@ -35,7 +38,7 @@ struct Scal {            // device-resident CG state
   int done, iters, maxit;
 };
 
-// partial sums of the block reductions: [PQ | RZ(even) | RZ(odd) | ZZ | ZZ0] x nblk
+// partial sums of the block reductions: [PQ | RZ(even) | RZ(odd) | ZZ | ZZ0] x MAXBLK
 enum { P_PQ = 0, P_RZ0 = 1, P_RZ1 = 2, P_ZZ = 3, P_ZZ0 = 4, P_N = 5 };
 
 // workgroup sum of NV values; the result is valid in thread 0
@@ -52,76 +55,137 @@ CEP_D void block_sum(double (&v)[NV], double *sm)
 #pragma unroll
     for (int k = 0; k < NV; ++k) sm[wid * NV + k] = v[k];
   __syncthreads();
-  if (threadIdx.x == 0)
+  // the TB/64 wave sums by a fixed shuffle tree in the first wave (not a serial chain of LDS
+  // reads on one lane)
+  static_assert(TB / 64 == 16, "sixteen wave sums");
+  if (wid == 0)
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-      double s = 0.0;
-      for (int w = 0; w < TB / 64; ++w) s += sm[w * NV + k];
+      double s = lane < 16 ? sm[lane * NV + k] : 0.0;
+#pragma unroll
+      for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
       v[k] = s;
     }
 }
 
-// the total of one partial array (written by the previous kernel), computed identically by
-// every workgroup: fixed strided order per thread, fixed tree; broadcast through LDS
-CEP_D double grid_total(const double *part, int nblk, double *sm, double *bc)
+// the totals of NV partial arrays (part[k], cnt[k] values each, written by an earlier kernel),
+// computed identically by every workgroup -- fixed strided order per thread, fixed tree -- in
+// one reduction, and broadcast through LDS
+template <int NV>
+CEP_D void grid_totals(const double *const (&part)[NV], const int (&cnt)[NV], double (&tot)[NV], double *sm,
+                       double *bc)
 {
-  double v[1] = { 0.0 };
-  for (int i = threadIdx.x; i < nblk; i += TB) v[0] += part[i];
-  block_sum<1>(v, sm);
-  if (threadIdx.x == 0) *bc = v[0];
+  double v[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    v[k] = 0.0;
+    for (int i = threadIdx.x; i < cnt[k]; i += TB) v[k] += part[k][i];
+  }
+  block_sum<NV>(v, sm);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) bc[k] = v[k];
   __syncthreads();
-  return *bc;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) tot[k] = bc[k];
 }
 
 struct Grid {
-  int nxn, nyn, nn, nblk;   // nodes per row / column, nodes, workgroups
-  int lin;                  // 1: a mesh's nodes in a 1-D launch of TB-node workgroups
+  int nxn, nyn, nn;
+  int nblk;      // SpMV workgroups: 2-D tiles of the grid, or TB nodes of a mesh
+  int nblk1;     // vector-kernel workgroups: TB consecutive nodes
+  int cp;        // grid: row pitch of the coefficient planes (nxn rounded up to TX); plane = cp * nyn
+  int lin;       // 1: a mesh (CSR SpMV)
+  int maxblk;    // stride of the partial arrays
 };
 
 CEP_D int block_id() { return blockIdx.y * gridDim.x + blockIdx.x; }
 
-// this thread's node (-1 outside): a 32x32 tile of the grid, or TB consecutive mesh nodes
-CEP_D int node_index(const Grid &g)
+// the node of this thread in the vector kernels: TB consecutive nodes per workgroup
+CEP_D int node_1d(const Grid &g)
 {
-  if (g.lin) {
-    const int i = blockIdx.x * TB + threadIdx.x;
-    return i < g.nn ? i : -1;
-  }
-  const int ix = blockIdx.x * TX + threadIdx.x % TX, iy = blockIdx.y * TY + threadIdx.x / TX;
-  return (ix < g.nxn && iy < g.nyn) ? iy * g.nxn + ix : -1;
+  const int i = blockIdx.x * TB + threadIdx.x;
+  return i < g.nn ? i : -1;
 }
 
-// q = A p over a 32x32 tile (LDS-staged p with halo); partial p.q (mode 1)
-__global__ void __launch_bounds__(TB) cg_spmv(Grid g, const double *__restrict__ A, const double *__restrict__ p,
-                                              double *__restrict__ q, double *__restrict__ part, const Scal *S,
-                                              int mode)
+// The grid SpMV q = A p over a 32 x 32 tile, the tile of p and its one-node halo staged in LDS
+// (zero outside the grid).  Coefficient plane s at A[s * cp * nyn + iy * cp + ix]: with the
+// row pitch a multiple of the tile width, every tile row of a plane is two whole 128 B lines.
+// DIR: the CG direction of iteration k >= 1 is formed while staging, p_k = z + b p_(k-1) with
+// b = RZ(k) / RZ(k-1), read from pin and written to pout (two buffers: no workgroup overwrites
+// the previous direction that another one stages as its halo); the stop test of iteration k-1
+// comes first (||z|| <= rtol ||z0||, PETSc KSPCG's preconditioned norm, or k == maxit).  Every
+// workgroup computes the same totals in the same order, so every halo value equals what its
+// owner writes, and all workgroups take the same decision.  The partial of p.q goes to PQ.
+// The loads of the tile, the coefficients and the stop flag are issued before the reduction of
+// the partials (a scheduling barrier keeps them there), so their memory latencies overlap
+// instead of following one another.
+constexpr int NSTAGE = ((TX + 2) * (TY + 2) + TB - 1) / TB;   // tile elements per thread
+template <bool DIR>
+__global__ void __launch_bounds__(TB) cg_spmv(Grid g, const double *__restrict__ A, const double *__restrict__ z,
+                                              const double *__restrict__ pin, double *__restrict__ pout,
+                                              double *__restrict__ q, double *__restrict__ part, Scal *S, int k)
 {
-  if (mode && S->done) return;
   __shared__ double tile[TY + 2][TX + 2];
-  __shared__ double sm[TB / 64 * 2];
+  __shared__ double sm[TB / 64 * 4];
+  __shared__ double bc[4];
   const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
   const int ix = blockIdx.x * TX + tx, iy = blockIdx.y * TY + ty;
-  // stage p for the tile and its one-node halo (zero outside the grid)
-  for (int k = threadIdx.x; k < (TX + 2) * (TY + 2); k += TB) {
-    const int hx = k % (TX + 2), hy = k / (TX + 2);
+  const bool mine = ix < g.nxn && iy < g.nyn;
+  const bool form = DIR && k > 0;
+  double pv[NSTAGE], zv[NSTAGE], a[9];
+#pragma unroll
+  for (int u = 0; u < NSTAGE; ++u) {
+    const int m = threadIdx.x + u * TB;
+    const int hx = m % (TX + 2), hy = m / (TX + 2);
     const int gx = blockIdx.x * TX + hx - 1, gy = blockIdx.y * TY + hy - 1;
-    tile[hy][hx] = (gx >= 0 && gx < g.nxn && gy >= 0 && gy < g.nyn) ? p[gy * g.nxn + gx] : 0.0;
+    const bool in = m < (TX + 2) * (TY + 2) && gx >= 0 && gx < g.nxn && gy >= 0 && gy < g.nyn;
+    const int n = in ? gy * g.nxn + gx : 0;
+    pv[u] = in ? pin[n] : 0.0;
+    zv[u] = (in && form) ? z[n] : 0.0;
+  }
+  const size_t plane = (size_t)g.cp * g.nyn, o = mine ? (size_t)iy * g.cp + ix : 0;
+#pragma unroll
+  for (int sl = 0; sl < 9; ++sl) a[sl] = mine ? A[sl * plane + o] : 0.0;
+  const int done = DIR ? S->done : 0;
+  __builtin_amdgcn_sched_barrier(0);
+  if (done) return;
+  double bt = 0.0;
+  if (form) {
+    const double *pa[4] = { part + P_ZZ * g.maxblk, part + P_ZZ0 * g.maxblk,
+                            part + ((k & 1) ? P_RZ1 : P_RZ0) * g.maxblk, part + ((k & 1) ? P_RZ0 : P_RZ1) * g.maxblk };
+    const int ca[4] = { g.nblk1, g.nblk1, g.nblk1, g.nblk1 };
+    double tot[4];
+    grid_totals<4>(pa, ca, tot, sm, bc);     // zz, zz0, RZ(k), RZ(k-1)
+    if (tot[0] <= S->rtol2 * tot[1] || k >= S->maxit) {
+      if (block_id() == 0 && threadIdx.x == 0) { S->done = 1; S->iters = (tot[1] > 0.0) ? k : 0; }
+      return;
+    }
+    bt = tot[2] / tot[3];
+    if (block_id() == 0 && threadIdx.x == 0) S->iters = k;
+  }
+#pragma unroll
+  for (int u = 0; u < NSTAGE; ++u) {
+    const int m = threadIdx.x + u * TB;
+    if (m < (TX + 2) * (TY + 2)) tile[m / (TX + 2)][m % (TX + 2)] = form ? zv[u] + bt * pv[u] : pv[u];
   }
   __syncthreads();
   double v[1] = { 0.0 };
-  if (ix < g.nxn && iy < g.nyn) {
+  if (mine) {
     const int i = iy * g.nxn + ix;
     double s = 0.0;
 #pragma unroll
     for (int dy = -1; dy <= 1; ++dy)
 #pragma unroll
-      for (int dx = -1; dx <= 1; ++dx) s += A[((dy + 1) * 3 + (dx + 1)) * g.nn + i] * tile[ty + 1 + dy][tx + 1 + dx];
+      for (int dx = -1; dx <= 1; ++dx) s += a[(dy + 1) * 3 + (dx + 1)] * tile[ty + 1 + dy][tx + 1 + dx];
     q[i] = s;
-    v[0] = tile[ty + 1][tx + 1] * s;
+    const double pi = tile[ty + 1][tx + 1];
+    if (form) pout[i] = pi;
+    v[0] = pi * s;
   }
-  if (mode == 0) return;                        // plain product (b = B v)
+  if (!DIR) return;                              // plain product (b = B v)
   block_sum<1>(v, sm);
-  if (threadIdx.x == 0) part[P_PQ * g.nblk + block_id()] = v[0];
+  if (threadIdx.x == 0) part[P_PQ * g.maxblk + block_id()] = v[0];
 }
 
 // q = A p for a mesh matrix in CSR (columns ascending in each row); partial p.q (mode 1)
@@ -132,7 +196,7 @@ __global__ void __launch_bounds__(TB) cg_spmv_csr(Grid g, const int *__restrict_
 {
   if (mode && S->done) return;
   __shared__ double sm[TB / 64 * 2];
-  const int i = node_index(g);
+  const int i = node_1d(g);
   double v[1] = { 0.0 };
   if (i >= 0) {
     double s = 0.0;
@@ -142,7 +206,7 @@ __global__ void __launch_bounds__(TB) cg_spmv_csr(Grid g, const int *__restrict_
   }
   if (mode == 0) return;                        // plain product (b = B v)
   block_sum<1>(v, sm);
-  if (threadIdx.x == 0) part[P_PQ * g.nblk + block_id()] = v[0];
+  if (threadIdx.x == 0) part[P_PQ * g.maxblk + block_id()] = v[0];
 }
 
 // CG start from x = 0: r = b, z = D^-1 r, p = z; partials r.z (into RZ(even)) and z.z (ZZ0);
@@ -154,7 +218,7 @@ __global__ void __launch_bounds__(TB) cg_init(Grid g, const double *__restrict__
 {
   __shared__ double sm[TB / 64 * 2];
   if (block_id() == 0 && threadIdx.x == 0) { S->rtol2 = rtol2; S->done = 0; S->iters = 0; S->maxit = maxit; }
-  const int i = node_index(g);
+  const int i = node_1d(g);
   double v[2] = { 0.0, 0.0 };
   if (i >= 0) {
     const double ri = b[i], zi = dinv[i] * ri;
@@ -163,57 +227,66 @@ __global__ void __launch_bounds__(TB) cg_init(Grid g, const double *__restrict__
   }
   block_sum<2>(v, sm);
   if (threadIdx.x == 0) {
-    part[P_RZ0 * g.nblk + block_id()] = v[0];
-    part[P_ZZ0 * g.nblk + block_id()] = v[1];
+    part[P_RZ0 * g.maxblk + block_id()] = v[0];
+    part[P_ZZ0 * g.maxblk + block_id()] = v[1];
   }
 }
 
-// iteration k: a = RZ(k) / PQ; x += a p, r -= a q, z = D^-1 r; partials of r.z (RZ(k+1)) and z.z
+// iteration k: a = RZ(k) / PQ; x += a p, r -= a q, z = D^-1 r; partials of r.z (RZ(k+1)) and
+// z.z; TB consecutive nodes per workgroup (whole, aligned lines)
 __global__ void __launch_bounds__(TB) cg_update(Grid g, const double *__restrict__ p, const double *__restrict__ q,
                                                 const double *__restrict__ dinv, double *__restrict__ x,
                                                 double *__restrict__ r, double *__restrict__ z,
                                                 double *__restrict__ part, const Scal *S, int k)
 {
-  if (S->done) return;
   __shared__ double sm[TB / 64 * 2];
-  __shared__ double bc;
-  const double rz = grid_total(part + ((k & 1) ? P_RZ1 : P_RZ0) * g.nblk, g.nblk, sm, &bc);
-  const double pq = grid_total(part + P_PQ * g.nblk, g.nblk, sm, &bc);
-  const double a = (pq != 0.0) ? rz / pq : 0.0;    // b = 0: x stays 0
-  const int i = node_index(g);
+  __shared__ double bc[2];
+  // the vector loads and the stop flag first, then the reduction (their latencies overlap)
+  const int i = node_1d(g);
+  const int j = i >= 0 ? i : 0;
+  const double pj = p[j], qj = q[j], xj = x[j], rj = r[j], dj = dinv[j];
+  const int done = S->done;
+  __builtin_amdgcn_sched_barrier(0);
+  if (done) return;
+  const double *pa[2] = { part + ((k & 1) ? P_RZ1 : P_RZ0) * g.maxblk, part + P_PQ * g.maxblk };
+  const int ca[2] = { g.nblk1, g.nblk };
+  double tot[2];
+  grid_totals<2>(pa, ca, tot, sm, bc);
+  const double a = (tot[1] != 0.0) ? tot[0] / tot[1] : 0.0;    // b = 0: x stays 0
   double v[2] = { 0.0, 0.0 };
   if (i >= 0) {
-    x[i] += a * p[i];
-    const double ri = r[i] - a * q[i];
-    const double zi = dinv[i] * ri;
+    x[i] = xj + a * pj;
+    const double ri = rj - a * qj;
+    const double zi = dj * ri;
     r[i] = ri; z[i] = zi;
     v[0] = ri * zi; v[1] = zi * zi;
   }
   block_sum<2>(v, sm);
   if (threadIdx.x == 0) {
-    part[((k & 1) ? P_RZ0 : P_RZ1) * g.nblk + block_id()] = v[0];
-    part[P_ZZ * g.nblk + block_id()] = v[1];
+    part[((k & 1) ? P_RZ0 : P_RZ1) * g.maxblk + block_id()] = v[0];
+    part[P_ZZ * g.maxblk + block_id()] = v[1];
   }
 }
 
-// iteration k: stop test ||z|| <= rtol ||z0|| (preconditioned norm, as PETSc KSPCG) or
-// k+1 == maxit; else b = RZ(k+1) / RZ(k), p = z + b p
+// mesh tissue, iteration k: stop test ||z|| <= rtol ||z0|| (preconditioned norm, as PETSc
+// KSPCG) or k+1 == maxit; else b = RZ(k+1) / RZ(k), p = z + b p
 __global__ void __launch_bounds__(TB) cg_dir(Grid g, const double *__restrict__ z, double *__restrict__ p,
                                              const double *__restrict__ part, Scal *S, int k)
 {
   if (S->done) return;
-  __shared__ double sm[TB / 64 * 2];
-  __shared__ double bc;
-  const double zz = grid_total(part + P_ZZ * g.nblk, g.nblk, sm, &bc);
-  const double zz0 = grid_total(part + P_ZZ0 * g.nblk, g.nblk, sm, &bc);
-  if (zz <= S->rtol2 * zz0 || k + 1 >= S->maxit) {
-    if (block_id() == 0 && threadIdx.x == 0) { S->done = 1; S->iters = (zz0 > 0.0) ? k + 1 : 0; }
+  __shared__ double sm[TB / 64 * 4];
+  __shared__ double bc[4];
+  const double *pa[4] = { part + P_ZZ * g.maxblk, part + P_ZZ0 * g.maxblk,
+                          part + ((k & 1) ? P_RZ0 : P_RZ1) * g.maxblk, part + ((k & 1) ? P_RZ1 : P_RZ0) * g.maxblk };
+  const int ca[4] = { g.nblk1, g.nblk1, g.nblk1, g.nblk1 };
+  double tot[4];
+  grid_totals<4>(pa, ca, tot, sm, bc);        // zz, zz0, RZ(k+1), RZ(k)
+  if (tot[0] <= S->rtol2 * tot[1] || k + 1 >= S->maxit) {
+    if (block_id() == 0 && threadIdx.x == 0) { S->done = 1; S->iters = (tot[1] > 0.0) ? k + 1 : 0; }
     return;
   }
-  const double rz_new = grid_total(part + ((k & 1) ? P_RZ0 : P_RZ1) * g.nblk, g.nblk, sm, &bc);
-  const double rz_old = grid_total(part + ((k & 1) ? P_RZ1 : P_RZ0) * g.nblk, g.nblk, sm, &bc);
-  const double bt = rz_new / rz_old;
-  const int i = node_index(g);
+  const double bt = tot[2] / tot[3];
+  const int i = node_1d(g);
   if (i >= 0) p[i] = z[i] + bt * p[i];
   if (block_id() == 0 && threadIdx.x == 0) S->iters = k + 1;
 }
@@ -353,6 +426,8 @@ struct cep_tissue {
   cep::tis::S2State *d_s2 = nullptr;
   double *d_A = nullptr, *d_B = nullptr, *d_dinv = nullptr;
   double *d_b = nullptr, *d_r = nullptr, *d_z = nullptr, *d_p = nullptr, *d_q = nullptr, *d_part = nullptr;
+  double *d_p2 = nullptr;                  // grid: the second direction buffer (directions alternate)
+  int cp = 0;                              // grid: row pitch of the coefficient planes
   cep::tis::Scal *d_S = nullptr;
   std::vector<void *> allocs;
   std::vector<hipEvent_t> ev;              // pairs for kernel timing
@@ -599,6 +674,20 @@ static int tissue_build(cep_tissue *t, double dt)
   if (t->mesh) tissue_assemble_mesh(t, dt, A, Bm, dinv);
   else tissue_assemble(t, dt, A, Bm, dinv);
   hipStream_t s = t->cells->stream;
+  if (!t->mesh) {
+    // nine planes with the row pitch padded to the tile width (tile rows on whole lines)
+    const size_t plane = (size_t)t->cp * t->nyn;
+    std::vector<double> Ap(9 * plane, 0.0), Bp(9 * plane, 0.0);
+    for (int sl = 0; sl < 9; ++sl)
+      for (int iy = 0; iy < t->nyn; ++iy)
+        for (int ix = 0; ix < t->nxn; ++ix) {
+          const size_t from = (size_t)sl * t->nn + (size_t)iy * t->nxn + ix, to = sl * plane + (size_t)iy * t->cp + ix;
+          Ap[to] = A[from];
+          Bp[to] = Bm[from];
+        }
+    A.swap(Ap);
+    Bm.swap(Bp);
+  }
   TCHK(t, hipMemcpyAsync(t->d_A, A.data(), A.size() * sizeof(double), hipMemcpyHostToDevice, s));
   TCHK(t, hipMemcpyAsync(t->d_B, Bm.data(), Bm.size() * sizeof(double), hipMemcpyHostToDevice, s));
   TCHK(t, hipMemcpyAsync(t->d_dinv, dinv.data(), dinv.size() * sizeof(double), hipMemcpyHostToDevice, s));
@@ -625,47 +714,93 @@ static void tissue_timed_collect(cep_tissue *t, int slot)
   }
 }
 
-// one diffusion solve A x = B v with v = x on entry (x: device vector of nn)
+static cep::tis::Grid tissue_grid(const cep_tissue *t)
+{
+  using namespace cep::tis;
+  Grid g{};
+  g.nn = t->nn;
+  g.nblk1 = (t->nn + TB - 1) / TB;
+  if (t->mesh) {
+    g.lin = 1;
+    g.nblk = g.nblk1;
+  } else {
+    g.nxn = t->nxn; g.nyn = t->nyn; g.cp = t->cp;
+    g.nblk = ((t->nxn + TX - 1) / TX) * ((t->nyn + TY - 1) / TY);
+  }
+  g.maxblk = std::max(g.nblk, g.nblk1);
+  return g;
+}
+
+// one diffusion solve A x = B v with v = x on entry (x: device vector of nn).  Grid: two
+// kernels per iteration (the SpMV with the direction update and the previous stop test, then
+// the vector update); mesh: three (CSR SpMV, vector update, direction).  The host reads the
+// stop flag every CHUNK iterations.
 static int tissue_solve(cep_tissue *t, double dt, double *x, double rtol, int maxit, int *iters)
 {
   using namespace cep::tis;
   if (int rc = tissue_build(t, dt)) return rc;
   hipStream_t s = t->cells->stream;
-  const int nlin = (t->nn + TB - 1) / TB;
-  const Grid g = t->mesh ? Grid{ 0, 0, t->nn, nlin, 1 }
-                         : Grid{ t->nxn, t->nyn, t->nn, ((t->nxn + TX - 1) / TX) * ((t->nyn + TY - 1) / TY), 0 };
-  const dim3 grid = t->mesh ? dim3(nlin) : dim3((t->nxn + TX - 1) / TX, (t->nyn + TY - 1) / TY), blk(TB);
-  // q = M p with the matrix of this tissue (grid stencil or mesh CSR)
-  auto spmv = [&](const double *M, const double *p, double *q, int mode) {
-    if (t->mesh)
-      hipLaunchKernelGGL(cg_spmv_csr, grid, blk, 0, s, g, t->d_rp, t->d_ci, M, p, q, t->d_part, t->d_S, mode);
-    else
-      hipLaunchKernelGGL(cg_spmv, grid, blk, 0, s, g, M, p, q, t->d_part, t->d_S, mode);
-  };
-  spmv(t->d_B, x, t->d_b, 0);                                                                   // b = B v
-  hipLaunchKernelGGL(cg_init, grid, blk, 0, s, g, t->d_b, t->d_dinv, x, t->d_r, t->d_z, t->d_p, t->d_part, t->d_S,
+  const Grid g = tissue_grid(t);
+  const dim3 g2 = t->mesh ? dim3(g.nblk) : dim3((t->nxn + TX - 1) / TX, (t->nyn + TY - 1) / TY);
+  const dim3 g1(g.nblk1), blk(TB);
+  if (t->mesh)
+    hipLaunchKernelGGL(cg_spmv_csr, g1, blk, 0, s, g, t->d_rp, t->d_ci, t->d_B, x, t->d_b, t->d_part, t->d_S, 0);
+  else
+    hipLaunchKernelGGL(cg_spmv<false>, g2, blk, 0, s, g, t->d_B, nullptr, x, nullptr, t->d_b, t->d_part, t->d_S, 0);
+  hipLaunchKernelGGL(cg_init, g1, blk, 0, s, g, t->d_b, t->d_dinv, x, t->d_r, t->d_z, t->d_p, t->d_part, t->d_S,
                      rtol * rtol, maxit);
   TCHK(t, hipGetLastError());
   constexpr int CHUNK = 4;
   Scal h{};
-  for (int it = 0; it < maxit; it += CHUNK) {
-    for (int k = it; k < it + CHUNK && k < maxit; ++k) {
-      // time the first iteration of each chunk (later ones may exit early once converged)
-      const bool tm = k == it;
-      if (tm) tissue_timed_begin(t, 1);
-      spmv(t->d_A, t->d_p, t->d_q, 1);
-      if (tm) { tissue_timed_end(t, 1); tissue_timed_begin(t, 2); }
-      hipLaunchKernelGGL(cg_update, grid, blk, 0, s, g, t->d_p, t->d_q, t->d_dinv, x, t->d_r, t->d_z, t->d_part,
-                         t->d_S, k);
-      if (tm) tissue_timed_end(t, 2);
-      hipLaunchKernelGGL(cg_dir, grid, blk, 0, s, g, t->d_z, t->d_p, t->d_part, t->d_S, k);
+  double *pb[2] = { t->d_p, t->d_p2 };
+  if (t->mesh) {
+    for (int it = 0; it < maxit; it += CHUNK) {
+      for (int k = it; k < it + CHUNK && k < maxit; ++k) {
+        // time the first iteration of each chunk (later ones may exit early once converged)
+        const bool tm = k == it;
+        if (tm) tissue_timed_begin(t, 1);
+        hipLaunchKernelGGL(cg_spmv_csr, g1, blk, 0, s, g, t->d_rp, t->d_ci, t->d_A, t->d_p, t->d_q, t->d_part,
+                           t->d_S, 1);
+        if (tm) { tissue_timed_end(t, 1); tissue_timed_begin(t, 2); }
+        hipLaunchKernelGGL(cg_update, g1, blk, 0, s, g, t->d_p, t->d_q, t->d_dinv, x, t->d_r, t->d_z, t->d_part,
+                           t->d_S, k);
+        if (tm) tissue_timed_end(t, 2);
+        hipLaunchKernelGGL(cg_dir, g1, blk, 0, s, g, t->d_z, t->d_p, t->d_part, t->d_S, k);
+      }
+      TCHK(t, hipGetLastError());
+      TCHK(t, hipMemcpyAsync(&h, t->d_S, sizeof(Scal), hipMemcpyDeviceToHost, s));
+      TCHK(t, hipStreamSynchronize(s));
+      tissue_timed_collect(t, 1);
+      tissue_timed_collect(t, 2);
+      if (h.done) break;
     }
-    TCHK(t, hipGetLastError());
-    TCHK(t, hipMemcpyAsync(&h, t->d_S, sizeof(Scal), hipMemcpyDeviceToHost, s));
-    TCHK(t, hipStreamSynchronize(s));
-    tissue_timed_collect(t, 1);
-    tissue_timed_collect(t, 2);
-    if (h.done) break;
+  } else {
+    // SpMV k (with the stop test after update k-1 and direction k), update k, ...; SpMV
+    // maxit only ends the solve
+    // direction k is read from pb[0] at k = 0 (cg_init's p = z), else formed from direction k-1
+    // in pb[(k-1) & 1] and written to pb[k & 1]
+    auto spmv = [&](int k) {
+      hipLaunchKernelGGL(cg_spmv<true>, g2, blk, 0, s, g, t->d_A, t->d_z, pb[k == 0 ? 0 : (k - 1) & 1], pb[k & 1],
+                         t->d_q, t->d_part, t->d_S, k);
+    };
+    spmv(0);
+    for (int k = 0; k < maxit;) {
+      for (int c = 0; c < CHUNK && k < maxit; ++c, ++k) {
+        // time the first iteration of each chunk (later ones may exit early once converged)
+        if (c == 0) tissue_timed_begin(t, 2);
+        hipLaunchKernelGGL(cg_update, g1, blk, 0, s, g, pb[k & 1], t->d_q, t->d_dinv, x, t->d_r, t->d_z,
+                           t->d_part, t->d_S, k);
+        if (c == 0) { tissue_timed_end(t, 2); tissue_timed_begin(t, 1); }
+        spmv(k + 1);
+        if (c == 0) tissue_timed_end(t, 1);
+      }
+      TCHK(t, hipGetLastError());
+      TCHK(t, hipMemcpyAsync(&h, t->d_S, sizeof(Scal), hipMemcpyDeviceToHost, s));
+      TCHK(t, hipStreamSynchronize(s));
+      tissue_timed_collect(t, 1);
+      tissue_timed_collect(t, 2);
+      if (h.done) break;
+    }
   }
   if (iters) *iters = h.iters;
   return 0;
@@ -751,10 +886,12 @@ cep_tissue *cep_tissue_create_model(int nx, int ny, double hx, double hy, int mo
   t->sxx.assign((size_t)nx * ny, 0.0);
   t->sxy.assign((size_t)nx * ny, 0.0);
   t->syy.assign((size_t)nx * ny, 0.0);
-  const int nblk = ((t->nxn + cep::tis::TX - 1) / cep::tis::TX) * ((t->nyn + cep::tis::TY - 1) / cep::tis::TY);
+  t->cp = (t->nxn + cep::tis::TX - 1) / cep::tis::TX * cep::tis::TX;
+  const int nblk = tissue_grid(t).maxblk;
   int rc = 0;
-  rc |= talloc(t, &t->d_A, (size_t)9 * t->nn);
-  rc |= talloc(t, &t->d_B, (size_t)9 * t->nn);
+  rc |= talloc(t, &t->d_A, (size_t)9 * t->cp * t->nyn);
+  rc |= talloc(t, &t->d_B, (size_t)9 * t->cp * t->nyn);
+  rc |= talloc(t, &t->d_p2, t->nn);
   rc |= talloc(t, &t->d_dinv, t->nn);
   rc |= talloc(t, &t->d_b, t->nn);
   rc |= talloc(t, &t->d_r, t->nn);
@@ -820,7 +957,7 @@ cep_tissue *cep_tissue_create_mesh(const cep_mesh *m, int model, int precision)
   t->dten.assign((size_t)t->ne * 6, 0.0);
   t->cells = batch_create(model, t->nn, precision, false);
   if (!t->cells) { delete t; return nullptr; }
-  const int nblk = (t->nn + cep::tis::TB - 1) / cep::tis::TB;
+  const int nblk = tissue_grid(t).maxblk;
   int rc = 0;
   rc |= talloc(t, &t->d_A, (size_t)t->nnz);
   rc |= talloc(t, &t->d_B, (size_t)t->nnz);

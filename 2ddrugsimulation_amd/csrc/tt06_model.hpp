@@ -218,8 +218,19 @@ CEP_D void rush_larsen(double t, double dt, double *__restrict__ y, const PA &P,
 #pragma unroll
   for (int g = 0; g < 12; ++g) {
     const int i = Gates::idx[g];
-    const R e = Mx::ex(-(R)dt * Mx::rcp(tau[g]));
-    y[i] = (y[i] - (double)inf[g]) * (double)e + (double)inf[g];
+    if constexpr (sizeof(R) == sizeof(float)) {
+      // FP32 rates: exp(-x) rounded to FP32 is off by up to 6e-8 near 1, which for a slow gate
+      // (x = dt/tau ~ 1e-4 at dt 0.02 ms) is ~1e-3 of the step's change, with the same sign at
+      // every step.  The change is formed from 1 - exp(-x) at its own relative accuracy instead:
+      // the Taylor series to x^4 below x = 1/32 (truncation x^4/120 < 1e-8 of it).
+      const R x = (R)dt * Mx::rcp(tau[g]);
+      const R d = x < (R)0.03125 ? x * ((R)1 - x * ((R)0.5 - x * ((R)(1.0 / 6.0) - x * (R)(1.0 / 24.0))))
+                                 : (R)1 - Mx::ex(-x);
+      y[i] += ((double)inf[g] - y[i]) * (double)d;
+    } else {
+      const R e = Mx::ex(-(R)dt * Mx::rcp(tau[g]));
+      y[i] = (y[i] - (double)inf[g]) * (double)e + (double)inf[g];
+    }
   }
   y[0] += dt * f[0];
   y[18] += f[18] * dt;

@@ -345,9 +345,10 @@ def _serve(srv, jobs, njobs: int, rate: float, beats):
 
 # Tissue (monodomain) workload: 512 x 512 Q1 elements, 0.1 mm, one seeded drug at 2 x Cmax.
 TISSUE_NX, TISSUE_H, TISSUE_DT = 512, 0.1, 0.01
-# algorithmic HBM bytes per node and CG iteration (FP64): SpMV 9 coefficients + p + q = 88,
-# update x r z read+write with p q dinv read = 64, direction z p read + p write = 24
-B_SPMV, B_UPDATE, B_DIR = 88, 64, 24
+# algorithmic HBM bytes per node and CG iteration (FP64): the SpMV with the direction update
+# fused, 9 coefficients + z + previous p read, new p + q written = 104; the vector update, x r
+# read+write, p q dinv read, z written = 64
+B_SPMV, B_UPDATE = 104, 64
 PEAK_HBM = 8.0e12
 TISSUE_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "tissue_monodomain_513x513_traffic.txt")
 

@@ -206,6 +206,33 @@ def test_tt06_population_rules_vs_oracle(prec):
             assert abs(dr["apd"][i] - r[0]) <= 2.0 and abs(dr["mean_cai"][i] - r[1]) <= 5e-3 * abs(r[1])
 
 
+def test_tt06_mixed_tracks_fp64_over_fifty_beats():
+    """C3's FP32-rate kernel against the FP64 kernel (itself pinned to the oracle beat by beat)
+    over 50 beats of 64 population cells of all three cell types: APD90 within 0.1 ms and CaD90
+    within 1 ms (the Ca trace's 1 ms grid) on every cell, Vm within 1e-3 mV at the end.  The
+    FP32 Rush-Larsen factor 1 - exp(-dt/tau) is formed at its own relative accuracy; rounding
+    exp(-dt/tau) itself to FP32 had drifted Vm by 2e-4 mV in 20 beats."""
+    n, nb = 64, 50
+    pop = cep.workload.tt_population(20251205, 4096)
+    sel = np.linspace(0, 4095, n).astype(int)
+    herg = np.concatenate([pop["ic50"], pop["hill"]], axis=1)[sel]
+    out = {}
+    for prec in (cep.PREC_MIXED, cep.PREC_FP64):
+        with cep.CellBatch(cep.MODEL_TT06, n, prec) as b:
+            b.set_cells(celltype=pop["celltype"][sel], conc=pop["conc"][sel], herg=herg,
+                        gscale=pop["gscale"][sel], bcl=pop["bcl"][sel])
+            b.set_run_config(dt=0.02, nbeats_total=nb, nwin=1)
+            b.init_consts()
+            b.run_beats(nb)
+            out[prec] = (b.biomarkers(), b.states(), b.status())
+    (bm, y, st), (bm64, y64, st64) = out[cep.PREC_MIXED], out[cep.PREC_FP64]
+    assert (st == 0).all() and (st64 == 0).all()
+    assert set(pop["celltype"][sel].tolist()) == {0.0, 1.0, 2.0}
+    assert np.abs(bm["apd90"] - bm64["apd90"]).max() <= 0.1
+    assert np.abs(bm["cad90"] - bm64["cad90"]).max() <= 1.0
+    assert np.abs(y[:, 0] - y64[:, 0]).max() <= 1e-3
+
+
 def test_tissue_strang_step_matches_the_oracle_composition():
     """Strang splitting: reaction over dt/2 (ORd 2011 nodes: computeRates with the tissue stimulus +
     solveAnalytical, the reference's scheme), the FEM diffusion solve over dt, reaction over dt/2 at
