@@ -238,8 +238,12 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
   }
   if (is_out) {
     Biomarker &bm = s_bm[slot];
+    // a cell's own window (cells restarted one by one) in the beat launches, which are the
+    // ones that run such cells; the output-point launches keep the batch's window and the
+    // code they had before the per-cell window existed (their FP64 build integrated nothing
+    // with this one load added: the code of this kernel has been sensitive to such changes)
     bm_config(bm, B.prm[ord::OP_BCL * B.pitch + c], B.prm[ord::OP_STIM_START * B.pitch + c], rc, B.cap,
-              B.win[c]);
+              kBeats ? B.win[c] : -1);
     const int iout0 = B.si[11 * B.pitch + c];
     const bool fresh = B.si[4 * B.pitch + c] == PH_FRESH;
     if (fresh) bm.begin(B.y[c], B.y[9 * B.pitch + c]);

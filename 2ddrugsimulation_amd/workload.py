@@ -87,3 +87,26 @@ def tt_population(seed: int, ncell: int, sigma: float = 0.25, bcl: float = 1000.
     gscale = np.exp(rng.normal(0.0, sigma, size=(ncell, 10)))
     return {"celltype": celltype, "conc": conc, "ic50": ic50, "hill": hill,
             "gscale": gscale, "bcl": np.full(ncell, bcl)}
+
+
+def group_slots(bcl: np.ndarray, per_group: dict | None = None, group: int = 16) -> np.ndarray:
+    """Slot layout of an ORd-CiPA population for the BDF kernel, whose wavefronts hold `group`
+    cells each: cells ordered by BCL (longest first), each BCL class filling groups of `group`
+    slots with `per_group[bcl]` cells (default: all `group`), the rest of a group padding.
+    Returns the cell of each slot, -1 for padding (to be parked with ``park_cells``).
+
+    A wavefront's pass costs nearly the same with fewer cells, but a group of long beats with
+    fewer cells meets fewer cells' rare events (factorisations, Jacobians) per pass, and the
+    long beats set a launch's time; padding the long-beat groups trades slots for time while the
+    groups still all start at once."""
+    per_group = per_group or {}
+    slots = []
+    for b in sorted(set(np.asarray(bcl, dtype=float).tolist()), reverse=True):
+        idx = np.nonzero(bcl == b)[0]
+        c = int(per_group.get(b, group))
+        if not 1 <= c <= group:
+            raise ValueError(f"cells per group for BCL {b}: {c}")
+        for g0 in range(0, len(idx), c):
+            part = list(idx[g0:g0 + c])
+            slots += part + [-1] * (group - len(part))
+    return np.array(slots, dtype=np.int64)

@@ -86,6 +86,29 @@ def make_batch(pop, prec, nbeats_total):
     return b
 
 
+def parse_group_cells(spec: str) -> dict:
+    """'2000:8,1000:16' -> {2000.0: 8, 1000.0: 16} (cells per 16-slot wavefront by BCL)."""
+    out = {}
+    for part in filter(None, spec.split(",")):
+        k, v = part.split(":")
+        out[float(k)] = int(v)
+    return out
+
+
+def make_slot_batch(pop, prec, nbeats_total, per_group: dict):
+    """The population in the BDF kernel's wavefront groups laid out by BCL class
+    (workload.group_slots): padding slots parked.  Returns the batch and each cell's slot."""
+    slots = cep.workload.group_slots(pop["bcl"], per_group)
+    real = slots >= 0
+    src = np.where(real, slots, slots[real][0])
+    b = make_batch({k: v[src] for k, v in pop.items()}, prec, nbeats_total)
+    if not real.all():
+        b.park_cells(np.nonzero(~real)[0])
+    pos = np.empty(len(pop["bcl"]), dtype=np.int64)
+    pos[slots[real]] = np.nonzero(real)[0]
+    return b, pos
+
+
 def flops_of(stats: np.ndarray) -> float:
     """Counted flops of the integration work recorded in the per-cell statistics
     (steps, rhs evals incl. Jacobian columns, jacobians, setups, Newton iterations, ...,
@@ -546,6 +569,31 @@ def tune_main(args):
         cases.append({"workload": name, "cells": n, "candidates": rows, "chosen": chosen, "default": default,
                       "default_ms": dms[0] if dms else None, "chosen_ms": rows[best]["ms"],
                       "speedup_vs_default": (dms[0] / rows[best]["ms"]) if dms else None})
+    # the population's wavefront layout by BCL class (host side: cells per 16-slot group of the
+    # BDF kernel, padding parked): one warm beat, then one timed beat per candidate; results
+    # are bitwise those of the 16-cell layout
+    pop = make_population(10000)
+    lay = []
+    ref = None
+    for spec in ("", "2000:12", "2000:8", "2000:6", "2000:8,1000:12"):
+        pg = parse_group_cells(spec)
+        b, pos = make_slot_batch(pop, cep.PREC_MIXED, 200, pg)
+        b.run_beats(1)
+        t0 = time.perf_counter()
+        b.run_beats(1)
+        ms = 1e3 * (time.perf_counter() - t0)
+        bm = b.biomarker_array()[pos]
+        same = True if ref is None else bool(np.array_equal(bm, ref))
+        ref = bm if ref is None else ref
+        lay.append({"cells_per_wavefront_by_bcl": spec or "16 everywhere", "slots": b.n, "ms": ms,
+                    "bitwise_equal_to_16": same})
+        log(f"bench: layout '{spec or '16'}': {b.n} slots, {ms:.1f} ms per beat")
+        b.close()
+    best = min(lay, key=lambda r: r["ms"])
+    cases.append({"workload": "ORd-CiPA BDF, 10,000-cell population, wavefront layout by BCL class",
+                  "cells": 10000, "candidates": lay, "chosen": best["cells_per_wavefront_by_bcl"],
+                  "default_ms": lay[0]["ms"], "chosen_ms": best["ms"],
+                  "speedup_vs_default": lay[0]["ms"] / best["ms"]})
     print(json.dumps({"metric": "launch-shape autotuner: one-beat trial per candidate", "unit": "ms",
                       "higher_is_better": False, "n_gpus": 1, "cases": cases}), flush=True)
 
@@ -753,6 +801,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--tissue-nx", type=int, default=TISSUE_NX)
     ap.add_argument("--ncell", type=int, default=10000)
+    ap.add_argument("--group-cells", default="2000:8",
+                    help="population: cells per 16-slot BDF wavefront by BCL ('bcl:cells,...'; '' = 16 everywhere)")
     ap.add_argument("--precision", choices=("mixed", "fp64"), default=None,
                     help="population default: mixed; tissue default: fp64")
     ap.add_argument("--check-cells", type=int, default=64, help="tt06: FP64 check cells")
@@ -798,22 +848,24 @@ def main():
     pop = make_population(args.ncell)
     n = len(pop["bcl"])
     nb_total = 200
-    b = make_batch(pop, prec, nb_total)
-    log(f"bench: {n} cells, warmup {W} beat(s)")
+    per_group = parse_group_cells(args.group_cells)
+    b, pos = make_slot_batch(pop, prec, nb_total, per_group)
+    log(f"bench: {n} cells in {b.n} slots, warmup {W} beat(s)")
     if W:
         b.run_beats(W)
-    states_w = b.states()               # the state the timed beats start from (CPU baseline)
+    states_w = b.states()[pos]          # the state the timed beats start from (CPU baseline)
     b.kernel_time()                     # reset the event timer
-    stats0 = b.stats()
+    stats0 = b.stats()[pos]
     log(f"bench: timing {K} beat(s)")
     t0 = time.perf_counter()
     for _ in range(K):
         b.run_beats(1)
     t1 = time.perf_counter()
     nl, kavg_ms, ktot_ms = b.kernel_time()
-    stats1 = b.stats()
-    st = b.status()
-    bm_timed = b.biomarker_array()
+    stats1 = b.stats()[pos]
+    st = b.status()[pos]
+    bm_timed = b.biomarker_array()[pos]
+    nslots = b.n
     elapsed = t1 - t0
     value = n * K / elapsed
     flops_launch = flops_of(stats1 - stats0) / max(nl, 1)
@@ -828,7 +880,7 @@ def main():
         oprec = cep.PREC_FP64 if prec == cep.PREC_MIXED else cep.PREC_MIXED
         log(f"bench: {'FP64' if oprec == cep.PREC_FP64 else 'mixed'} batch of {n} cells, "
             f"1 + {args.fp64_beats} beat(s)")
-        bo = make_batch(pop, oprec, nb_total)
+        bo, _ = make_slot_batch(pop, oprec, nb_total, per_group)
         bo.run_beats(1)
         t2 = time.perf_counter()
         bo.run_beats(args.fp64_beats)
@@ -875,7 +927,9 @@ def main():
         "data": "synthetic (seeded drug/IC50/conductance population)",
         "config": {"workload": "ORd-CiPA batched population, 10,000 cells = 50 drugs x 4 conc x 50 "
                                "variants, BCL {500,1000,2000} ms, BDF rtol 1e-6/atol 1e-8, step = 1 beat",
-                   "cells": n, "beats_per_job": nb_total},
+                   "cells": n, "beats_per_job": nb_total,
+                   "layout": {"cells_per_wavefront_by_bcl": {str(int(k)): v for k, v in per_group.items()},
+                              "default_cells_per_wavefront": 16, "slots": nslots}},
         "accuracy_vs_oracle": chk,
         "other_precision_full_batch": other,
         "failed_cells": int(np.sum(st != 0)),

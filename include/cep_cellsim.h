@@ -238,7 +238,8 @@ long long cep_batch_device_bytes(int model, int ncells, double max_bcl, double d
    cep_restart_cells: each of the m distinct cells idx[] takes the parameters last given by
    cep_set_cells, its model's initial states, a fresh integrator (CVodeInit at t = 0) and
    biomarker machine, status CEP_ST_OK, and its own biomarker window: the last nwin of
-   nbeats_total beats counted from this restart.  Every other cell is left as it is.  ORd-CiPA
+   nbeats_total beats counted from this restart (read by the beat runs, cep_run_beats and
+   cep_run_beats_async; the output-point runs keep the batch's window).  Every other cell is left as it is.  ORd-CiPA
    (BDF) and TT06 batches, after cep_init_consts and between runs (CEP_E_STATE while a run
    is in flight); a cell's BCL may not exceed the longest BCL of the batch at cep_init_consts
    (the Ca trace is sized for it); not with the driver rules on (they count beats for the
