@@ -402,13 +402,15 @@ CEP_D void rhs(double t, const YV &y, FV &&f, const PA &P, Currents *cur, GA &&g
 
 // The 9x9 transition matrix of the Markov IKr block (states 34..42: IC1, IC2, C1, C2, O,
 // IO, IObound, Obound, Cbound) at voltage v: d/dt S = A S, the fluxes of rhs() above
-// (:1059-1068) written as a matrix (columns sum to zero), in FP64.
-template <typename PA>
+// (:1059-1068) written as a matrix (columns sum to zero).  The matrix is FP64; its
+// voltage-dependent rate constants exp(B v) are evaluated in the rate precision R.
+template <typename R = double, typename PA>
 CEP_D void markov_matrix(double v, const PA &P, double (&A)[9][9])
 {
+  using Mx = M<R>;
   double k[14];
 #pragma unroll
-  for (int i = 0; i < 14; ++i) k[i] = g_markov.AQ[i] * exp(g_markov.B[i] * v);
+  for (int i = 0; i < 14; ++i) k[i] = g_markov.AQ[i] * (double)Mx::ex((R)(g_markov.B[i] * v));
   const double k1 = k[0], k2 = k[1], k3m = k[2], k4m = k[3], k11 = k[4], k21 = k[5], k31 = k[6], k41 = k[7];
   const double k51 = k[8], k52 = k[9], k53 = k[10], k61 = k[11], k62 = k[12], k63 = k[13];
   const double Ku = P(OP_KU), Kt = P(OP_KT), bind = P(OP_BIND);

@@ -117,9 +117,9 @@ CEP_D int node_1d(const Grid &g)
 // comes first (||z|| <= rtol ||z0||, PETSc KSPCG's preconditioned norm, or k == maxit).  Every
 // workgroup computes the same totals in the same order, so every halo value equals what its
 // owner writes, and all workgroups take the same decision.  The partial of p.q goes to PQ.
-// The loads of the tile, the coefficients and the stop flag are issued before the reduction of
-// the partials (a scheduling barrier keeps them there), so their memory latencies overlap
-// instead of following one another.
+// After the stop flag (launches queued past the end of the solve return at once), the loads of
+// the tile and the coefficients are issued before the reduction of the partials (a scheduling
+// barrier keeps them there), so their memory latencies overlap instead of following one another.
 constexpr int NSTAGE = ((TX + 2) * (TY + 2) + TB - 1) / TB;   // tile elements per thread
 template <bool DIR>
 __global__ void __launch_bounds__(TB) cg_spmv(Grid g, const double *__restrict__ A, const double *__restrict__ z,
@@ -133,6 +133,7 @@ __global__ void __launch_bounds__(TB) cg_spmv(Grid g, const double *__restrict__
   const int ix = blockIdx.x * TX + tx, iy = blockIdx.y * TY + ty;
   const bool mine = ix < g.nxn && iy < g.nyn;
   const bool form = DIR && k > 0;
+  if (DIR && S->done) return;                    // launches queued past the end of the solve
   double pv[NSTAGE], zv[NSTAGE], a[9];
 #pragma unroll
   for (int u = 0; u < NSTAGE; ++u) {
@@ -147,9 +148,7 @@ __global__ void __launch_bounds__(TB) cg_spmv(Grid g, const double *__restrict__
   const size_t plane = (size_t)g.cp * g.nyn, o = mine ? (size_t)iy * g.cp + ix : 0;
 #pragma unroll
   for (int sl = 0; sl < 9; ++sl) a[sl] = mine ? A[sl * plane + o] : 0.0;
-  const int done = DIR ? S->done : 0;
   __builtin_amdgcn_sched_barrier(0);
-  if (done) return;
   double bt = 0.0;
   if (form) {
     const double *pa[4] = { part + P_ZZ * g.maxblk, part + P_ZZ0 * g.maxblk,
@@ -241,13 +240,12 @@ __global__ void __launch_bounds__(TB) cg_update(Grid g, const double *__restrict
 {
   __shared__ double sm[TB / 64 * 2];
   __shared__ double bc[2];
-  // the vector loads and the stop flag first, then the reduction (their latencies overlap)
+  if (S->done) return;                           // launches queued past the end of the solve
+  // the vector loads first, then the reduction (their latencies overlap)
   const int i = node_1d(g);
   const int j = i >= 0 ? i : 0;
   const double pj = p[j], qj = q[j], xj = x[j], rj = r[j], dj = dinv[j];
-  const int done = S->done;
   __builtin_amdgcn_sched_barrier(0);
-  if (done) return;
   const double *pa[2] = { part + ((k & 1) ? P_RZ1 : P_RZ0) * g.maxblk, part + P_PQ * g.maxblk };
   const int ca[2] = { g.nblk1, g.nblk };
   double tot[2];
@@ -341,7 +339,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) ti
   ord::rhs<R>(t, s, f, P, &cur, a);
   // Markov block: (I - h A) S1 = S0, no pivoting (an M-matrix)
   double M[9][9], b[9];
-  ord::markov_matrix(s[0], P, M);
+  ord::markov_matrix<R>(s[0], P, M);
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
 #pragma unroll
@@ -368,9 +366,24 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) ti
 #pragma unroll
   for (int i = 0; i < ord::NS; ++i) {
     double v;
-    if (i >= 34 && i <= 42) v = b[i - 34];
-    else if (a[i] > 0.0) v = s[i] + f[i] * (-expm1(-h * a[i]) / a[i]);   // Rush-Larsen
-    else v = s[i] + h * f[i];
+    if (i >= 34 && i <= 42) {
+      v = b[i - 34];
+    } else if (a[i] > 0.0) {
+      // Rush-Larsen: s + f (1 - exp(-h a)) / a = s + f h phi(h a), phi(x) = (1 - e^-x) / x
+      if constexpr (sizeof(R) == sizeof(float)) {
+        // FP32 rates: phi at FP32 relative accuracy (its Taylor series to x^4 below x = 1/32,
+        // truncation x^5/720 < 1e-10), so the FP64 state takes the step's change without the
+        // rounding of exp(-x) to FP32 near 1
+        const float x = (float)(h * a[i]);
+        const float ph = x < 0.03125f ? 1.f - x * (0.5f - x * ((1.f / 6.f) - x * ((1.f / 24.f) - x * (1.f / 120.f))))
+                                      : (1.f - __expf(-x)) / x;
+        v = s[i] + f[i] * h * (double)ph;
+      } else {
+        v = s[i] + f[i] * (-expm1(-h * a[i]) / a[i]);
+      }
+    } else {
+      v = s[i] + h * f[i];
+    }
     y[(size_t)i * pitch + c] = v;
   }
 }

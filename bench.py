@@ -652,6 +652,16 @@ def tissue_main(args):
     ach = spmv_bytes / (avg[1] * 1e-3) if avg[1] > 0 else 0.0
     cpu = None if args.no_cpu_baseline else tissue_cpu_baseline(tis, tensors, drug, nx)
     tis.close()
+    # FP32 rates: the same W + K steps in FP64 for the Vm difference they make
+    acc = None
+    if prec == cep.PREC_MIXED:
+        log(f"bench: FP64 tissue for the same {W + K} steps")
+        t64, _, _ = tissue_setup(nx, cep.PREC_FP64)
+        t64.step(W + K, TISSUE_DT, cep.SPLIT_STRANG)
+        d = np.abs(vm - t64.vm())
+        t64.close()
+        acc = {"vm_linf_mv_vs_fp64": float(d.max()), "vm_median_abs_mv_vs_fp64": float(np.median(d)),
+               "steps": W + K, "simulated_ms": (W + K) * TISSUE_DT}
     out = {
         "metric": "tissue node-steps/s (2-D monodomain, ORd-CiPA nodes, Q1 FEM + Jacobi-PCG)",
         "value": nn * K / elapsed,
@@ -681,6 +691,7 @@ def tissue_main(args):
                      "bytes_per_launch": spmv_bytes,
                      "update_gbs": nn * B_UPDATE / (avg[2] * 1e-3) / 1e9 if avg[2] > 0 else None},
         "cpu_baseline": cpu,
+        "accuracy_vs_fp64": acc,
     }
     print(json.dumps(out), flush=True)
 
@@ -804,7 +815,7 @@ def main():
     ap.add_argument("--group-cells", default="2000:8",
                     help="population: cells per 16-slot BDF wavefront by BCL ('bcl:cells,...'; '' = 16 everywhere)")
     ap.add_argument("--precision", choices=("mixed", "fp64"), default=None,
-                    help="population default: mixed; tissue default: fp64")
+                    help="default mixed (FP32 rates; FP64 state, BDF/LU, CG)")
     ap.add_argument("--check-cells", type=int, default=64, help="tt06: FP64 check cells")
     ap.add_argument("--check-per-bcl", type=int, default=32,
                     help="population: oracle-checked cells per BCL class (0: no accuracy check)")
@@ -838,7 +849,7 @@ def main():
         tt06_main(args)
         return
     if args.workload == "tissue":
-        args.precision = args.precision or "fp64"
+        args.precision = args.precision or "mixed"
         tissue_main(args)
         return
     args.precision = args.precision or "mixed"
@@ -952,7 +963,8 @@ def main():
 
 def compact_legs(args):
     """Short runs of the two other configurations, timed by the same driver run: one beat of the
-    200,000-cell TT06 population (mixed) and 100 Strang steps of the 512 x 512 tissue (FP64)."""
+    200,000-cell TT06 population (mixed) and 100 Strang steps of the 512 x 512 tissue (FP32 rates,
+    FP64 state and CG; its Vm against an FP64 run of the same steps)."""
     legs = {}
     pop = cep.workload.tt_population(20251205, args.tt_cells)
     herg = np.concatenate([pop["ic50"], pop["hill"]], axis=1)
@@ -977,7 +989,7 @@ def compact_legs(args):
                                  "unit": "TFLOP/s", "frac": fl / (kavg * 1e-3) / PEAK_FP32,
                                  "valu_counters": read_valu("tt")}}
     log("bench: tissue leg, 512 x 512, 20 + 100 steps")
-    tis, _, _ = tissue_setup(TISSUE_NX, cep.PREC_FP64)
+    tis, _, _ = tissue_setup(TISSUE_NX, cep.PREC_MIXED)
     tis.step(20, TISSUE_DT, cep.SPLIT_STRANG)
     tis.kernel_times()
     t0 = time.perf_counter()
@@ -985,9 +997,16 @@ def compact_legs(args):
     t1 = time.perf_counter()
     avg, _ = tis.kernel_times()
     nn = tis.nn
+    vm = tis.vm()
     tis.close()
-    legs["tissue"] = {"workload": "512x512 Q1 monodomain, ORd-CiPA nodes, Strang dt 0.01 ms, FP64 CG",
+    t64, _, _ = tissue_setup(TISSUE_NX, cep.PREC_FP64)
+    t64.step(120, TISSUE_DT, cep.SPLIT_STRANG)
+    dv = np.abs(vm - t64.vm())
+    t64.close()
+    legs["tissue"] = {"workload": "512x512 Q1 monodomain, ORd-CiPA nodes, Strang dt 0.01 ms, FP32 rates, "
+                                  "FP64 state and CG",
                       "node_steps_per_s": nn * 100 / (t1 - t0), "cg_iters_per_step": iters / 100,
+                      "vm_linf_mv_vs_fp64": float(dv.max()),
                       "cg_spmv_gbs": nn * B_SPMV / (avg[1] * 1e-3) / 1e9 if avg[1] > 0 else None,
                       "kernels_avg_ms": {"tissue_react": float(avg[0]), "cg_spmv": float(avg[1]),
                                          "cg_update": float(avg[2])}}
