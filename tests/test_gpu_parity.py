@@ -192,6 +192,42 @@ def test_ord_cell_result_independent_of_batch_neighbours():
     np.testing.assert_array_equal(res["alone"][1], res["mix"][1])
 
 
+def test_ord_population_layout_by_bcl_is_bitwise_neutral():
+    """The bench's layout (BCL 2000 cells eight to a 16-slot wavefront group, the rest of each
+    group parked, `workload.group_slots`) gives bitwise the results of the dense layout, for
+    cells of all three BCLs, in both precisions; parked slots stay idle."""
+    pop = cep.workload.ord_population(20251204, 2, 2, 6)          # 24 cells
+    order = np.argsort(pop["bcl"], kind="stable")
+    pop = {k: v[order] for k, v in pop.items()}
+    assert set(pop["bcl"].tolist()) == {500.0, 1000.0, 2000.0}
+    n = len(pop["bcl"])
+    for prec in (cep.PREC_FP64, cep.PREC_MIXED):
+        out = {}
+        for per_group in (None, {2000.0: 8, 1000.0: 5}):
+            slots = cep.workload.group_slots(pop["bcl"], per_group)
+            real = slots >= 0
+            src = np.where(real, slots, slots[real][0])
+            with cep.CellBatch(cep.MODEL_ORD_CIPA, len(slots), prec) as b:
+                b.set_cells(celltype=pop["celltype"][src], conc=pop["conc"][src], herg=pop["herg"][src],
+                            gscale=pop["gscale"][src], bcl=pop["bcl"][src])
+                b.set_run_config(dt=0.1, rtol=1e-6, atol=1e-8, nbeats_total=2, nwin=2)
+                b.init_consts()
+                y0 = b.states()
+                if not real.all():
+                    b.park_cells(np.nonzero(~real)[0])
+                b.run_beats(2)
+                pos = np.empty(n, dtype=np.int64)
+                pos[slots[real]] = np.nonzero(real)[0]
+                st = b.status()
+                out[per_group is None] = (b.states()[pos], b.biomarker_array()[pos], st[pos])
+                assert (st[~real] == cep.ST_IDLE).all()
+                np.testing.assert_array_equal(b.states()[~real], y0[~real])
+        dense, laid = out[True], out[False]
+        assert (dense[2] == 0).all()
+        for a, c in zip(dense, laid):
+            np.testing.assert_array_equal(a, c)
+
+
 def test_failing_cell_does_not_poison_batch():
     """A cell driven non-finite is flagged; its neighbours finish normally."""
     pop = _ord_pop(4)

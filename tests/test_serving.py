@@ -336,3 +336,37 @@ def test_restart_and_park_refuse_bad_calls():
         b.init_consts()
         with pytest.raises(cep.CepError, match="ORd-CiPA and TT06"):
             b.restart_cells([0], 1)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("model", [cep.MODEL_ORD_CIPA, cep.MODEL_TT06])
+def test_restarted_cells_keep_their_own_biomarker_window(model):
+    """A restart with a window shorter than the job (the last 2 of 4 beats) selects the same beat
+    and gives bitwise the biomarkers of a batch configured with that window, while the batch's
+    other cells keep the batch's own window (all of their beats)."""
+    tt = 1.0 if model == cep.MODEL_TT06 else 0.0
+    job = serving.synthetic_jobs(31, 1, max_cells=6, beats=(4, 4), tt06_fraction=tt, bcl_choices=(500.0,))[0]
+    job.nwin = 2
+    c = job.cells
+    n = job.ncells
+    with cep.CellBatch(model, n, cep.PREC_MIXED) as b:
+        b.set_cells(celltype=c["celltype"], conc=c["conc"], herg=c["herg"], gscale=c["gscale"], bcl=c["bcl"])
+        b.set_run_config(dt=job.step_dt(), nbeats_total=4, nwin=2)
+        b.init_consts()
+        b.run_beats(4)
+        want = b.biomarker_array()
+    cap = n + 3
+    with cep.CellBatch(model, cap, cep.PREC_MIXED) as b:
+        b.set_cells(celltype=np.zeros(cap), conc=np.zeros(cap), bcl=np.full(cap, 500.0))
+        b.set_cells(celltype=c["celltype"], conc=c["conc"], herg=c["herg"], gscale=c["gscale"], bcl=c["bcl"])
+        b.set_run_config(dt=job.step_dt(), nbeats_total=4, nwin=4)
+        b.init_consts()
+        b.restart_cells(np.arange(n), 4, 2)
+        b.run_beats(4)
+        got = b.biomarker_array()
+        st = b.status()
+    assert (st == 0).all()
+    np.testing.assert_array_equal(got[:n], want)
+    steep = list(cep.BM_FIELDS).index("steepest_pace")
+    assert (got[:n, steep] >= 2).all()          # the selected beat lies in the job's window
+    assert (got[n:, steep] <= 3).all()
