@@ -6,6 +6,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -2800,12 +2801,16 @@ int cep_step_rl(cep_batch *b, double t, double dt)
     else hipLaunchKernelGGL(tt_step_kernel<float>, dim3(blocks), dim3(256), 0, b->stream, b->n, b->pitch, t, dt, b->d_y, b->d_prm);
   } else if (b->model == CEP_MODEL_ORD2011) {
     const int blocks = (b->n + 63) / 64;
-    if (f64) hipLaunchKernelGGL(cep::tis::tissue_react11<double>, dim3(blocks), dim3(64), 0, b->stream, b->n, b->pitch, t, dt, -1, b->d_y, b->d_prm);
-    else hipLaunchKernelGGL(cep::tis::tissue_react11<float>, dim3(blocks), dim3(64), 0, b->stream, b->n, b->pitch, t, dt, -1, b->d_y, b->d_prm);
+    if (f64) hipLaunchKernelGGL(cep::tis::tissue_react11<double>, dim3(blocks), dim3(64), 0, b->stream, b->n, b->pitch, t, dt, -1, b->d_y, b->d_prm,
+                                (const int *)nullptr, b->pitch);
+    else hipLaunchKernelGGL(cep::tis::tissue_react11<float>, dim3(blocks), dim3(64), 0, b->stream, b->n, b->pitch, t, dt, -1, b->d_y, b->d_prm,
+                                (const int *)nullptr, b->pitch);
   } else {
     const int blocks = (b->n + 63) / 64;
-    if (f64) hipLaunchKernelGGL(cep::tis::tissue_react<double>, dim3(blocks), dim3(64), 0, b->stream, b->n, b->pitch, t, dt, b->d_y, b->d_prm);
-    else hipLaunchKernelGGL(cep::tis::tissue_react<float>, dim3(blocks), dim3(64), 0, b->stream, b->n, b->pitch, t, dt, b->d_y, b->d_prm);
+    if (f64) hipLaunchKernelGGL(cep::tis::tissue_react<double>, dim3(blocks), dim3(64), 0, b->stream, b->n, b->pitch, t, dt, b->d_y, b->d_prm,
+                                (const int *)nullptr, b->pitch);
+    else hipLaunchKernelGGL(cep::tis::tissue_react<float>, dim3(blocks), dim3(64), 0, b->stream, b->n, b->pitch, t, dt, b->d_y, b->d_prm,
+                            (const int *)nullptr, b->pitch);
   }
   HIPCHK(b, hipGetLastError());
   HIPCHK(b, hipStreamSynchronize(b->stream));

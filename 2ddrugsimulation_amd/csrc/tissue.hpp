@@ -325,9 +325,15 @@ __global__ void __launch_bounds__(256) tissue_s2(int target, int pitch, int ns, 
 // CaMK and the charge accumulators.  All rates at the start of the step.
 // at most 256 registers, two waves per SIMD (measured, 263,169 nodes in FP64: 108 us per
 // launch against 116 us with 326 registers and one wave)
+// Node parameters come from a table of the distinct parameter rows (`ptab`, row r's slot k at
+// ptab[k * nclass + r]) and each node's row index (`pcls`): the nodes of a tissue share a few
+// rows (cell type, drug, paced or not), so a node reads one index instead of its 30 doubles,
+// and the table stays in cache.  pcls null: every cell its own row (ptab = the batch's SoA
+// parameters, nclass = its pitch).
 template <typename R>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) tissue_react(int n, int pitch, double t, double h, double *__restrict__ y,
-                                                   const double *__restrict__ prm)
+                                                   const double *__restrict__ ptab, const int *__restrict__ pcls,
+                                                   int nclass)
 {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
@@ -335,7 +341,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) ti
 #pragma unroll
   for (int i = 0; i < ord::NS; ++i) { s[i] = y[(size_t)i * pitch + c]; a[i] = 0.0; }
   ord::Currents cur;
-  const PrmAcc P{prm + c, pitch};
+  const PrmAcc P{ptab + (pcls ? pcls[c] : c), nclass};
   ord::rhs<R>(t, s, f, P, &cur, a);
   // Markov block: (I - h A) S1 = S0, no pivoting (an M-matrix)
   double M[9][9], b[9];
@@ -393,14 +399,15 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) ti
 // (isS1 ? amp : 0; unpaced nodes carry amp = 0) then solveAnalytical(h).
 template <typename R>
 __global__ void __launch_bounds__(64) tissue_react11(int n, int pitch, double t, double h, int stim_on,
-                                                     double *__restrict__ y, const double *__restrict__ prm)
+                                                     double *__restrict__ y, const double *__restrict__ ptab,
+                                                     const int *__restrict__ pcls, int nclass)
 {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
   double s[ord11::NS], f[ord11::NS];
 #pragma unroll
   for (int i = 0; i < ord11::NS; ++i) s[i] = y[(size_t)i * pitch + c];
-  const PrmAcc P{prm + c, pitch};
+  const PrmAcc P{ptab + (pcls ? pcls[c] : c), nclass};
   ord11::Gates g;
   ord11::Currents cur;
   ord11::rhs<R>(t, s, f, P, cur, g, stim_on);
@@ -440,6 +447,10 @@ struct cep_tissue {
   double *d_A = nullptr, *d_B = nullptr, *d_dinv = nullptr;
   double *d_b = nullptr, *d_r = nullptr, *d_z = nullptr, *d_p = nullptr, *d_q = nullptr, *d_part = nullptr;
   double *d_p2 = nullptr;                  // grid: the second direction buffer (directions alternate)
+  // node parameters as a table of distinct rows and a row index per node
+  double *d_ptab = nullptr;
+  int *d_pcls = nullptr;
+  int nclass = 0, ptab_cap = 0;
   int cp = 0;                              // grid: row pitch of the coefficient planes
   cep::tis::Scal *d_S = nullptr;
   std::vector<void *> allocs;
@@ -819,9 +830,14 @@ static int tissue_solve(cep_tissue *t, double dt, double *x, double rtol, int ma
   return 0;
 }
 
+static int tissue_upload_params(cep_tissue *t);
+
 static int tissue_react_launch(cep_tissue *t, double h)
 {
   cep_batch *b = t->cells;
+  // the parameter table of the nodes (built by every parameter change; here if none came yet)
+  if (t->nclass == 0)
+    if (int rc = tissue_upload_params(t)) return rc;
   const int blocks = (t->nn + 63) / 64;
   tissue_timed_begin(t, 0);
   if (t->model == CEP_MODEL_ORD2011) {
@@ -831,20 +847,20 @@ static int tissue_react_launch(cep_tissue *t, double h)
     const int on = (!t->s1_stopped && tb >= t->stim_start && tb <= t->stim_start + t->stim_dur) ? 1 : 0;
     if (b->prec == CEP_PREC_FP64)
       hipLaunchKernelGGL(cep::tis::tissue_react11<double>, dim3(blocks), dim3(64), 0, b->stream, t->nn, b->pitch, t->t,
-                         h, on, b->d_y, b->d_prm);
+                         h, on, b->d_y, t->d_ptab, t->d_pcls, t->nclass);
     else
       hipLaunchKernelGGL(cep::tis::tissue_react11<float>, dim3(blocks), dim3(64), 0, b->stream, t->nn, b->pitch, t->t,
-                         h, on, b->d_y, b->d_prm);
+                         h, on, b->d_y, t->d_ptab, t->d_pcls, t->nclass);
     tissue_timed_end(t, 0);
     TCHK(t, hipGetLastError());
     return 0;
   }
   if (b->prec == CEP_PREC_FP64)
     hipLaunchKernelGGL(cep::tis::tissue_react<double>, dim3(blocks), dim3(64), 0, b->stream, t->nn, b->pitch, t->t, h,
-                       b->d_y, b->d_prm);
+                       b->d_y, t->d_ptab, t->d_pcls, t->nclass);
   else
     hipLaunchKernelGGL(cep::tis::tissue_react<float>, dim3(blocks), dim3(64), 0, b->stream, t->nn, b->pitch, t->t, h,
-                       b->d_y, b->d_prm);
+                       b->d_y, t->d_ptab, t->d_pcls, t->nclass);
   tissue_timed_end(t, 0);
   TCHK(t, hipGetLastError());
   return 0;
@@ -874,6 +890,27 @@ static int tissue_upload_params(cep_tissue *t)
     for (int k = 0; k < b->np; ++k) prm[k * P + i] = p[k];
   }
   TCHK(t, hipMemcpyAsync(b->d_prm, prm.data(), prm.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  // the distinct rows (bitwise) and each node's row: the reaction kernels read these
+  std::map<std::vector<double>, int> rows;
+  std::vector<int> cls(t->nn);
+  std::vector<double> row(b->np);
+  for (int i = 0; i < t->nn; ++i) {
+    for (int k = 0; k < b->np; ++k) row[k] = prm[k * P + i];
+    auto it = rows.emplace(row, (int)rows.size()).first;
+    cls[i] = it->second;
+  }
+  const int K = (int)rows.size();
+  std::vector<double> tab((size_t)K * b->np);
+  for (const auto &r : rows)
+    for (int k = 0; k < b->np; ++k) tab[(size_t)k * K + r.second] = r.first[k];
+  if (K > t->ptab_cap) {
+    if (talloc(t, &t->d_ptab, (size_t)K * b->np)) return CEP_E_HIP;
+    t->ptab_cap = K;
+  }
+  if (!t->d_pcls && talloc(t, &t->d_pcls, (size_t)t->nn)) return CEP_E_HIP;
+  TCHK(t, hipMemcpyAsync(t->d_ptab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  TCHK(t, hipMemcpyAsync(t->d_pcls, cls.data(), cls.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
+  t->nclass = K;
   TCHK(t, hipStreamSynchronize(b->stream));
   return 0;
 }
