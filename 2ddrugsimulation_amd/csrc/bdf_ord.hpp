@@ -297,10 +297,6 @@ struct Lane {
   // load/store interleaving would serialise every load on memory latency.
   static constexpr int CH = 13;
   static_assert(N % CH == 0, "whole chunks");
-  // sweeps over the whole Nordsieck array hold six rows per component: smaller chunks,
-  // whose loads still issue together, keep them in registers
-  static constexpr int CHZ = 4;
-  static_assert(N % CHZ == 0, "whole chunks");
 
   // CVODE's weights from zn[0]: 1 / (rtol |y| + atol)
   CEP_D void set_ewt()
@@ -330,45 +326,12 @@ struct Lane {
   // undo a failed prediction (order q0), change the order (cvIncreaseBDF / cvDecreaseBDF
   // with coefficients ll and a1, order q0), rescale row j by eta^j (order q1), predict
   // (order q1).  All of it is linear in a component's column (zn[0..5][i]), so the
-  // operations are first composed into one upper-triangular 6x6 map M (applied to the unit
-  // columns, in registers), then one sweep computes M z for the 52 components with no
-  // order-dependent branches.  Rows above the new order, except the saved correction in
-  // row QMAX, leave the map as zeros: the array keeps rows q+1..QMAX-1 at zero, which the
-  // interpolation and the other sweeps rely on.  y receives the new zn[0].
-  template <typename YV>
-  CEP_D void attempt_sweep(int ops, int q0, int q1, const double *ll, double a1, double eta, const YV &y)
-  {
-    double M[QMAX + 1][QMAX + 1];
-    build_map(ops, q0, q1, ll, a1, eta, M);
-#pragma unroll
-    for (int c0 = 0; c0 < N; c0 += CHZ) {
-      double z[CHZ][QMAX + 1];
-#pragma unroll
-      for (int k = 0; k < CHZ; ++k)
-#pragma unroll
-        for (int j = 0; j <= QMAX; ++j) z[k][j] = ZN(j, c0 + k);
-      __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-      for (int k = 0; k < CHZ; ++k) {
-        double w[QMAX + 1];
-#pragma unroll
-        for (int r = 0; r <= QMAX; ++r) {
-          double acc = 0.0;
-#pragma unroll
-          for (int col = QMAX; col >= r; --col) acc += M[r][col] * z[k][col];
-          w[r] = acc;
-        }
-#pragma unroll
-        for (int r = 0; r <= QMAX; ++r) z[k][r] = w[r];
-        y[c0 + k] = w[0];
-      }
-#pragma unroll
-      for (int k = 0; k < CHZ; ++k)
-#pragma unroll
-        for (int j = 0; j <= QMAX; ++j) ZN(j, c0 + k) = z[k][j];
-    }
-  }
-  // the composed map of attempt_sweep (applied to the unit columns)
+  // operations are first composed into one upper-triangular 6x6 map M (build_map: applied
+  // to the unit columns, in registers), then the second wave sweep (output_attempt_part)
+  // computes M z for the 52 components with no order-dependent branches.  Rows above the new
+  // order, except the saved correction in row QMAX, leave the map as zeros: the array keeps
+  // rows q+1..QMAX-1 at zero, which the interpolation and the other sweeps rely on.  The
+  // staging vector receives the new zn[0].
   CEP_D static void build_map(int ops, int q0, int q1, const double *ll, double a1, double eta,
                               double (&M)[QMAX + 1][QMAX + 1])
   {
@@ -504,7 +467,7 @@ struct Lane {
     }
   }
   // order change dq at the current order (CVODE's cvAdjustOrder: no decrease below 2),
-  // folded into `ops`/`ll`/`a1` for attempt_sweep
+  // folded into `ops`/`ll`/`a1` for build_map
   CEP_D int order_ops(int dq, double *ll, double &a1) const
   {
     a1 = 0.0;
@@ -900,33 +863,35 @@ struct Lane {
   {
     const int mm = m.mnewt;
     const double gm = m.gamma, r1 = m.rl1;
+    // half the components per chunk: two memory round trips per loop (chunks of 13 took four)
+    constexpr int NCH = N / 2;
 #pragma unroll
-    for (int c0 = 0; c0 < N; c0 += CH) {
-      double z1[CH], a[CH];
+    for (int c0 = 0; c0 < N; c0 += NCH) {
+      double z1[NCH], a[NCH];
       fetch(z1, V.zn, N + c0);
       if (mm > 0) fetch(a, V.acor, c0);
       else {
 #pragma unroll
-        for (int k = 0; k < CH; ++k) a[k] = 0.0;
+        for (int k = 0; k < NCH; ++k) a[k] = 0.0;
       }
 #pragma unroll
-      for (int k = 0; k < CH; ++k) y[c0 + k] = gm * y[c0 + k] - (r1 * z1[k] + a[k]);
+      for (int k = 0; k < NCH; ++k) y[c0 + k] = gm * y[c0 + k] - (r1 * z1[k] + a[k]);
     }
     solve(y);
     m.nni++;
     double s2 = 0.0, sa = 0.0;
 #pragma unroll
-    for (int c0 = 0; c0 < N; c0 += CH) {
-      double a[CH], z[CH], w[CH];
+    for (int c0 = 0; c0 < N; c0 += NCH) {
+      double a[NCH], z[NCH], w[NCH];
 #pragma unroll
-      for (int k = 0; k < CH; ++k) {
+      for (int k = 0; k < NCH; ++k) {
         a[k] = mm > 0 ? ACOR(c0 + k) : 0.0;
         z[k] = ZN(0, c0 + k);
         w[k] = EWT(c0 + k);
       }
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-      for (int k = 0; k < CH; ++k) {
+      for (int k = 0; k < NCH; ++k) {
         const int i = c0 + k;
         const double bi = y[i];
         a[k] += bi;
@@ -936,7 +901,7 @@ struct Lane {
         sa += u * u;
       }
 #pragma unroll
-      for (int k = 0; k < CH; ++k) ACOR(c0 + k) = a[k];
+      for (int k = 0; k < NCH; ++k) ACOR(c0 + k) = a[k];
     }
     const double del = sqrt(s2 / N);
     if (mm > 0) { const double r = del / m.delp; m.crate = (CRDOWN * m.crate > r) ? CRDOWN * m.crate : r; }
@@ -959,7 +924,7 @@ struct Lane {
   // zn[QMAX], weighted with this step's ewt); the error weights of the next step.
   // ---------------- array sweeps split across the wavefront: the four lanes of group g =
   // 0..3 of a cell each take NQ consecutive components.  Per component the arithmetic is
-  // that of complete_step / interpolate / attempt_sweep; a sum over components (the
+  // that of CVODE's cvCompleteStep / CVodeGetDky / the composed map; a sum over components (the
   // order-selection norms) is left as per-component terms in LDS and added in component
   // order by the integrator lane, so the results do not depend on the split.
   static constexpr int NGRP = 4, NQ = N / NGRP;
@@ -969,7 +934,7 @@ struct Lane {
     return BElem{r, (ln & 63u) * 8u + (unsigned)g * (NQ * CPB * 8u), kc * CPB * 8};
   }
 
-  // scalar part of complete_step before its sweep (what the sweep and complete_end need
+  // scalar part of the completed step (cvCompleteStep) before its sweep (what the sweep and complete_end need
   // goes to the cell's LDS scalars)
   CEP_D void complete_begin(double dsm)
   {
@@ -989,7 +954,7 @@ struct Lane {
     m.c_cquot = up ? (m.tq5 / m.saved_tq5) * hpow(m.h / m.tau[2], (double)(q + 1)) : 0.0;
     m.c_dsm = dsm;
   }
-  // complete_step's sweep over group g's components; the norm terms go to t2 / t3
+  // the completed step's sweep over group g's components; the norm terms go to t2 / t3
   template <typename TV>
   CEP_D void complete_part(int g, const TV &t2, const TV &t3) const
   {
@@ -999,7 +964,9 @@ struct Lane {
     for (int j = 0; j <= QMAX; ++j) { l[j] = (j <= q) ? m.l[j] : 0.0; oh[j] = (j == q) ? 1.0 : 0.0; }
     const bool save = m.c_save;
     const double cquot = m.c_cquot, rt = B.rtol, at_ = B.atol;
-    constexpr int C4 = 4;
+    // all of the lane's components in one chunk: one memory round trip per sweep (chunks of
+    // four components took four, 2.6% of the population's beat time)
+    constexpr int C4 = NQ;
 #pragma unroll
     for (int c0 = 0; c0 < NQ; c0 += C4) {
       constexpr int dummy = 0;
@@ -1042,7 +1009,7 @@ struct Lane {
       }
     }
   }
-  // complete_step after its sweep: the norms from the per-component terms, in component order
+  // the completed step after its sweep: the norms from the per-component terms, in component order
   template <typename TV>
   CEP_D void complete_end(const TV &t2, const TV &t3)
   {
@@ -1072,10 +1039,10 @@ struct Lane {
     prepare_next(m.c_dsm, s2, s3, m.c_dn, m.c_up);
     m.etamax = (m.nst <= SMALL_NST) ? ETAMX2 : ETAMX3;
   }
-  // interpolate() (at s = m.i_s) and/or the attempt map (apply_map with M from LDS) over group
+  // interpolate() (at s = m.i_s) and/or the attempt map (apply_map with M, entry k at Mp[k] in LDS) over group
   // g's components; with `ycopy` the interpolated values also become the cell's STATES
   template <typename YV>
-  CEP_D void output_attempt_part(int g, bool interp, bool ycopy, const YV &yo, bool attempt, const double *Mp,
+  CEP_D void output_attempt_part(int g, bool interp, bool ycopy, const YV &yo, bool attempt, const YV &Mp,
                                  const YV &yp) const
   {
     const double s = m.i_s;
@@ -1085,7 +1052,7 @@ struct Lane {
 #pragma unroll
       for (int k = 0; k < NMAP; ++k) M[k] = Mp[k];
     }
-    constexpr int C4 = 4;
+    constexpr int C4 = NQ;   // one chunk, one memory round trip (as in complete_part)
 #pragma unroll
     for (int c0 = 0; c0 < NQ; c0 += C4) {
       double z[C4][QMAX + 1];
@@ -1127,65 +1094,6 @@ struct Lane {
           }
       }
     }
-  }
-
-  CEP_D void complete_step(double dsm)
-  {
-    m.nst++;
-    const int q = m.q;
-#pragma unroll
-    for (int i = QMAX; i >= 2; --i) if (i <= q) m.tau[i] = m.tau[i - 1];
-    if ((q == 1) && (m.nst > 1)) m.tau[2] = m.tau[1];
-    m.tau[1] = m.h;
-    const bool save = (m.qwait == 2) && (q != B.maxord);   // qwait becomes 1
-    m.qwait--;
-    // prepare_next needs the norms only when it may change the order
-    const bool norms = (m.etamax != 1.0) && (m.qwait == 0);
-    const bool dn = norms && q > 1;
-    const bool up = norms && (q != B.maxord) && (m.saved_tq5 != 0.0);
-    const double cquot = up ? (m.tq5 / m.saved_tq5) * hpow(m.h / m.tau[2], (double)(q + 1)) : 0.0;
-    // coefficients masked to the current order (no order-dependent branches in the sweep);
-    // oh selects row q for the order-decrease norm
-    double l[QMAX + 1], oh[QMAX + 1];
-#pragma unroll
-    for (int j = 0; j <= QMAX; ++j) { l[j] = (j <= q) ? m.l[j] : 0.0; oh[j] = (j == q) ? 1.0 : 0.0; }
-    const double rt = B.rtol, at_ = B.atol;
-    double s2 = 0.0, s3 = 0.0;
-#pragma unroll
-    for (int c0 = 0; c0 < N; c0 += CHZ) {
-      double z[CHZ][QMAX + 1], a[CHZ], w[CHZ];
-#pragma unroll
-      for (int k = 0; k < CHZ; ++k) {
-#pragma unroll
-        for (int j = 0; j <= QMAX; ++j) z[k][j] = ZN(j, c0 + k);
-        a[k] = ACOR(c0 + k);
-        w[k] = EWT(c0 + k);
-      }
-      __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-      for (int k = 0; k < CHZ; ++k) {
-        const double zqm = z[k][QMAX];
-        double zq = 0.0;
-#pragma unroll
-        for (int j = 0; j <= QMAX; ++j) {
-          z[k][j] += l[j] * a[k];
-          zq += oh[j] * z[k][j];
-        }
-        if (dn) { const double t = zq * w[k]; s2 += t * t; }
-        if (up) { const double t = (a[k] - cquot * zqm) * w[k]; s3 += t * t; }
-        if (save) z[k][QMAX] = a[k];
-        w[k] = 1.0 / (rt * fabs(z[k][0]) + at_);
-      }
-#pragma unroll
-      for (int k = 0; k < CHZ; ++k) {
-#pragma unroll
-        for (int j = 0; j <= QMAX; ++j) ZN(j, c0 + k) = z[k][j];
-        EWT(c0 + k) = w[k];
-      }
-    }
-    if (save) m.saved_tq5 = m.tq5;
-    prepare_next(dsm, s2, s3, dn, up);
-    m.etamax = (m.nst <= SMALL_NST) ? ETAMX2 : ETAMX3;
   }
 
   CEP_D void set_eta()
@@ -1246,30 +1154,6 @@ struct Lane {
       for (int i = 0; i < N; ++i) ZN(QMAX, i) = a[i];
     }
     set_eta();
-  }
-
-  // y(t) from the Nordsieck array (CVodeGetDky with k = 0)
-  template <typename YV>
-  CEP_D void interpolate(double t, const YV &y) const
-  {
-    const double s = (t - m.tn) / m.h;
-    const bool q5 = m.q == QMAX;     // below order 5 row QMAX holds the saved correction
-#pragma unroll
-    for (int c0 = 0; c0 < N; c0 += CHZ) {
-      double z[CHZ][QMAX + 1];
-#pragma unroll
-      for (int k = 0; k < CHZ; ++k)
-#pragma unroll
-        for (int j = 0; j <= QMAX; ++j) z[k][j] = ZN(j, c0 + k);
-      __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-      for (int k = 0; k < CHZ; ++k) {
-        double v = q5 ? z[k][QMAX] : 0.0;
-#pragma unroll
-        for (int j = QMAX - 1; j >= 0; --j) v = z[k][j] + s * v;   // rows above q are zero
-        y[c0 + k] = v;
-      }
-    }
   }
 
   // ---- initial step size (CVODE's cvHin): bounds, then up to MAX_ITERS trial RHS

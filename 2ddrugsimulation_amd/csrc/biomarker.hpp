@@ -11,6 +11,8 @@
 // The Ca trace of the current beat lives in HBM (ctrace, SoA); everything else is a few
 // dozen scalars per cell carried in registers inside a launch and in HBM between launches.
 #pragma once
+#include <cstddef>
+#include <type_traits>
 #include "common.hpp"
 
 namespace cep {
@@ -45,7 +47,17 @@ CEP_D void bm_res_init(BmResult &r, double v, double ca)
   r.ca_peak = -999; r.ca_valley = ca; r.ca_dia = -999; r.cad90 = 0.; r.cad50 = 0.;
 }
 
-struct Biomarker {
+// The selection key of the selected beat (its dvmdt_repol), for machines that keep the
+// selected beat's record in HBM.
+struct BmKey { double dvmdt_repol; };
+
+// BEST_IN_HBM = false: the selected beat's record is a member, like the rest of the state.
+// BEST_IN_HBM = true: the record is written to its persistent rows (slots BM_D_N + BM_RES_N
+// onwards of the SoA below) when a beat is selected and by begin(); the struct keeps only the
+// selection key.  This is for machines held in LDS, where the 15 doubles per cell count.
+// `best_row` is the cell's first row of that record, d + c + (BM_D_N + BM_RES_N) * pitch.
+template <bool BEST_IN_HBM = false>
+struct BiomarkerT {
   // configuration (per cell, from parameters)
   double bcl, dt, stim_start;
   int beat_steps, win_start, print_freq, cap;
@@ -54,12 +66,25 @@ struct Biomarker {
       ca_amp90, cad50_prev, cad50_curr, cad90_prev, cad90_curr;
   int pace, steepest_pace, eligible, nca;
   int cur_slot, best_slot, best_n, nrep;   // time series (TsView): slot in use, selected slot
-  BmResult temp, best;
+  BmResult temp;
+  std::conditional_t<BEST_IN_HBM, BmKey, BmResult> best;
 
-  CEP_D void begin(double v0, double ca0)
+  // best = temp
+  CEP_D void select_temp(double *best_row, int pitch)
   {
-    bm_res_init(best, v0, ca0);
+    if constexpr (BEST_IN_HBM) {
+      best.dvmdt_repol = temp.dvmdt_repol;
+      const double *tr = &temp.qnet;
+      for (int k = 0; k < BM_RES_N; ++k) best_row[(size_t)k * pitch] = tr[k];
+    } else {
+      best = temp;
+    }
+  }
+
+  CEP_D void begin(double v0, double ca0, double *best_row = nullptr, int pitch = 0)
+  {
     bm_res_init(temp, v0, ca0);
+    select_temp(best_row, pitch);
     qnet = inal_auc = ical_auc = 0.;
     pace = 0; steepest_pace = 0; eligible = 0; nca = 0;
     cur_slot = 0; best_slot = 0; best_n = 0; nrep = 0;
@@ -73,7 +98,7 @@ struct Biomarker {
   // ires: INaL, ICaL, Ito, IKr, IKs, IK1 at this point (read only when ts.t is set)
   CEP_D void sample(int iout, double t, double v, double ca, double dvdt, double qsum, double inal,
                     double ical, double *ct, double *cv, int pitch, const TsView &ts = TsView{},
-                    const double *ires = nullptr)
+                    const double *ires = nullptr, double *best_row = nullptr)
   {
     qnet += qsum * dt;
     inal_auc += inal * dt;
@@ -116,7 +141,7 @@ struct Biomarker {
         }
         if (temp.dvmdt_repol > best.dvmdt_repol) {
           steepest_pace = pace;
-          best = temp;
+          select_temp(best_row, pitch);
           best_slot = cur_slot;
           best_n = nca;
         }
@@ -124,7 +149,7 @@ struct Biomarker {
       const int fin_slot = cur_slot, fin_n = nca;
       pace++;
       if (pace == win_start + 1) {
-        best = temp;
+        select_temp(best_row, pitch);
         steepest_pace = pace - 1;
         best_slot = fin_slot;
         best_n = fin_n;
@@ -173,9 +198,14 @@ struct Biomarker {
     t_ca_peak = p[7 * (size_t)pitch]; ca_amp50 = p[8 * (size_t)pitch]; ca_amp90 = p[9 * (size_t)pitch];
     cad50_prev = p[10 * (size_t)pitch]; cad50_curr = p[11 * (size_t)pitch];
     cad90_prev = p[12 * (size_t)pitch]; cad90_curr = p[13 * (size_t)pitch];
-    double *tr = &temp.qnet, *br = &best.qnet;
+    double *tr = &temp.qnet;
     for (int k = 0; k < BM_RES_N; ++k) tr[k] = p[(size_t)(BM_D_N + k) * pitch];
-    for (int k = 0; k < BM_RES_N; ++k) br[k] = p[(size_t)(BM_D_N + BM_RES_N + k) * pitch];
+    if constexpr (BEST_IN_HBM) {
+      best.dvmdt_repol = p[(size_t)(BM_D_N + BM_RES_N + offsetof(BmResult, dvmdt_repol) / 8) * pitch];
+    } else {
+      double *br = &best.qnet;
+      for (int k = 0; k < BM_RES_N; ++k) br[k] = p[(size_t)(BM_D_N + BM_RES_N + k) * pitch];
+    }
     const int *q = iv + c;
     pace = q[0]; steepest_pace = q[(size_t)pitch]; eligible = q[2 * (size_t)pitch]; nca = q[3 * (size_t)pitch];
     cur_slot = q[4 * (size_t)pitch]; best_slot = q[5 * (size_t)pitch]; best_n = q[6 * (size_t)pitch];
@@ -189,14 +219,19 @@ struct Biomarker {
     p[7 * (size_t)pitch] = t_ca_peak; p[8 * (size_t)pitch] = ca_amp50; p[9 * (size_t)pitch] = ca_amp90;
     p[10 * (size_t)pitch] = cad50_prev; p[11 * (size_t)pitch] = cad50_curr;
     p[12 * (size_t)pitch] = cad90_prev; p[13 * (size_t)pitch] = cad90_curr;
-    const double *tr = &temp.qnet, *br = &best.qnet;
+    const double *tr = &temp.qnet;
     for (int k = 0; k < BM_RES_N; ++k) p[(size_t)(BM_D_N + k) * pitch] = tr[k];
-    for (int k = 0; k < BM_RES_N; ++k) p[(size_t)(BM_D_N + BM_RES_N + k) * pitch] = br[k];
+    if constexpr (!BEST_IN_HBM) {   // else the rows are written when a beat is selected
+      const double *br = &best.qnet;
+      for (int k = 0; k < BM_RES_N; ++k) p[(size_t)(BM_D_N + BM_RES_N + k) * pitch] = br[k];
+    }
     int *q = iv + c;
     q[0] = pace; q[(size_t)pitch] = steepest_pace; q[2 * (size_t)pitch] = eligible; q[3 * (size_t)pitch] = nca;
     q[4 * (size_t)pitch] = cur_slot; q[5 * (size_t)pitch] = best_slot; q[6 * (size_t)pitch] = best_n;
     q[7 * (size_t)pitch] = nrep;
   }
 };
+
+using Biomarker = BiomarkerT<false>;
 
 }  // namespace cep

@@ -94,7 +94,8 @@ struct RunCfg {
 
 // win_cell: the cell's own first window beat (set when the cell was restarted on its own), or
 // -1 for the batch's window, the last nwin of nbeats_total beats
-CEP_D void bm_config(Biomarker &bm, double bcl, double stim_start, const RunCfg &rc, int cap, int win_cell)
+template <class BM>
+CEP_D void bm_config(BM &bm, double bcl, double stim_start, const RunCfg &rc, int cap, int win_cell)
 {
   bm.bcl = bcl;
   bm.dt = rc.dt;
@@ -132,7 +133,6 @@ CEP_D int lanes_below(unsigned long long mask)
 // interpolations to output points and the maps of the next attempts.
 template <typename R, int CPW>
 CEP_D void wave_sweeps(const ordbdf::Buffers &B, const ordbdf::BlockBufs &V, ordbdf::Mach *s_m, double (*s_v)[64],
-                       double (*s_map)[ordbdf::NMAP],
                        int cell0, int lane, int pp, bool complete)
 {
   using namespace ordbdf;
@@ -148,7 +148,7 @@ CEP_D void wave_sweeps(const ordbdf::Buffers &B, const ordbdf::BlockBufs &V, ord
   } else if (fl & (SWF_INTERP | SWF_ATTEMPT)) {
     Lane<R> Ls{B, s_m[s], V, (unsigned)s, (unsigned)(cell0 + s), pp};
     Ls.output_attempt_part(g, (fl & SWF_INTERP) != 0, (fl & SWF_YCOPY) != 0, ov, (fl & SWF_ATTEMPT) != 0,
-                           &s_map[s][0], yv);
+                           LVec{&s_v[0][2 * CPW + s]}, yv);
   }
 }
 
@@ -171,18 +171,27 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
   using namespace ordbdf;
   static_assert(2 * CPW <= 64, "an integrator and an output lane per cell");
   static_assert(CPW == CPB, "one block of integrator vectors per workgroup");
-  constexpr int QCAP = CPW * NJC;                 // each cell has at most one Jacobian pending
+  // LDS is kept within 40 KiB, so that four one-wave workgroups share a CU (one per SIMD)
   __shared__ Mach s_m[CPW];
-  __shared__ Biomarker s_bm[CPW];
+  __shared__ BiomarkerT<true> s_bm[CPW];          // the selected beat's record goes to HBM
   // each lane's right-hand-side argument, then its result: [component][lane].  The RHS
-  // reads the argument into registers before it writes the result.
+  // reads the argument into registers before it writes the result.  Between a pass's
+  // consume and the next pass's Jacobian columns nothing else uses the columns of lanes
+  // [2*CPW, 3*CPW): they hold the attempt maps (NMAP entries of cell slot s in column 2*CPW+s),
+  // written by the integrators in the second bookkeeping stage and read by the second sweep.
   __shared__ double s_v[N][64];
   __shared__ double s_tout[CPW];
   __shared__ int s_oiout[CPW], s_opend[CPW], s_ofail[CPW];
-  __shared__ unsigned short s_jq[QCAP];           // Jacobian tasks: (slot << 6) | column slot
-  __shared__ double s_map[CPW][NMAP];             // each cell's attempt map for the wave sweep
-  __shared__ DriverRules s_dr[CPW];               // driver rules of the output lanes (rc.dr set)
-  __shared__ double s_acc[CPW][3];                // output point's STATES[qnet, INaL_AUC, ICaL_AUC]
+  // Jacobian tasks: a cell publishes its NJC columns as one run, so task t is column t % NJC
+  // of the cell slot in ring entry (t / NJC) % CPW (at most one run per cell is pending)
+  __shared__ unsigned char s_jcell[CPW];
+  // driver rules of the output lanes, then the output point's STATES[qnet, INaL_AUC, ICaL_AUC]:
+  // dynamic LDS that only the launches with driver rules (rc.dr set) allocate
+  extern __shared__ double s_dyn[];
+  DriverRules *const s_dr = reinterpret_cast<DriverRules *>(s_dyn);
+  double (*const s_acc)[3] = reinterpret_cast<double (*)[3]>(s_dyn + CPW * (sizeof(DriverRules) / sizeof(double)));
+  static_assert(sizeof(DriverRules) % sizeof(double) == 0, "the STATES triples follow the rules");
+  static_assert(3 * CPW <= 64 && NMAP <= N, "attempt maps in the staging columns of lanes [2*CPW, 3*CPW)");
 
   constexpr bool kBeats = MODE == LAUNCH_BEATS || MODE == LAUNCH_BEATS_ORDERED;
   const int lane = threadIdx.x;
@@ -237,8 +246,10 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
       if (m.iout >= m.iout_end && ph != PH_INIT_F) { ph = PH_DONE; m.phase = ph; }
     }
   }
+  // the output lane's rows of the selected beat's record
+  double *const best_row = B.bmd + c + (size_t)(BM_D_N + BM_RES_N) * B.pitch;
   if (is_out) {
-    Biomarker &bm = s_bm[slot];
+    BiomarkerT<true> &bm = s_bm[slot];
     // a cell's own window (cells restarted one by one) in the beat launches, which are the
     // ones that run such cells; the output-point launches keep the batch's window and the
     // code they had before the per-cell window existed (their FP64 build integrated nothing
@@ -247,7 +258,7 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
               kBeats ? B.win[c] : -1);
     const int iout0 = B.si[11 * B.pitch + c];
     const bool fresh = B.si[4 * B.pitch + c] == PH_FRESH;
-    if (fresh) bm.begin(B.y[c], B.y[9 * B.pitch + c]);
+    if (fresh) bm.begin(B.y[c], B.y[9 * B.pitch + c], best_row, B.pitch);
     else bm.load(B.bmd, B.bmi, B.pitch, c);
     if (rc.dr) {
       if (fresh) s_dr[slot].begin(B.y[c]);
@@ -363,7 +374,7 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
       }
     }
     __syncthreads();
-    wave_sweeps<R, CPW>(B, V, s_m, s_v, s_map, cell0, lane, pp, true);
+    wave_sweeps<R, CPW>(B, V, s_m, s_v, cell0, lane, pp, true);
     __syncthreads();
     if (is_int) {
       Lane<R> L{B, s_m[slot], V, (unsigned)slot, (unsigned)c, pp};
@@ -428,7 +439,7 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
 #pragma unroll
           for (int r = 0; r <= QMAX; ++r)
 #pragma unroll
-            for (int col = r; col <= QMAX; ++col) s_map[slot][map_idx(r, col)] = M[r][col];
+            for (int col = r; col <= QMAX; ++col) s_v[map_idx(r, col)][2 * CPW + slot] = M[r][col];
           m.wsw |= SWF_ATTEMPT;                // applied by the wave sweep below
         }
         m.sw_ops = 0; m.sw_dq = 0; m.sw_q1 = m.q;
@@ -462,7 +473,7 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
       m.phase = ph;
     }
     __syncthreads();
-    wave_sweeps<R, CPW>(B, V, s_m, s_v, s_map, cell0, lane, pp, false);
+    wave_sweeps<R, CPW>(B, V, s_m, s_v, cell0, lane, pp, false);
     __syncthreads();
     // ================= output lanes
     if (is_out && s_opend[slot]) {                 // y(tout) is already in s_v
@@ -477,9 +488,9 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
     if (req == RQ_NONE) {
       const int r = lanes_below(freem);
       if (r < navail) {
-        const int t = s_jq[(jq_head + r) % QCAP];
-        jown = t >> 6;
-        jcol = t & 63;
+        const unsigned t = (unsigned)(jq_head + r);
+        jown = s_jcell[(t / NJC) % CPW];
+        jcol = (int)(t % NJC);
         const int co = cell0 + jown;
         Lane<R> Lo{B, s_m[jown], V, (unsigned)jown, (unsigned)co, pp};
         const int k = jstate(jcol);
@@ -578,7 +589,7 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
       const double ires[6] = { cur.INaL, cur.ICaL, cur.Ito, cur.IKr, cur.IKs, cur.IK1 };
       s_bm[slot].sample(iout, treq, v0, ca0, f[0],
                         cur.INaL + cur.ICaL + cur.Ito + cur.IKr + cur.IKs + cur.IK1, cur.INaL, cur.ICaL,
-                        B.ct + c, B.cv + c, pp, tv, ires);
+                        B.ct + c, B.cv + c, pp, tv, ires, best_row);
       if (B.vtrace && iout > trace_start && iout - trace_start <= B.tcap)
         B.vtrace[(iout - trace_start - 1) * pp + c] = v0;
       if (rc.dr) {
@@ -613,9 +624,9 @@ __global__ void __launch_bounds__(64) ord_bdf_kernel(const Buffers B, const RunC
     // integrators that need a fresh Jacobian queue their 48 columns
     const unsigned long long pubm = __ballot(publish);
     if (publish) {
-      const int base = jq_tail + NJC * lanes_below(pubm);
-      for (int k = 0; k < NJC; ++k) s_jq[(base + k) % QCAP] = (unsigned short)((slot << 6) | k);
-      s_m[slot].jend = base + NJC;
+      const int run = jq_tail / NJC + lanes_below(pubm);   // jq_tail is a multiple of NJC
+      s_jcell[run % CPW] = (unsigned char)slot;
+      s_m[slot].jend = (run + 1) * NJC;
     }
     jq_tail += NJC * __popcll(pubm);
     __syncthreads();
@@ -2180,20 +2191,22 @@ static void launch_ord_bdf(cep_batch *b, const RunCfg &rc, int nbeats, int nout_
   const int blocks = (b->n + CPW - 1) / CPW;
   const dim3 g(blocks), w(64);
   const bool f64 = b->prec == CEP_PREC_FP64;
+  // the driver rules' state is dynamic LDS, allocated only when the rules are on
+  const unsigned dyn = rc.dr ? CPW * (unsigned)(sizeof(DriverRules) + 3 * sizeof(double)) : 0u;
   B.gorder = nullptr;
   if (nbeats > 0 && bdf_group_order(b) == CEP_ORDER_LONGEST_FIRST) {
     B.gorder = b->d_gorder;
-    if (f64) hipLaunchKernelGGL((ord_bdf_kernel<double, CPW, LAUNCH_BEATS_ORDERED>), g, w, 0, b->stream, B, rc, nbeats, max_passes);
-    else hipLaunchKernelGGL((ord_bdf_kernel<float, CPW, LAUNCH_BEATS_ORDERED>), g, w, 0, b->stream, B, rc, nbeats, max_passes);
+    if (f64) hipLaunchKernelGGL((ord_bdf_kernel<double, CPW, LAUNCH_BEATS_ORDERED>), g, w, dyn, b->stream, B, rc, nbeats, max_passes);
+    else hipLaunchKernelGGL((ord_bdf_kernel<float, CPW, LAUNCH_BEATS_ORDERED>), g, w, dyn, b->stream, B, rc, nbeats, max_passes);
   } else if (nbeats > 0) {
-    if (f64) hipLaunchKernelGGL((ord_bdf_kernel<double, CPW, LAUNCH_BEATS>), g, w, 0, b->stream, B, rc, nbeats, max_passes);
-    else hipLaunchKernelGGL((ord_bdf_kernel<float, CPW, LAUNCH_BEATS>), g, w, 0, b->stream, B, rc, nbeats, max_passes);
+    if (f64) hipLaunchKernelGGL((ord_bdf_kernel<double, CPW, LAUNCH_BEATS>), g, w, dyn, b->stream, B, rc, nbeats, max_passes);
+    else hipLaunchKernelGGL((ord_bdf_kernel<float, CPW, LAUNCH_BEATS>), g, w, dyn, b->stream, B, rc, nbeats, max_passes);
   } else if (nout_rel > 0) {
-    if (f64) hipLaunchKernelGGL((ord_bdf_kernel<double, CPW, LAUNCH_OUTPUTS>), g, w, 0, b->stream, B, rc, nout_rel, max_passes);
-    else hipLaunchKernelGGL((ord_bdf_kernel<float, CPW, LAUNCH_OUTPUTS>), g, w, 0, b->stream, B, rc, nout_rel, max_passes);
+    if (f64) hipLaunchKernelGGL((ord_bdf_kernel<double, CPW, LAUNCH_OUTPUTS>), g, w, dyn, b->stream, B, rc, nout_rel, max_passes);
+    else hipLaunchKernelGGL((ord_bdf_kernel<float, CPW, LAUNCH_OUTPUTS>), g, w, dyn, b->stream, B, rc, nout_rel, max_passes);
   } else {
-    if (f64) hipLaunchKernelGGL((ord_bdf_kernel<double, CPW, LAUNCH_TO_OUTPUT>), g, w, 0, b->stream, B, rc, iout_abs, max_passes);
-    else hipLaunchKernelGGL((ord_bdf_kernel<float, CPW, LAUNCH_TO_OUTPUT>), g, w, 0, b->stream, B, rc, iout_abs, max_passes);
+    if (f64) hipLaunchKernelGGL((ord_bdf_kernel<double, CPW, LAUNCH_TO_OUTPUT>), g, w, dyn, b->stream, B, rc, iout_abs, max_passes);
+    else hipLaunchKernelGGL((ord_bdf_kernel<float, CPW, LAUNCH_TO_OUTPUT>), g, w, dyn, b->stream, B, rc, iout_abs, max_passes);
   }
 }
 
