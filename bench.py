@@ -473,16 +473,37 @@ def tt06_main(args):
     chk = {}
     if args.check_cells > 0 and prec == cep.PREC_MIXED:
         sel = np.linspace(0, n - 1, args.check_cells).astype(int)
-        b64 = batch(sel, cep.PREC_FP64)
-        b64.run_beats(W + K)
-        bm64 = b64.biomarker_array()
-        b64.close()
         bi = {k: i for i, k in enumerate(cep.BM_FIELDS)}
-        ea = np.abs(bm[sel, bi["apd90"]] - bm64[:, bi["apd90"]])
+        vm_linf = None
+        if args.check_beats > 0:
+            # a run of its own: the sample for check_beats beats in both precisions (the window
+            # covers every beat, so both select their steepest beat among all of them)
+            ends = {}
+            for pr in (cep.PREC_MIXED, cep.PREC_FP64):
+                bb = batch(sel, pr)
+                for done in range(0, args.check_beats, 25):
+                    bb.run_beats(min(25, args.check_beats - done))
+                    log(f"bench: TT06 check, {'mixed' if pr == cep.PREC_MIXED else 'fp64'}: "
+                        f"{min(done + 25, args.check_beats)} of {args.check_beats} beats")
+                ends[pr] = (bb.biomarker_array(), bb.states()[:, 0].copy(), bb.status().copy())
+                bb.close()
+            (bmc, vmc, stc), (bm64, vm64, st64) = ends[cep.PREC_MIXED], ends[cep.PREC_FP64]
+            vm_linf = float(np.abs(vmc - vm64).max())
+            bad_check = int(np.sum(stc != 0) + np.sum(st64 != 0))
+        else:
+            b64 = batch(sel, cep.PREC_FP64)
+            b64.run_beats(W + K)
+            bm64 = b64.biomarker_array()
+            b64.close()
+            bmc = bm[sel]
+            bad_check = None
+        ea = np.abs(bmc[:, bi["apd90"]] - bm64[:, bi["apd90"]])
         chk = {"apd90_max_abs_err_ms": float(ea.max()), "apd90_median_abs_err_ms": float(np.median(ea)),
                "apd90_frac_within_1ms": float(np.mean(ea <= 1.0)),
-               "cad90_max_abs_err_ms": float(np.abs(bm[sel, bi["cad90"]] - bm64[:, bi["cad90"]]).max()),
-               "cells_checked": int(len(sel))}
+               "cad90_max_abs_err_ms": float(np.abs(bmc[:, bi["cad90"]] - bm64[:, bi["cad90"]]).max()),
+               "cells_checked": int(len(sel)), "beats": int(args.check_beats or (W + K))}
+        if vm_linf is not None:
+            chk.update({"vm_linf_mv_at_end": vm_linf, "failed_cells": bad_check})
     cpu = None
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -817,6 +838,9 @@ def main():
     ap.add_argument("--precision", choices=("mixed", "fp64"), default=None,
                     help="default mixed (FP32 rates; FP64 state, BDF/LU, CG)")
     ap.add_argument("--check-cells", type=int, default=64, help="tt06: FP64 check cells")
+    ap.add_argument("--check-beats", type=int, default=0,
+                    help="tt06: run the check cells for this many beats in both precisions "
+                         "(0: compare with the timed batch at the beats it ran)")
     ap.add_argument("--check-per-bcl", type=int, default=32,
                     help="population: oracle-checked cells per BCL class (0: no accuracy check)")
     ap.add_argument("--fp64-beats", type=int, default=2,
