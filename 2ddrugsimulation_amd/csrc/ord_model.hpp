@@ -120,7 +120,7 @@ CEP_D void rhs(double t, const YV &y, FV &&f, const PA &P, Currents *cur, GA &&g
   const R v = (R)y[0];
   const double celltype = P(OP_CELLTYPE);
   const R camkt = (R)y[1], cass = (R)y[2], nai = (R)y[3], nass = (R)y[4], ki = (R)y[5];
-  const R kss = (R)y[6], cansr = (R)y[7], cajsr = (R)y[8], cai = (R)y[9];
+  const R kss = (R)y[6], cajsr = (R)y[8], cai = (R)y[9];   // cansr (y[7]) enters in FP64 below
 
   // ---- HH gate steady states and time constants (:849-887)
   const R hLss = Mx::rcp(c<R>(1) + Mx::ex((v + c<R>(87.61)) * c<R>(1 / 7.488)));
