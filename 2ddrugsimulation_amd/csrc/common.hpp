@@ -20,7 +20,9 @@ struct Soa {
 
 // Precision traits for the rate/gating arithmetic.  double: IEEE division and the
 // ocml exp/log.  float: hardware transcendental (v_exp_f32 / v_log_f32 through the
-// fast intrinsics) and reciprocal-multiply division; state stays in double.
+// fast intrinsics) and reciprocal-multiply division on the hardware reciprocal
+// (v_rcp_f32, 1 ulp): two instructions where the IEEE-rounded quotient (what __fdividef and
+// __frcp_rn compile to here) is a dependent sequence of about ten; state stays in double.
 template <typename R>
 struct M;
 
@@ -38,8 +40,8 @@ template <>
 struct M<float> {
   static CEP_D float ex(float x) { return __expf(x); }
   static CEP_D float lg(float x) { return __logf(x); }
-  static CEP_D float dv(float a, float b) { return __fdividef(a, b); }
-  static CEP_D float rcp(float a) { return __frcp_rn(a); }
+  static CEP_D float dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+  static CEP_D float rcp(float a) { return __builtin_amdgcn_rcpf(a); }
   static CEP_D float pw(float a, float b) { return __powf(a, b); }
   static CEP_D float sq(float a) { return __fsqrt_rn(a); }
 };

@@ -93,7 +93,10 @@ CEP_D double ghk<double>(double A, double U)
 template <>
 CEP_D float ghk<float>(float A, float U)
 {
-  // float: series near 0 where exp(U) - 1 cancels (|U| < 1e-3 -> error < U^4/720)
+  // float: series near 0 where exp(U) - 1 cancels (|U| < 1e-3 -> error < U^4/720); elsewhere
+  // the correctly rounded quotient, not the reciprocal-multiply of the other FP32 rates: this
+  // driving force feeds ICaL, the largest plateau current, and the 1-ulp reciprocal here moved
+  // a non-repolarising cell's Vm trace beyond the parity bar
   if (fabsf(U) < 1e-3f) return A * (1.0f - 0.5f * U + U * U * (1.0f / 12.0f));
   return A * __fdividef(U, __expf(U) - 1.0f);
 }

@@ -382,7 +382,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) ti
         // rounding of exp(-x) to FP32 near 1
         const float x = (float)(h * a[i]);
         const float ph = x < 0.03125f ? 1.f - x * (0.5f - x * ((1.f / 6.f) - x * ((1.f / 24.f) - x * (1.f / 120.f))))
-                                      : (1.f - __expf(-x)) / x;
+                                      : (1.f - __expf(-x)) * __builtin_amdgcn_rcpf(x);
         v = s[i] + f[i] * h * (double)ph;
       } else {
         v = s[i] + f[i] * (-expm1(-h * a[i]) / a[i]);
