@@ -29,6 +29,8 @@ struct M;
 template <>
 struct M<double> {
   static CEP_D double ex(double x) { return exp(x); }
+  // e^(x s) for a constant scale s, as exp(x * s)
+  static CEP_D double exs(double x, double s) { return exp(x * s); }
   static CEP_D double lg(double x) { return log(x); }
   static CEP_D double dv(double a, double b) { return a / b; }
   static CEP_D double rcp(double a) { return 1.0 / a; }
@@ -39,6 +41,9 @@ struct M<double> {
 template <>
 struct M<float> {
   static CEP_D float ex(float x) { return __expf(x); }
+  // e^(x s) for a constant scale s: 2^(x (s log2 e)), the two constants folded into one
+  // multiply ahead of v_exp_f32 (__expf(x * s) multiplies by s, then by log2 e)
+  static CEP_D float exs(float x, float s) { return __builtin_amdgcn_exp2f(x * (s * 1.44269504088896341f)); }
   static CEP_D float lg(float x) { return __logf(x); }
   static CEP_D float dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
   static CEP_D float rcp(float a) { return __builtin_amdgcn_rcpf(a); }

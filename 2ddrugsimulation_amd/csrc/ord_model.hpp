@@ -126,48 +126,48 @@ CEP_D void rhs(double t, const YV &y, FV &&f, const PA &P, Currents *cur, GA &&g
   const R kss = (R)y[6], cajsr = (R)y[8], cai = (R)y[9];   // cansr (y[7]) enters in FP64 below
 
   // ---- HH gate steady states and time constants (:849-887)
-  const R hLss = Mx::rcp(c<R>(1) + Mx::ex((v + c<R>(87.61)) * c<R>(1 / 7.488)));
-  const R hLssp = Mx::rcp(c<R>(1) + Mx::ex((v + c<R>(93.81)) * c<R>(1 / 7.488)));
-  const R mss = Mx::rcp(c<R>(1) + Mx::ex(-(v + c<R>(mssV1)) * c<R>(1 / mssV2)));
-  const R tm_inv = c<R>(mtD1) * Mx::ex((v + c<R>(mtV1)) * c<R>(1 / mtV2)) + c<R>(mtD2) * Mx::ex(-(v + c<R>(mtV3)) * c<R>(1 / mtV4));
-  const R hss = Mx::rcp(c<R>(1) + Mx::ex((v + c<R>(hssV1)) * c<R>(1 / hssV2)));
-  const R thf_inv = c<R>(1.432e-05) * Mx::ex(-(v + c<R>(1.196)) * c<R>(1 / 6.285)) + c<R>(6.149) * Mx::ex((v + c<R>(0.5096)) * c<R>(1 / 20.27));
-  const R ths_inv = c<R>(0.009794) * Mx::ex(-(v + c<R>(17.95)) * c<R>(1 / 28.05)) + c<R>(0.3343) * Mx::ex((v + c<R>(5.73)) * c<R>(1 / 56.66));
-  const R ass = Mx::rcp(c<R>(1) + Mx::ex(-(v - c<R>(14.34)) * c<R>(1 / 14.82)));
-  const R ta = Mx::dv(c<R>(1.0515), Mx::rcp(c<R>(1.2089) * (c<R>(1) + Mx::ex(-(v - c<R>(18.4099)) * c<R>(1 / 29.3814))))
-                                        + Mx::dv(c<R>(3.5), c<R>(1) + Mx::ex((v + c<R>(100)) * c<R>(1 / 29.3814))));
-  const R dss = Mx::rcp(c<R>(1) + Mx::ex(-(v + c<R>(3.94)) * c<R>(1 / 4.23)));
-  const R td = c<R>(0.6) + Mx::rcp(Mx::ex(c<R>(-0.05) * (v + c<R>(6))) + Mx::ex(c<R>(0.09) * (v + c<R>(14))));
-  const R fss = Mx::rcp(c<R>(1) + Mx::ex((v + c<R>(19.58)) * c<R>(1 / 3.696)));
-  const R tff = c<R>(7) + Mx::rcp(c<R>(0.0045) * Mx::ex(-(v + c<R>(20)) * c<R>(0.1)) + c<R>(0.0045) * Mx::ex((v + c<R>(20)) * c<R>(0.1)));
-  const R tfs = c<R>(1000) + Mx::rcp(c<R>(3.5e-05) * Mx::ex(-(v + c<R>(5)) * c<R>(0.25)) + c<R>(3.5e-05) * Mx::ex((v + c<R>(5)) * c<R>(1 / 6.0)));
+  const R hLss = Mx::rcp(c<R>(1) + Mx::exs((v + c<R>(87.61)), c<R>(1 / 7.488)));
+  const R hLssp = Mx::rcp(c<R>(1) + Mx::exs((v + c<R>(93.81)), c<R>(1 / 7.488)));
+  const R mss = Mx::rcp(c<R>(1) + Mx::exs(-(v + c<R>(mssV1)), c<R>(1 / mssV2)));
+  const R tm_inv = c<R>(mtD1) * Mx::exs((v + c<R>(mtV1)), c<R>(1 / mtV2)) + c<R>(mtD2) * Mx::exs(-(v + c<R>(mtV3)), c<R>(1 / mtV4));
+  const R hss = Mx::rcp(c<R>(1) + Mx::exs((v + c<R>(hssV1)), c<R>(1 / hssV2)));
+  const R thf_inv = c<R>(1.432e-05) * Mx::exs(-(v + c<R>(1.196)), c<R>(1 / 6.285)) + c<R>(6.149) * Mx::exs((v + c<R>(0.5096)), c<R>(1 / 20.27));
+  const R ths_inv = c<R>(0.009794) * Mx::exs(-(v + c<R>(17.95)), c<R>(1 / 28.05)) + c<R>(0.3343) * Mx::exs((v + c<R>(5.73)), c<R>(1 / 56.66));
+  const R ass = Mx::rcp(c<R>(1) + Mx::exs(-(v - c<R>(14.34)), c<R>(1 / 14.82)));
+  const R ta = Mx::dv(c<R>(1.0515), Mx::rcp(c<R>(1.2089) * (c<R>(1) + Mx::exs(-(v - c<R>(18.4099)), c<R>(1 / 29.3814))))
+                                        + Mx::dv(c<R>(3.5), c<R>(1) + Mx::exs((v + c<R>(100)), c<R>(1 / 29.3814))));
+  const R dss = Mx::rcp(c<R>(1) + Mx::exs(-(v + c<R>(3.94)), c<R>(1 / 4.23)));
+  const R td = c<R>(0.6) + Mx::rcp(Mx::exs((v + c<R>(6)), c<R>(-0.05)) + Mx::exs((v + c<R>(14)), c<R>(0.09)));
+  const R fss = Mx::rcp(c<R>(1) + Mx::exs((v + c<R>(19.58)), c<R>(1 / 3.696)));
+  const R tff = c<R>(7) + Mx::rcp(c<R>(0.0045) * Mx::exs(-(v + c<R>(20)), c<R>(0.1)) + c<R>(0.0045) * Mx::exs((v + c<R>(20)), c<R>(0.1)));
+  const R tfs = c<R>(1000) + Mx::rcp(c<R>(3.5e-05) * Mx::exs(-(v + c<R>(5)), c<R>(0.25)) + c<R>(3.5e-05) * Mx::exs((v + c<R>(5)), c<R>(1 / 6.0)));
   const R jca = (R)y[30];
   const R anca = Mx::rcp(Mx::dv(c<R>(k2n), jca) + [&] { const R q = c<R>(1) + Mx::dv(c<R>(Kmn), cass); const R q2 = q * q; return q2 * q2; }());
-  const R xs1ss = Mx::rcp(c<R>(1) + Mx::ex(-(v + c<R>(11.6)) * c<R>(1 / 8.932)));
-  const R txs1 = c<R>(txs1_max) + Mx::rcp(c<R>(0.0002326) * Mx::ex((v + c<R>(48.28)) * c<R>(1 / 17.8)) + c<R>(0.001292) * Mx::ex(-(v + c<R>(210)) * c<R>(1 / 230.0)));
+  const R xs1ss = Mx::rcp(c<R>(1) + Mx::exs(-(v + c<R>(11.6)), c<R>(1 / 8.932)));
+  const R txs1 = c<R>(txs1_max) + Mx::rcp(c<R>(0.0002326) * Mx::exs((v + c<R>(48.28)), c<R>(1 / 17.8)) + c<R>(0.001292) * Mx::exs(-(v + c<R>(210)), c<R>(1 / 230.0)));
   const R xk1ss = Mx::rcp(c<R>(1) + Mx::ex(-(v + c<R>(2.5538 * ko + 144.59)) * c<R>(1 / (1.5692 * ko + 3.8115))));
-  const R txk1_inv = (Mx::ex(-(v + c<R>(127.2)) * c<R>(1 / 20.36)) + Mx::ex((v + c<R>(236.8)) * c<R>(1 / 69.33))) * c<R>(1 / 122.2);
+  const R txk1_inv = (Mx::exs(-(v + c<R>(127.2)), c<R>(1 / 20.36)) + Mx::exs((v + c<R>(236.8)), c<R>(1 / 69.33))) * c<R>(1 / 122.2);
   const R CaMKb = Mx::dv(c<R>(CaMKo) * (c<R>(1) - camkt), c<R>(1) + Mx::dv(c<R>(KmCaM), cass));
-  const R tj = c<R>(2.038) + Mx::rcp(c<R>(0.02136) * Mx::ex(-(v + c<R>(100.6)) * c<R>(1 / 8.281)) + c<R>(0.3052) * Mx::ex((v + c<R>(0.9941)) * c<R>(1 / 38.45)));
-  const R assp = Mx::rcp(c<R>(1) + Mx::ex(-(v - c<R>(24.34)) * c<R>(1 / 14.82)));
-  const R tfcaf = c<R>(7) + Mx::rcp(c<R>(0.04) * Mx::ex(-(v - c<R>(4)) * c<R>(1 / 7.0)) + c<R>(0.04) * Mx::ex((v - c<R>(4)) * c<R>(1 / 7.0)));
-  const R tfcas = c<R>(100) + Mx::rcp(c<R>(0.00012) * Mx::ex(-v * c<R>(1 / 3.0)) + c<R>(0.00012) * Mx::ex(v * c<R>(1 / 7.0)));
-  const R txs2_inv = c<R>(0.01) * Mx::ex((v - c<R>(50)) * c<R>(0.05)) + c<R>(0.0193) * Mx::ex(-(v + c<R>(66.54)) * c<R>(1 / 31.0));
-  const R hssp = Mx::rcp(c<R>(1) + Mx::ex((v + c<R>(89.1)) * c<R>(1 / 6.086)));
-  const R mLss = Mx::rcp(c<R>(1) + Mx::ex(-(v + c<R>(42.85)) * c<R>(1 / 5.264)));
-  const R iss = Mx::rcp(c<R>(1) + Mx::ex((v + c<R>(43.94)) * c<R>(1 / 5.711)));
-  const R delta_epi = (celltype == 1.0) ? c<R>(1) - Mx::dv(c<R>(0.95), c<R>(1) + Mx::ex((v + c<R>(70)) * c<R>(0.2))) : c<R>(1);
-  const R tiF = (c<R>(4.562) + Mx::rcp(c<R>(0.3933) * Mx::ex(-(v + c<R>(100)) * c<R>(0.01)) + c<R>(0.08004) * Mx::ex((v + c<R>(50)) * c<R>(1 / 16.59)))) * delta_epi;
-  const R tiS = (c<R>(23.62) + Mx::rcp(c<R>(0.001416) * Mx::ex(-(v + c<R>(96.52)) * c<R>(1 / 59.05)) + c<R>(1.78e-08) * Mx::ex((v + c<R>(114.1)) * c<R>(1 / 8.079)))) * delta_epi;
-  const R dti_develop = c<R>(1.354) + Mx::dv(c<R>(0.0001), Mx::ex((v - c<R>(167.4)) * c<R>(1 / 15.89)) + Mx::ex(-(v - c<R>(12.23)) * c<R>(1 / 0.2154)));
-  const R dti_recover = c<R>(1) - Mx::dv(c<R>(0.5), c<R>(1) + Mx::ex((v + c<R>(70)) * c<R>(0.05)));
+  const R tj = c<R>(2.038) + Mx::rcp(c<R>(0.02136) * Mx::exs(-(v + c<R>(100.6)), c<R>(1 / 8.281)) + c<R>(0.3052) * Mx::exs((v + c<R>(0.9941)), c<R>(1 / 38.45)));
+  const R assp = Mx::rcp(c<R>(1) + Mx::exs(-(v - c<R>(24.34)), c<R>(1 / 14.82)));
+  const R tfcaf = c<R>(7) + Mx::rcp(c<R>(0.04) * Mx::exs(-(v - c<R>(4)), c<R>(1 / 7.0)) + c<R>(0.04) * Mx::exs((v - c<R>(4)), c<R>(1 / 7.0)));
+  const R tfcas = c<R>(100) + Mx::rcp(c<R>(0.00012) * Mx::exs(-v, c<R>(1 / 3.0)) + c<R>(0.00012) * Mx::exs(v, c<R>(1 / 7.0)));
+  const R txs2_inv = c<R>(0.01) * Mx::exs((v - c<R>(50)), c<R>(0.05)) + c<R>(0.0193) * Mx::exs(-(v + c<R>(66.54)), c<R>(1 / 31.0));
+  const R hssp = Mx::rcp(c<R>(1) + Mx::exs((v + c<R>(89.1)), c<R>(1 / 6.086)));
+  const R mLss = Mx::rcp(c<R>(1) + Mx::exs(-(v + c<R>(42.85)), c<R>(1 / 5.264)));
+  const R iss = Mx::rcp(c<R>(1) + Mx::exs((v + c<R>(43.94)), c<R>(1 / 5.711)));
+  const R delta_epi = (celltype == 1.0) ? c<R>(1) - Mx::dv(c<R>(0.95), c<R>(1) + Mx::exs((v + c<R>(70)), c<R>(0.2))) : c<R>(1);
+  const R tiF = (c<R>(4.562) + Mx::rcp(c<R>(0.3933) * Mx::exs(-(v + c<R>(100)), c<R>(0.01)) + c<R>(0.08004) * Mx::exs((v + c<R>(50)), c<R>(1 / 16.59)))) * delta_epi;
+  const R tiS = (c<R>(23.62) + Mx::rcp(c<R>(0.001416) * Mx::exs(-(v + c<R>(96.52)), c<R>(1 / 59.05)) + c<R>(1.78e-08) * Mx::exs((v + c<R>(114.1)), c<R>(1 / 8.079)))) * delta_epi;
+  const R dti_develop = c<R>(1.354) + Mx::dv(c<R>(0.0001), Mx::exs((v - c<R>(167.4)), c<R>(1 / 15.89)) + Mx::exs(-(v - c<R>(12.23)), c<R>(1 / 0.2154)));
+  const R dti_recover = c<R>(1) - Mx::dv(c<R>(0.5), c<R>(1) + Mx::exs((v + c<R>(70)), c<R>(0.05)));
   const R dti = dti_develop * dti_recover;
 
   // ---- ICaL, ICaNa, ICaK (:888-896, :962-965, :1003-1006)
   const R ff = (R)y[26], fs = (R)y[27], fcaf = (R)y[28], fcas = (R)y[29], ffp = (R)y[31], fcafp = (R)y[32];
   const R nca = (R)y[33], d = (R)y[25];
   const R f_ = c<R>(Aff) * ff + c<R>(Afs) * fs;
-  const R Afcaf = c<R>(0.3) + Mx::dv(c<R>(0.6), c<R>(1) + Mx::ex((v - c<R>(10)) * c<R>(0.1)));
+  const R Afcaf = c<R>(0.3) + Mx::dv(c<R>(0.6), c<R>(1) + Mx::exs((v - c<R>(10)), c<R>(0.1)));
   const R Afcas = c<R>(1) - Afcaf;
   const R fca = Afcaf * fcaf + Afcas * fcas;
   const R fp = c<R>(Aff) * ffp + c<R>(Afs) * fs;
@@ -200,7 +200,7 @@ CEP_D void rhs(double t, const YV &y, FV &&f, const PA &P, Currents *cur, GA &&g
 
   // ---- K currents (:905-918)
   const R EK = c<R>(RTF) * Mx::lg(Mx::dv(c<R>(ko), ki));
-  const R AiF = Mx::rcp(c<R>(1) + Mx::ex((v - c<R>(213.6)) * c<R>(1 / 151.2)));
+  const R AiF = Mx::rcp(c<R>(1) + Mx::exs((v - c<R>(213.6)), c<R>(1 / 151.2)));
   const R AiS = c<R>(1) - AiF;
   const R a = (R)y[19], iF = (R)y[20], iS = (R)y[21], ap = (R)y[22], iFp = (R)y[23], iSp = (R)y[24];
   const R ii = AiF * iF + AiS * iS;
@@ -212,12 +212,12 @@ CEP_D void rhs(double t, const YV &y, FV &&f, const PA &P, Currents *cur, GA &&g
   const R KsCa = c<R>(1) + Mx::dv(c<R>(0.6), c<R>(1) + Mx::pw(Mx::dv(c<R>(3.8e-05), cai), c<R>(1.4)));
   const R xs1 = (R)y[44], xs2 = (R)y[45], xk1 = (R)y[46];
   const R IKs = (R)P(OP_GKS) * KsCa * xs1 * xs2 * (v - EKs);
-  const R rk1 = Mx::rcp(c<R>(1) + Mx::ex((v + c<R>(105.8 - 2.6 * ko)) * c<R>(1 / 9.493)));
+  const R rk1 = Mx::rcp(c<R>(1) + Mx::exs((v + c<R>(105.8 - 2.6 * ko)), c<R>(1 / 9.493)));
   const R IK1 = (R)P(OP_GK1) * c<R>(2.3237900077244502) /* sqrt(ko) */ * rk1 * xk1 * (v - EK);
 
   // ---- INaK (:919-941)
   const R Knao = c<R>(Knao0) * Mx::ex(c<R>((1.0 - delta) / 3.0 * frt) * v);
-  const R Knai = c<R>(Knai0) * Mx::ex(c<R>(delta / 3.0 * frt) * v);
+  const R Knai = c<R>(Knai0) * Mx::exs(v, c<R>(delta / 3.0 * frt));
   const R ro = c<R>(1) + Mx::dv(c<R>(nao), Knao), ro3 = ro * ro * ro;
   const R ri = c<R>(1) + Mx::dv(nai, Knai), ri3 = ri * ri * ri;
   const R kki = c<R>(1) + ki * c<R>(1 / Kki), kki2 = kki * kki;
@@ -243,7 +243,7 @@ CEP_D void rhs(double t, const YV &y, FV &&f, const PA &P, Currents *cur, GA &&g
   const R JnakK = c<R>(2) * (x4 * xs_inv * B1 - x3 * xs_inv * a1);
   const R INaK = (R)P(OP_PNAK) * (JnakNa + JnakK);
 
-  const R IKb = (R)P(OP_GKB) * Mx::rcp(c<R>(1) + Mx::ex(-(v - c<R>(14.48)) * c<R>(1 / 18.34))) * (v - EK);
+  const R IKb = (R)P(OP_GKB) * Mx::rcp(c<R>(1) + Mx::exs(-(v - c<R>(14.48)), c<R>(1 / 18.34))) * (v - EK);
   const double Istim = stimulus(t, P(OP_STIM_START), P(OP_STIM_END), P(OP_STIM_AMP), P(OP_BCL), P(OP_STIM_DUR));
 
   // ---- Na currents (:966-973)
@@ -256,8 +256,8 @@ CEP_D void rhs(double t, const YV &y, FV &&f, const PA &P, Currents *cur, GA &&g
   const R INaL = (R)P(OP_GNAL) * (v - ENa) * mL * ((c<R>(1) - fCaMKp) * hL + fCaMKp * hLp);
 
   // ---- INaCa_i and INaCa_ss (:974-1034): shared Na/Ca-independent terms
-  const R hna = Mx::ex(c<R>(qna * frt) * v);
-  const R hca = Mx::ex(c<R>(qca * frt) * v);
+  const R hna = Mx::exs(v, c<R>(qna * frt));
+  const R hca = Mx::exs(v, c<R>(qca * frt));
   const R h7 = c<R>(1) + c<R>(nao / kna3) * (c<R>(1) + Mx::rcp(hna));
   const R h8 = Mx::dv(c<R>(nao / kna3), hna * h7);
   const R k3pp = h8 * c<R>(wnaca);
@@ -308,7 +308,7 @@ CEP_D void rhs(double t, const YV &y, FV &&f, const PA &P, Currents *cur, GA &&g
   const R IC1 = (R)y[34], IC2 = (R)y[35], C1 = (R)y[36], C2 = (R)y[37], IO = (R)y[39];
   const R IObound = (R)y[40], Obound = (R)y[41], Cbound = (R)y[42];
   const R Ku = (R)P(OP_KU), Kt = (R)P(OP_KT), bind = (R)P(OP_BIND);
-  const R trap = Mx::dv(Kt, c<R>(1) + Mx::ex(-(v - (R)P(OP_VHALF)) * c<R>(1 / 6.789)));
+  const R trap = Mx::dv(Kt, c<R>(1) + Mx::exs(-(v - (R)P(OP_VHALF)), c<R>(1 / 6.789)));
   const R unbindIO = Mx::dv(Ku * k53, k63);
   const R t_IC1_IC2 = k11 * IC1 - k21 * IC2, t_C1_IC1 = k51 * C1 - k61 * IC1;
   const R t_IC2_IO = k3m_ * IC2 - k4m_ * IO, t_C2_IC2 = k52 * C2 - k62 * IC2;

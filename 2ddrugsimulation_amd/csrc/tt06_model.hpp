@@ -51,60 +51,60 @@ CEP_D void gates(const double *__restrict__ y, const PA &P, R *inf, R *tau)
   const R Cass = (R)y[10];
   const bool endo = P(TP_CELLTYPE) == 0.0;
   const R d27 = V + c<R>(27);
-  inf[0] = Mx::rcp(c<R>(1) + Mx::ex((V + c<R>(20)) * c<R>(1 / 7.0)));
-  tau[0] = c<R>(1102.5) * Mx::ex(-d27 * d27 * c<R>(1 / 225.0)) + Mx::dv(c<R>(200), c<R>(1) + Mx::ex((c<R>(13) - V) * c<R>(0.1)))
-         + Mx::dv(c<R>(180), c<R>(1) + Mx::ex((V + c<R>(30)) * c<R>(0.1))) + c<R>(20);
-  inf[1] = Mx::dv(c<R>(0.67), c<R>(1) + Mx::ex((V + c<R>(35)) * c<R>(1 / 7.0))) + c<R>(0.33);
-  tau[1] = c<R>(562) * Mx::ex(-d27 * d27 * c<R>(1 / 240.0)) + Mx::dv(c<R>(31), c<R>(1) + Mx::ex((c<R>(25) - V) * c<R>(0.1)))
-         + Mx::dv(c<R>(80), c<R>(1) + Mx::ex((V + c<R>(30)) * c<R>(0.1)));
+  inf[0] = Mx::rcp(c<R>(1) + Mx::exs((V + c<R>(20)), c<R>(1 / 7.0)));
+  tau[0] = c<R>(1102.5) * Mx::exs(-d27 * d27, c<R>(1 / 225.0)) + Mx::dv(c<R>(200), c<R>(1) + Mx::exs((c<R>(13) - V), c<R>(0.1)))
+         + Mx::dv(c<R>(180), c<R>(1) + Mx::exs((V + c<R>(30)), c<R>(0.1))) + c<R>(20);
+  inf[1] = Mx::dv(c<R>(0.67), c<R>(1) + Mx::exs((V + c<R>(35)), c<R>(1 / 7.0))) + c<R>(0.33);
+  tau[1] = c<R>(562) * Mx::exs(-d27 * d27, c<R>(1 / 240.0)) + Mx::dv(c<R>(31), c<R>(1) + Mx::exs((c<R>(25) - V), c<R>(0.1)))
+         + Mx::dv(c<R>(80), c<R>(1) + Mx::exs((V + c<R>(30)), c<R>(0.1)));
   const R cq = Cass * c<R>(1 / 0.05), cq2 = c<R>(1) + cq * cq;
   inf[2] = Mx::dv(c<R>(0.6), cq2) + c<R>(0.4);
   tau[2] = Mx::dv(c<R>(80), cq2) + c<R>(2);
   if (endo) {
-    inf[3] = Mx::rcp(c<R>(1) + Mx::ex((V + c<R>(28)) * c<R>(0.2)));
+    inf[3] = Mx::rcp(c<R>(1) + Mx::exs((V + c<R>(28)), c<R>(0.2)));
     const R d67 = V + c<R>(67);
-    tau[3] = c<R>(1000) * Mx::ex(-d67 * d67 * c<R>(1 / 1000.0)) + c<R>(8);
+    tau[3] = c<R>(1000) * Mx::exs(-d67 * d67, c<R>(1 / 1000.0)) + c<R>(8);
   } else {
-    inf[3] = Mx::rcp(c<R>(1) + Mx::ex((V + c<R>(20)) * c<R>(0.2)));
+    inf[3] = Mx::rcp(c<R>(1) + Mx::exs((V + c<R>(20)), c<R>(0.2)));
     const R d45 = V + c<R>(45);
-    tau[3] = c<R>(85) * Mx::ex(-d45 * d45 * c<R>(1 / 320.0)) + Mx::dv(c<R>(5), c<R>(1) + Mx::ex((V - c<R>(20)) * c<R>(0.2))) + c<R>(3);
+    tau[3] = c<R>(85) * Mx::exs(-d45 * d45, c<R>(1 / 320.0)) + Mx::dv(c<R>(5), c<R>(1) + Mx::exs((V - c<R>(20)), c<R>(0.2))) + c<R>(3);
   }
-  inf[4] = Mx::rcp(c<R>(1) + Mx::ex((c<R>(20) - V) * c<R>(1 / 6.0)));
+  inf[4] = Mx::rcp(c<R>(1) + Mx::exs((c<R>(20) - V), c<R>(1 / 6.0)));
   const R d40 = V + c<R>(40);
-  tau[4] = c<R>(9.5) * Mx::ex(-d40 * d40 * c<R>(1 / 1800.0)) + c<R>(0.8);
-  inf[5] = Mx::rcp(c<R>(1) + Mx::ex((c<R>(-26) - V) * c<R>(1 / 7.0)));
-  tau[5] = Mx::dv(c<R>(450), c<R>(1) + Mx::ex((c<R>(-45) - V) * c<R>(0.1))) * Mx::dv(c<R>(6), c<R>(1) + Mx::ex((V + c<R>(30)) * c<R>(1 / 11.5)));
-  inf[6] = Mx::rcp(c<R>(1) + Mx::ex((V + c<R>(88)) * c<R>(1 / 24.0)));
-  tau[6] = Mx::dv(c<R>(3), c<R>(1) + Mx::ex((c<R>(-60) - V) * c<R>(0.05))) * Mx::dv(c<R>(1.12), c<R>(1) + Mx::ex((V - c<R>(60)) * c<R>(0.05)));
-  inf[7] = Mx::rcp(c<R>(1) + Mx::ex((c<R>(-5) - V) * c<R>(1 / 14.0)));
-  tau[7] = Mx::dv(c<R>(1400), Mx::sq(c<R>(1) + Mx::ex((c<R>(5) - V) * c<R>(1 / 6.0))))
-         * Mx::rcp(c<R>(1) + Mx::ex((V - c<R>(35)) * c<R>(1 / 15.0))) + c<R>(80);
-  const R em = c<R>(1) + Mx::ex((c<R>(-56.86) - V) * c<R>(1 / 9.03));
+  tau[4] = c<R>(9.5) * Mx::exs(-d40 * d40, c<R>(1 / 1800.0)) + c<R>(0.8);
+  inf[5] = Mx::rcp(c<R>(1) + Mx::exs((c<R>(-26) - V), c<R>(1 / 7.0)));
+  tau[5] = Mx::dv(c<R>(450), c<R>(1) + Mx::exs((c<R>(-45) - V), c<R>(0.1))) * Mx::dv(c<R>(6), c<R>(1) + Mx::exs((V + c<R>(30)), c<R>(1 / 11.5)));
+  inf[6] = Mx::rcp(c<R>(1) + Mx::exs((V + c<R>(88)), c<R>(1 / 24.0)));
+  tau[6] = Mx::dv(c<R>(3), c<R>(1) + Mx::exs((c<R>(-60) - V), c<R>(0.05))) * Mx::dv(c<R>(1.12), c<R>(1) + Mx::exs((V - c<R>(60)), c<R>(0.05)));
+  inf[7] = Mx::rcp(c<R>(1) + Mx::exs((c<R>(-5) - V), c<R>(1 / 14.0)));
+  tau[7] = Mx::dv(c<R>(1400), Mx::sq(c<R>(1) + Mx::exs((c<R>(5) - V), c<R>(1 / 6.0))))
+         * Mx::rcp(c<R>(1) + Mx::exs((V - c<R>(35)), c<R>(1 / 15.0))) + c<R>(80);
+  const R em = c<R>(1) + Mx::exs((c<R>(-56.86) - V), c<R>(1 / 9.03));
   inf[8] = Mx::rcp(em * em);
-  tau[8] = Mx::rcp(c<R>(1) + Mx::ex((c<R>(-60) - V) * c<R>(0.2)))
-         * (Mx::dv(c<R>(0.1), c<R>(1) + Mx::ex((V + c<R>(35)) * c<R>(0.2))) + Mx::dv(c<R>(0.1), c<R>(1) + Mx::ex((V - c<R>(50)) * c<R>(1 / 200.0))));
-  const R eh = c<R>(1) + Mx::ex((V + c<R>(71.55)) * c<R>(1 / 7.43));
+  tau[8] = Mx::rcp(c<R>(1) + Mx::exs((c<R>(-60) - V), c<R>(0.2)))
+         * (Mx::dv(c<R>(0.1), c<R>(1) + Mx::exs((V + c<R>(35)), c<R>(0.2))) + Mx::dv(c<R>(0.1), c<R>(1) + Mx::exs((V - c<R>(50)), c<R>(1 / 200.0))));
+  const R eh = c<R>(1) + Mx::exs((V + c<R>(71.55)), c<R>(1 / 7.43));
   inf[9] = Mx::rcp(eh * eh);
   inf[10] = inf[9];
   R ah, bh, aj, bj;
   if (V < c<R>(-40)) {
-    ah = c<R>(0.057) * Mx::ex(-(V + c<R>(80)) * c<R>(1 / 6.8));
-    bh = c<R>(2.7) * Mx::ex(c<R>(0.079) * V) + c<R>(310000.) * Mx::ex(c<R>(0.3485) * V);
-    aj = Mx::dv((c<R>(-25428.0) * Mx::ex(c<R>(0.2444) * V) - c<R>(6.948e-06) * Mx::ex(c<R>(-0.04391) * V)) * (V + c<R>(37.78)),
-                c<R>(1) + Mx::ex(c<R>(0.311) * (V + c<R>(79.23))));
-    bj = Mx::dv(c<R>(0.02424) * Mx::ex(c<R>(-0.01052) * V), c<R>(1) + Mx::ex(c<R>(-0.1378) * (V + c<R>(40.14))));
+    ah = c<R>(0.057) * Mx::exs(-(V + c<R>(80)), c<R>(1 / 6.8));
+    bh = c<R>(2.7) * Mx::exs(V, c<R>(0.079)) + c<R>(310000.) * Mx::exs(V, c<R>(0.3485));
+    aj = Mx::dv((c<R>(-25428.0) * Mx::exs(V, c<R>(0.2444)) - c<R>(6.948e-06) * Mx::exs(V, c<R>(-0.04391))) * (V + c<R>(37.78)),
+                c<R>(1) + Mx::exs((V + c<R>(79.23)), c<R>(0.311)));
+    bj = Mx::dv(c<R>(0.02424) * Mx::exs(V, c<R>(-0.01052)), c<R>(1) + Mx::exs((V + c<R>(40.14)), c<R>(-0.1378)));
   } else {
     ah = c<R>(0);
-    bh = Mx::dv(c<R>(0.77), c<R>(0.13) * (c<R>(1) + Mx::ex((V + c<R>(10.66)) * c<R>(-1 / 11.1))));
+    bh = Mx::dv(c<R>(0.77), c<R>(0.13) * (c<R>(1) + Mx::exs((V + c<R>(10.66)), c<R>(-1 / 11.1))));
     aj = c<R>(0);
-    bj = Mx::dv(c<R>(0.6) * Mx::ex(c<R>(0.057) * V), c<R>(1) + Mx::ex(c<R>(-0.1) * (V + c<R>(32))));
+    bj = Mx::dv(c<R>(0.6) * Mx::exs(V, c<R>(0.057)), c<R>(1) + Mx::exs((V + c<R>(32)), c<R>(-0.1)));
   }
   tau[9] = Mx::rcp(ah + bh);
   tau[10] = Mx::rcp(aj + bj);
-  inf[11] = Mx::rcp(c<R>(1) + Mx::ex((c<R>(-8) - V) * c<R>(1 / 7.5)));
-  tau[11] = (Mx::dv(c<R>(1.4), c<R>(1) + Mx::ex((c<R>(-35) - V) * c<R>(1 / 13.0))) + c<R>(0.25))
-          * Mx::dv(c<R>(1.4), c<R>(1) + Mx::ex((V + c<R>(5)) * c<R>(0.2)))
-          + Mx::rcp(c<R>(1) + Mx::ex((c<R>(50) - V) * c<R>(0.05)));
+  inf[11] = Mx::rcp(c<R>(1) + Mx::exs((c<R>(-8) - V), c<R>(1 / 7.5)));
+  tau[11] = (Mx::dv(c<R>(1.4), c<R>(1) + Mx::exs((c<R>(-35) - V), c<R>(1 / 13.0))) + c<R>(0.25))
+          * Mx::dv(c<R>(1.4), c<R>(1) + Mx::exs((V + c<R>(5)), c<R>(0.2)))
+          + Mx::rcp(c<R>(1) + Mx::exs((c<R>(50) - V), c<R>(0.05)));
 }
 
 // Currents and the non-gate rates.  f receives d/dt for V, K_i, Na_i, Ca_i, Ca_ss, Ca_SR,
@@ -116,19 +116,19 @@ CEP_D void ionic(double t, const double *__restrict__ y, const PA &P, double *__
   const R V = (R)y[0];
   const R Ki = (R)y[1], Nai = (R)y[2], Cai = (R)y[3], Cass = (R)y[10], CaSR = (R)y[17], Rp = (R)y[18];
   const R INaK = Mx::dv(Mx::dv((R)P(TP_PNAK) * c<R>(Ko / (Ko + K_mk)) * Nai, Nai + c<R>(K_mNa)),
-                        c<R>(1) + c<R>(0.1245) * Mx::ex(c<R>(-0.1 * FRT) * V) + c<R>(0.0353) * Mx::ex(c<R>(-FRT) * V));
+                        c<R>(1) + c<R>(0.1245) * Mx::exs(V, c<R>(-0.1 * FRT)) + c<R>(0.0353) * Mx::exs(V, c<R>(-FRT)));
   const R ENa = c<R>(RTF) * Mx::lg(Mx::dv(c<R>(Nao), Nai));
   const R m = (R)y[7];
   const R INa = (R)P(TP_GNA) * m * m * m * (R)y[8] * (R)y[9] * (V - ENa);
   const R IbNa = (R)P(TP_GBNA) * (V - ENa);
-  const R egv = Mx::ex(c<R>(gmma * FRT) * V), eg1 = Mx::ex(c<R>((gmma - 1.0) * FRT) * V);
+  const R egv = Mx::exs(V, c<R>(gmma * FRT)), eg1 = Mx::ex(c<R>((gmma - 1.0) * FRT) * V);
   const R INaCa = Mx::dv(c<R>(K_NaCa) * (egv * Nai * Nai * Nai * c<R>(Cao) - eg1 * c<R>(Nao * Nao * Nao) * Cai * c<R>(alpha)),
                          c<R>((Km_Nai * Km_Nai * Km_Nai + Nao * Nao * Nao) * (Km_Ca + Cao)) * (c<R>(1) + c<R>(K_sat) * eg1));
   const R EK = c<R>(RTF) * Mx::lg(Mx::dv(c<R>(Ko), Ki));
   const R dvk = V - EK;
-  const R aK1 = Mx::dv(c<R>(0.1), c<R>(1) + Mx::ex(c<R>(0.06) * (dvk - c<R>(200))));
-  const R bK1 = Mx::dv(c<R>(3) * Mx::ex(c<R>(0.0002) * (dvk + c<R>(100))) + Mx::ex(c<R>(0.1) * (dvk - c<R>(10))),
-                       c<R>(1) + Mx::ex(c<R>(-0.5) * dvk));
+  const R aK1 = Mx::dv(c<R>(0.1), c<R>(1) + Mx::exs((dvk - c<R>(200)), c<R>(0.06)));
+  const R bK1 = Mx::dv(c<R>(3) * Mx::exs((dvk + c<R>(100)), c<R>(0.0002)) + Mx::exs((dvk - c<R>(10)), c<R>(0.1)),
+                       c<R>(1) + Mx::exs(dvk, c<R>(-0.5)));
   const R IK1 = (R)P(TP_GK1) * Mx::dv(aK1, aK1 + bK1) * c<R>(sqrt_ko54) * dvk;
   const R Ito = (R)P(TP_GTO) * (R)y[16] * (R)y[15] * dvk;
   const R IKr = (R)P(TP_GKR) * c<R>(sqrt_ko54) * (R)y[4] * (R)y[5] * dvk;
@@ -155,7 +155,7 @@ CEP_D void ionic(double t, const double *__restrict__ y, const PA &P, double *__
   }
   const R ECa = c<R>(0.5 * RTF) * Mx::lg(Mx::dv(c<R>(Cao), Cai));
   const R IbCa = (R)P(TP_GBCA) * (V - ECa);
-  const R IpK = Mx::dv((R)P(TP_GPK) * dvk, c<R>(1) + Mx::ex((c<R>(25) - V) * c<R>(1 / 5.98)));
+  const R IpK = Mx::dv((R)P(TP_GPK) * dvk, c<R>(1) + Mx::exs((c<R>(25) - V), c<R>(1 / 5.98)));
   const R IpCa = Mx::dv((R)P(TP_GPCA) * Cai, Cai + c<R>(K_pCa));
   const double bcl = P(TP_BCL);
   const double tm = t - floor(t / bcl) * bcl;

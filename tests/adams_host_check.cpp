@@ -29,6 +29,7 @@
 // float intrinsics the device headers name in their FP32 paths, which this FP64 build never
 // calls; the host HIP headers do not declare them
 static inline float __builtin_amdgcn_rcpf(float a) { return 1.0f / a; }
+static inline float __builtin_amdgcn_exp2f(float x) { return exp2f(x); }
 static inline float __fdividef(float a, float b) { return a / b; }
 static inline float __fsqrt_rn(float a) { return sqrtf(a); }
 
