@@ -326,7 +326,10 @@ struct Lane {
       if (3 <= ifail) knew = 1;
       h = temp2 * h;
       k = knew;
-      if (fabs(h) < fouru * fabs(x)) {
+      // a NaN error estimate fails every attempt: with h NaN, or halved to 0 at x = 0, the
+      // published test alone never ends the loop, so those two cases crash here as well (finite
+      // non-zero h: the published test unchanged)
+      if (fabs(h) < fouru * fabs(x) || !(fabs(h) > 0.0)) {
         h = fabs(fouru * fabs(x)) * sgn(h);
         eps = eps + eps;
         return true;

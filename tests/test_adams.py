@@ -193,6 +193,22 @@ def test_adams_host_build_of_gpu_path_matches_reference(adams_host, tmp_path, ke
     assert abs(tr[0, 1] - ns) <= 0.05 * ns
 
 
+def test_adams_host_build_nan_state_ends_as_a_crash(adams_host, tmp_path):
+    """A state with a NaN component makes every error estimate NaN, so every attempt of the
+    step fails: at x = 0 the step size halves to 0 and the published step-size test alone never
+    ends the loop.  The integrator reports that cell as a crash (iflag 3) after a bounded number
+    of attempts, and the finite cell beside it is not affected."""
+    s0 = np.loadtxt(os.path.join(HERE, "golden", "ord2011_output_orudy.dat"))
+    bad = s0.copy()
+    bad[5] = np.nan
+    ic50 = np.zeros(14)
+    cells = [(0.0, 1000.0, ic50, s0), (0.0, 1000.0, ic50, bad)]
+    body, tr = _host_run(adams_host, tmp_path, 1e-6, 1e-4, 5.0, 0.1, 10, cells)
+    assert tr[0, 0] == 2 and tr[1, 0] == 3
+    assert np.isfinite(body[:, 0]).all()
+    assert tr[1, 1] == 0                       # no step accepted
+
+
 def test_adams_host_build_random_cells_in_bounds(adams_host, tmp_path):
     """Random drugs, BCLs and perturbed steady states, at the hackathon's, tight and loose
     tolerances: every index stays in its array (the checks abort otherwise) and every cell
