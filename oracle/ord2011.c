@@ -17,10 +17,13 @@
  *     constants were formed (:460-467);
  *   - the Rush-Larsen update of fcafp uses exp(-d / tfcafp) with d the STATES index of the
  *     d gate (25), not dt (:1064).
- * The reference ships no compiled form of this file that builds here (its header pulls in
- * curl), so the restatement is pinned by the checks in tests/test_ord2011.py: closed forms,
- * the Rush-Larsen/rates consistency, the step rule, and the currents this model shares with
- * the ORd-CiPA oracle (itself pinned by the compiled ORd-CiPA reference).
+ * This copy of the model does not build here (its header pulls in curl).  The restatement is
+ * pinned by the checks in tests/test_ord2011.py: golden vectors of the reference's patch-clamp
+ * copy of the same model (CEPPatch_CiPA_Sim/patches/Ohara_Rudy_2011.cpp, compiled by
+ * oracle/Makefile) for the constants, the Hill block and every rate and current that copy
+ * completes; closed forms, the Rush-Larsen/rates consistency, the step rule, the currents this
+ * model shares with the ORd-CiPA oracle (itself pinned by the compiled ORd-CiPA reference), and
+ * the reference's logged EAD answers.
  */
 #ifndef _POSIX_C_SOURCE
 #define _POSIX_C_SOURCE 200809L

@@ -11,15 +11,25 @@
 //   ord_rk4   in: [celltype, conc, herg[19], t_end, h, every]      out: S[52] per sample
 //   tt_rhs    in: [celltype, t, S[19]] per record                  out: R[19] A[70]
 //   tt_rl     in: [celltype, t_end, dt, every]                      out: S[19] per sample
+//   o11_init  in: [celltype, conc, hill[14], dutta] per record      out: C[142] S[44]
+//   o11_rhs   in: [t, C[142], S[41]] per record                     out: R[41] A[198]
+// The ORd 2011 modes use the reference's patch-clamp copy of the model
+// (CEPPatch_CiPA_Sim/patches/Ohara_Rudy_2011.cpp), whose state and constant layout is that of
+// DrugSimulation's (states 0-40; it adds three charge accumulators at 41-43; constant 140 is
+// the stimulus period).  Its initConsts leaves PKNa and KmCaAct unset and handles the cell
+// type and Dutta's scaling differently from DrugSimulation's copy, so o11_rhs takes the
+// constants as input and evaluates computeRates alone.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
 #include <string>
-// The reference's enum headers clash when both models share a translation unit, so
-// this file is compiled twice: -DREF_ORD (ORd-CiPA modes) and -DREF_TT (TT06 modes).
+// The reference's enum headers clash when two models share a translation unit, so this file
+// is compiled once per model: -DREF_ORD (ORd-CiPA), -DREF_TT (TT06), -DREF_O11 (ORd 2011).
 #if defined(REF_ORD)
 #include "ohara_rudy_cipa_v1_2017.hpp"
+#elif defined(REF_O11)
+#include "Ohara_Rudy_2011.hpp"
 #else
 #include "ten_tusscher_model_2006_IK1Ko_endo_units.hpp"
 #include "ten_tusscher_model_2006_IK1Ko_epi_units.hpp"
@@ -51,6 +61,16 @@ static CellML *make_tt(int ct)
   return c;
 }
 
+#endif
+#if defined(REF_O11)
+static Ohara_Rudy_2011 *make_o11(const double *rec)
+{
+  Ohara_Rudy_2011 *c = new Ohara_Rudy_2011();
+  double hill[14];
+  memcpy(hill, rec + 2, sizeof(hill));
+  c->initConsts(rec[0], rec[1], hill, rec[16] != 0.0);
+  return c;
+}
 #endif
 #if defined(REF_ORD)
 static ohara_rudy_cipa_v1_2017 *make_ord(const double *rec)
@@ -113,6 +133,30 @@ int main(int argc, char **argv)
       for (int i = 0; i < n; ++i) y[i] += h / 6.0 * (k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i]);
       if ((s + 1) % every == 0) put(out, y, n);
     }
+#endif
+#if defined(REF_O11)
+  } else if (mode == "o11_init") {
+    for (size_t k = 0; k + 17 <= in.size(); k += 17) {
+      Ohara_Rudy_2011 *c = make_o11(&in[k]);
+      put(out, c->CONSTANTS, 142); put(out, c->STATES, 44);
+      delete c;
+    }
+  } else if (mode == "o11_rhs") {
+    const size_t rl = 1 + 142 + 41;
+    Ohara_Rudy_2011 *c = new Ohara_Rudy_2011();
+    for (size_t k = 0; k + rl <= in.size(); k += rl) {
+      double Cc[142], S[44], R[44], A[198];
+      memcpy(Cc, &in[k + 1], sizeof(Cc));
+      memset(S, 0, sizeof(S));
+      memcpy(S, &in[k + 143], 41 * sizeof(double));
+      memset(R, 0, sizeof(R)); memset(A, 0, sizeof(A));
+      // twice on the same inputs: the copy reads ALGEBRAIC[P] in its INaK block
+      // (Ohara_Rudy_2011.cpp:1459) before it sets it (:1464); the second call sees this P
+      c->computeRates(in[k], Cc, R, S, A);
+      c->computeRates(in[k], Cc, R, S, A);
+      put(out, R, 41); put(out, A, 198);
+    }
+    delete c;
 #endif
 #if defined(REF_TT)
   } else if (mode == "tt_rhs") {

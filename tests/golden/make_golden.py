@@ -4,7 +4,8 @@ Runs the binaries that oracle/Makefile builds from the reference's CellML model 
 (target ``ref``, outputs in oracle/_ref/) on inputs drawn here with numpy, and stores
 inputs + outputs as small .npz fixtures.  Run from the repository root:
 
-    make -C oracle ref && python tests/golden/make_golden.py
+    make -C oracle ref && python tests/golden/make_golden.py          (every fixture)
+    make -C oracle ref && python tests/golden/make_golden.py ord11    (the ORd 2011 ones only)
 
 The fixtures pin the C oracle (oracle/*.c) and the HIP kernels; nothing at run time on
 the GPU reads the reference tree.
@@ -57,6 +58,57 @@ def drug_sets(rng):
     hb = np.zeros(19); hb[:14] = BEPRIDIL_ROW; hb[14:] = h2[14:]
     sets.append((33.0, hb))
     return sets
+
+
+# ORd 2011 ALGEBRAIC slots of the fixtures: the rate-law intermediates and currents the oracle
+# stores (gate steady states and time constants 0-48, INa 62, ICaL/ICaNa/ICaK 81-85, Jrel 86-93
+# and 191, IKr 96, IKs 98, IK1 100, INaCa 132/162, INaK 181, background and pump 183-188, Jup 196)
+# plus assp 34, tiFp 54, tiSp 55, INaL 64 and Ito 70, which the patch-clamp copy never completes
+ORD11_ASLOTS = np.array([0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 19, 20, 21, 22,
+                         23, 24, 25, 26, 27, 28, 30, 31, 32, 34, 35, 36, 37, 38, 40, 41, 42, 44, 46,
+                         48, 54, 55, 62, 64, 70, 81, 82, 85, 86, 87, 92, 93, 96, 98, 100, 132, 162,
+                         181, 183, 184, 186, 188, 191, 196])
+# DrugSimulation's values of the two constants the patch-clamp copy's ___initConsts leaves unset
+# (DrugSimulation/cellmodels/Ohara_Rudy_2011.cpp:538 PKNa, :600 KmCaAct)
+ORD11_PKNA, ORD11_KMCAACT = (30, 0.01833), (63, 150e-6)
+
+
+def ord11_golden():
+    """ORd 2011 from the reference's patch-clamp copy of the model (ref_o11): its initConsts
+    with the Hill block (endocardial, no Dutta scaling, which that copy applies to other slots
+    than DrugSimulation's), and computeRates on perturbed states of the shipped steady state,
+    with and without the stimulus."""
+    rng = np.random.default_rng(20251205)
+    q = np.loadtxt(os.path.join(OUT, "quinidine_IC50_samples10.csv"), delimiter=",", skiprows=1)
+    steady = np.loadtxt(os.path.join(OUT, "ord2011_output_orudy.dat"))
+    drugs = [(0.0, np.zeros(14))] + [(6474.0, q[k]) for k in range(len(q))]
+    drugs.append((9711.0, np.where(np.arange(14) >= 12, 0.0, q[8])))     # "NA" hERG pair
+    inp = np.array([np.concatenate([[0.0, c], h, [0.0]]) for c, h in drugs])
+    out = run("ref_o11", "o11_init", inp, 142 + 44)
+    np.savez_compressed(os.path.join(OUT, "ord11_init.npz"), inp=inp, C=out[:, :142])
+    recs, meta = [], []
+    times = (5.0, 10.2, 500.0, 1010.3)                 # stimulus off, on, off, on (second beat)
+    for d, (c, h) in enumerate(drugs):
+        Cc = out[d, :142].copy()
+        for k, v in (ORD11_PKNA, ORD11_KMCAACT):
+            Cc[k] = v
+        for j in range(6):
+            st = steady.copy()
+            if j:
+                st[0] = rng.uniform(-90.0, 40.0)
+                st[1:10] *= rng.uniform(0.8, 1.2, 9)
+                st[10:39] = rng.uniform(0.0, 1.0, 29)
+                st[33] = rng.uniform(0.0, 0.01)
+                st[39:41] = rng.uniform(0.0, 0.01, 2)
+            t = times[(d + j) % 4]
+            recs.append(np.concatenate([[t], Cc, st]))
+            meta.append(d)
+    recs = np.array(recs)
+    out = run("ref_o11", "o11_rhs", recs, 41 + 198)
+    np.savez_compressed(os.path.join(OUT, "ord11_rhs.npz"), t=recs[:, 0], C=recs[:, 1:143], S=recs[:, 143:],
+                        drug=np.array(meta), conc=np.array([drugs[d][0] for d in meta]),
+                        hill=np.array([drugs[d][1] for d in meta]), R=out[:, :41],
+                        A=out[:, 41:][:, ORD11_ASLOTS], aslots=ORD11_ASLOTS)
 
 
 def main():
@@ -131,4 +183,8 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["ord11"]:
+        ord11_golden()
+    else:
+        main()
+        ord11_golden()

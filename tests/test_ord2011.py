@@ -1,8 +1,12 @@
 """ORd 2011 with the DrugSimulation adaptive Rush-Larsen loop
 (DrugSimulation/cellmodels/Ohara_Rudy_2011.cpp, DrugSimulation/modules/drug_sim.cpp:306-420).
 
-The reference's copy of this model does not build here (its header includes curl), so the
-oracle is pinned by:
+The reference's DrugSimulation copy of this model does not build here (its header includes
+curl), so the oracle is pinned by:
+  * golden vectors of the reference's patch-clamp copy of the same model
+    (CEPPatch_CiPA_Sim/patches/Ohara_Rudy_2011.cpp, compiled by oracle/Makefile, fixtures by
+    tests/golden/make_golden.py): its constants with the Hill block, and its computeRates on
+    perturbed states of the shipped steady state, for everything that copy completes;
   * the reference's logged known answers: quinidine, M cell, no Dutta, BCL 2000 ms, 1000 paces
     from the shipped steady state, concentrations 3237/6474/9711 nM over the 10 IC50 samples of
     the shipped CSV: EADs (dVm/dt_repol > 0) in exactly sample 8 at 6474 nM and samples 0, 4, 7,
@@ -130,6 +134,72 @@ def test_ord11_time_step_rule(t, dv, expected):
     assert ts == pytest.approx(expected, rel=1e-12)
 
 
+# ---- golden vectors of the reference's patch-clamp copy of the model
+GI = np.load(os.path.join(G, "ord11_init.npz"))
+GR = np.load(os.path.join(G, "ord11_rhs.npz"))
+# constants that copy's ___initConsts never sets (PKNa, KmCaAct): the fixtures' right-hand sides
+# take DrugSimulation's values for them
+O11_UNSET_C = (30, 63)
+# what that copy never completes: the CaMK-phosphorylated Ito gates ap, iFp, iSp (no initial
+# values, no rates; assp 34, tiFp 54, tiSp 55 and ip never computed, so Ito 70 misses its
+# phosphorylated part) and fINaLp (never computed, so INaL 64 misses its phosphorylated part).
+# The rates of V, nai and ki carry those two currents.
+O11_INCOMPLETE_A = (34, 54, 55, 64, 70)
+O11_INCOMPLETE_R = (0, 3, 5, 22, 23, 24)
+O11_GKR = 101
+
+
+def _oracle_on_golden():
+    n = len(GR["t"])
+    R = np.zeros((n, O.ORD11_NS))
+    A = np.zeros((n, len(GR["aslots"])))
+    C0, _ = O.ord11_init(0.0)
+    for i in range(n):
+        Cc = C0.copy()
+        Cc[:142] = GR["C"][i]
+        r, a = O.ord11_rates(GR["t"][i], Cc, GR["S"][i])
+        R[i], A[i] = r, a[GR["aslots"]]
+    return R, A
+
+
+def test_ord11_oracle_constants_match_reference_golden():
+    """Endocardial constants with the Hill block of 12 drug records (control, the ten quinidine
+    samples, an "NA" hERG pair) equal the reference copy's, and the GKr factor is the Hill
+    closed form 1 / (1 + (D / IC50)^h)."""
+    keep = np.ones(142, dtype=bool)
+    keep[list(O11_UNSET_C)] = False
+    ctl = GI["C"][0]
+    for rec, Cref in zip(GI["inp"], GI["C"]):
+        C, _ = O.ord11_init(rec[0], rec[1], rec[2:16], bool(rec[16]))
+        np.testing.assert_allclose(C[:142][keep], Cref[keep], rtol=1e-15, atol=0)
+        ic50, h = rec[2 + 12], rec[2 + 13]
+        f = 1.0 / (1.0 + (rec[1] / ic50) ** h) if ic50 > 0 and h > 0 else 1.0
+        assert Cref[O11_GKR] == pytest.approx(ctl[O11_GKR] * f, rel=1e-14)
+    assert len({float(c) for c in GI["C"][:, O11_GKR]}) >= 10      # the block differs by sample
+
+
+def test_ord11_oracle_rhs_matches_reference_golden():
+    """The oracle's right-hand side on the golden records: every rate and stored current or
+    rate-law term the reference copy completes within 1e-12 of its column's largest magnitude
+    (the xrf / xrs IKr gates and IKr among them), and the V rate differs from the copy's by
+    exactly the two currents the copy leaves incomplete."""
+    R, A = _oracle_on_golden()
+    Rr, Ar, sl = GR["R"], GR["A"], list(GR["aslots"])
+    rcols = [k for k in range(O.ORD11_NS) if k not in O11_INCOMPLETE_R]
+    sr = np.maximum(np.abs(Rr[:, rcols]).max(axis=0), 1e-300)
+    assert (np.abs(R[:, rcols] - Rr[:, rcols]) / sr).max() <= 1e-12
+    acols = [j for j, k in enumerate(sl) if k not in O11_INCOMPLETE_A]
+    sa = np.maximum(np.abs(Ar[:, acols]).max(axis=0), 1e-300)
+    assert (np.abs(A[:, acols] - Ar[:, acols]) / sa).max() <= 1e-12
+    for k in (9, 24, 25, 96):                   # xrss, txrf, txrs, IKr
+        assert k in sl
+    assert {34, 35} <= set(rcols)               # RATES of xrf, xrs
+    j64, j70 = sl.index(64), sl.index(70)
+    dv = (R[:, 0] - Rr[:, 0]) + (A[:, j64] - Ar[:, j64]) + (A[:, j70] - Ar[:, j70])
+    assert np.abs(dv).max() <= 1e-12 * np.abs(Rr[:, 0]).max()
+    assert np.abs(A[:, j64] - Ar[:, j64]).max() > 0          # the copy's INaL does differ
+
+
 def test_ord11_hill_block_and_dutta_slots():
     ic50 = np.array([1000., 1.0, 0., 0., 2e3, 0.5, 5e3, 2.0, 800., 1.5, 3e3, 1.0, 400., 0.8])
     base, _ = O.ord11_init()
@@ -200,6 +270,23 @@ def test_ord11_rhs_matches_oracle(prec, tol, dutta):
             Rg[m] = b.compute_rates(t, S)[m]
     scale = np.maximum(np.abs(Ro).max(axis=0), 1e-30)
     assert (np.abs(Rg - Ro) / scale).max() <= tol
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec,tol", [(cep.PREC_FP64, 1e-12), (cep.PREC_MIXED, 2e-5)])
+def test_ord11_rhs_matches_reference_golden(prec, tol):
+    """The device right-hand side on the golden records of the reference's patch-clamp copy:
+    every rate that copy completes, against its column's largest magnitude."""
+    n = len(GR["t"])
+    with _gpu_batch(n, prec, GR["conc"], GR["hill"], np.full(n, 1000.0), 1, 1) as b:
+        Rg = np.zeros((n, O.ORD11_NS))
+        for t in np.unique(GR["t"]):
+            m = GR["t"] == t
+            Rg[m] = b.compute_rates(float(t), GR["S"])[m]
+    rcols = [k for k in range(O.ORD11_NS) if k not in O11_INCOMPLETE_R]
+    Rr = GR["R"][:, rcols]
+    scale = np.maximum(np.abs(Rr).max(axis=0), 1e-300)
+    assert (np.abs(Rg[:, rcols] - Rr) / scale).max() <= tol
 
 
 @pytest.mark.gpu
