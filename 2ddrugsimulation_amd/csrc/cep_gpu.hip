@@ -1526,6 +1526,7 @@ static void o11_constants(const O11CellIn &in, bool dutta, double stim_scale, do
   double p[ord11::NP];
   o11_params(in, dutta, stim_scale, p);
   C[12] = p[ord11::P_STIM_AMP];
+  C[42] = p[ord11::P_GNA];                     // GNa with the drug block
   C[93] = 0.05; C[94] = 1.0 - 0.99; C[95] = 3.0 * 200; C[96] = p[ord11::P_GNAL]; C[97] = p[ord11::P_GTO];
   C[98] = p[ord11::P_AFF]; C[99] = p[ord11::P_PCA]; C[100] = p[ord11::P_TJCA]; C[101] = p[ord11::P_GKR];
   C[102] = p[ord11::P_GKS]; C[103] = p[ord11::P_GK1]; C[104] = vcell; C[105] = 0.003; C[106] = a_rel;

@@ -276,17 +276,32 @@ def test_ord11_rhs_matches_oracle(prec, tol, dutta):
 @pytest.mark.parametrize("prec,tol", [(cep.PREC_FP64, 1e-12), (cep.PREC_MIXED, 2e-5)])
 def test_ord11_rhs_matches_reference_golden(prec, tol):
     """The device right-hand side on the golden records of the reference's patch-clamp copy:
-    every rate that copy completes, against its column's largest magnitude."""
+    every rate that copy completes and the currents the device exports, against each column's
+    largest magnitude, and the constants the library reports for the same cells."""
     n = len(GR["t"])
+    sl = list(GR["aslots"])
     with _gpu_batch(n, prec, GR["conc"], GR["hill"], np.full(n, 1000.0), 1, 1) as b:
         Rg = np.zeros((n, O.ORD11_NS))
+        Ag = np.zeros((n, len(sl)))
         for t in np.unique(GR["t"]):
             m = GR["t"] == t
-            Rg[m] = b.compute_rates(float(t), GR["S"])[m]
+            r, a = b.compute_rates(float(t), GR["S"], algebraic=True)
+            Rg[m], Ag[m] = r[m], a[m][:, sl]
+        # the constants the library reports (cep_get_constants) are the copy's initConsts
+        keep = np.ones(142, dtype=bool)
+        keep[list(O11_UNSET_C)] = False
+        for i in range(n):
+            np.testing.assert_allclose(b.constants(i)[:142][keep], GR["C"][i][keep], rtol=1e-15, atol=0)
     rcols = [k for k in range(O.ORD11_NS) if k not in O11_INCOMPLETE_R]
     Rr = GR["R"][:, rcols]
     scale = np.maximum(np.abs(Rr).max(axis=0), 1e-300)
     assert (np.abs(Rg[:, rcols] - Rr) / scale).max() <= tol
+    # the currents the device exports (Istim, INa, ICaL, IKr, IKs, IK1; INaL and Ito as well,
+    # which that copy leaves incomplete)
+    acols = [sl.index(k) for k in (12, 62, 81, 96, 98, 100)]
+    Ar = GR["A"][:, acols]
+    sa = np.maximum(np.abs(Ar).max(axis=0), 1e-300)
+    assert (np.abs(Ag[:, acols] - Ar) / sa).max() <= tol
 
 
 @pytest.mark.gpu
