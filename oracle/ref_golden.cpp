@@ -11,6 +11,7 @@
 //   ord_rk4   in: [celltype, conc, herg[19], t_end, h, every]      out: S[52] per sample
 //   tt_rhs    in: [celltype, t, S[19]] per record                  out: R[19] A[70]
 //   tt_rl     in: [celltype, t_end, dt, every]                      out: S[19] per sample
+//   tt_init   in: [celltype] per record                             out: C[53] S[19]
 //   o11_init  in: [celltype, conc, hill[14], dutta] per record      out: C[142] S[44]
 //   o11_rhs   in: [t, C[142], S[41]] per record                     out: R[41] A[198]
 // The ORd 2011 modes use the reference's patch-clamp copy of the model
@@ -159,6 +160,12 @@ int main(int argc, char **argv)
     delete c;
 #endif
 #if defined(REF_TT)
+  } else if (mode == "tt_init") {
+    for (size_t k = 0; k < in.size(); ++k) {
+      CellML *c = make_tt((int)in[k]);
+      put(out, c->CONSTANTS, 53); put(out, c->STATES, 19);
+      delete c;
+    }
   } else if (mode == "tt_rhs") {
     const size_t rl = 2 + 19;
     for (size_t k = 0; k + rl <= in.size(); k += rl) {

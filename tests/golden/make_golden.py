@@ -111,6 +111,12 @@ def ord11_golden():
                         A=out[:, 41:][:, ORD11_ASLOTS], aslots=ORD11_ASLOTS)
 
 
+def tt06_init_golden():
+    """TT06 endo / epi / M constants and initial states of the compiled reference models."""
+    out = run("ref_tt", "tt_init", np.array([0.0, 1.0, 2.0]), 53 + 19)
+    np.savez_compressed(os.path.join(OUT, "tt06_init.npz"), C=out[:, :53], S=out[:, 53:])
+
+
 def main():
     rng = np.random.default_rng(20251203)
     sets = drug_sets(rng)
@@ -177,6 +183,7 @@ def main():
     recs = np.array(recs)
     out = run("ref_tt", "tt_rhs", recs, 19 + 70)
     np.savez_compressed(os.path.join(OUT, "tt06_rhs.npz"), inp=recs, R=out[:, :19], A=out[:, 19:])
+    tt06_init_golden()
     for f in sorted(os.listdir(OUT)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(OUT, f)))
@@ -185,6 +192,8 @@ def main():
 if __name__ == "__main__":
     if sys.argv[1:] == ["ord11"]:
         ord11_golden()
+    elif sys.argv[1:] == ["tt06_init"]:
+        tt06_init_golden()
     else:
         main()
         ord11_golden()

@@ -289,6 +289,21 @@ def test_tissue_strang_step_matches_the_oracle_composition():
     assert gap > 1e3 * max(err, 1e-12)
 
 
+def test_get_constants_matches_the_reference_tt06_models():
+    """cep_get_constants for TT06 endo, epi and M cells (no drug, BCL 1000 ms) is the CONSTANTS
+    row of the compiled reference models' initConsts (tests/golden/tt06_init.npz)."""
+    g = np.load(f"{G}/tt06_init.npz")
+    with cep.CellBatch(cep.MODEL_TT06, 3, cep.PREC_FP64) as b:
+        b.set_cells(celltype=np.array([0.0, 1.0, 2.0]), conc=np.zeros(3), herg=np.zeros((3, 6)),
+                    gscale=np.ones((3, 10)), bcl=np.full(3, 1000.0))
+        b.set_run_config(dt=0.02)
+        b.init_consts()
+        Cb = np.array([b.constants(i) for i in range(3)])
+        S = b.states()
+    np.testing.assert_allclose(Cb, g["C"], rtol=1e-15, atol=0)
+    np.testing.assert_allclose(S, g["S"], rtol=1e-15, atol=0)
+
+
 def test_get_constants_matches_the_reference_model():
     """cep_get_constants rebuilds the CellML CONSTANTS row of every golden drug tuple: the compiled
     reference model's ___initConsts + initConsts (tests/golden/ord_init.npz)."""
