@@ -767,24 +767,35 @@ ORD11_REF_CELL_BEATS_PER_S = 30 * 1000 / 1048.7
 ORD11_EAD_LOGGED = {(8, 6474.0), (4, 9711.0), (9, 9711.0), (7, 9711.0), (0, 9711.0), (8, 9711.0)}
 
 
-def ord11_main(args):
-    """The reference's own ORd 2011 run, on one GPU, plus a batched population of the same
-    model for throughput at scale."""
+def ord11_inputs():
+    """The reference's fixtures of its ORd 2011 run: quinidine IC50 samples and steady state."""
     G = os.path.join(ROOT, "tests", "golden")
     q = np.loadtxt(os.path.join(G, "quinidine_IC50_samples10.csv"), delimiter=",", skiprows=1)
     s0 = np.loadtxt(os.path.join(G, "ord2011_output_orudy.dat"))
+    return q, s0
+
+
+def ord11_batch(ncell, conc, ic50, prec, nbeats, s0):
+    """ORd 2011 M cells at BCL 2000 ms from the shipped steady state, as the reference's run."""
+    b = cep.CellBatch(cep.MODEL_ORD2011, ncell, prec)
+    b.set_cells(celltype=np.full(ncell, 2.0), conc=conc, herg=ic50, bcl=np.full(ncell, 2000.0))
+    b.set_run_config(dt=1.0, nbeats_total=nbeats, nwin=min(250, nbeats))
+    b.init_consts()
+    b.set_states(np.repeat(s0[None, :], ncell, axis=0))
+    return b
+
+
+def ord11_main(args):
+    """The reference's own ORd 2011 run, on one GPU, plus a batched population of the same
+    model for throughput at scale."""
+    q, s0 = ord11_inputs()
     nbeats = args.steps if args.steps is not None else 1000
     keys = [(i, c) for c in (3237.0, 6474.0, 9711.0) for i in range(len(q))]
     n = len(keys)
     prec = cep.PREC_FP64 if (args.precision or "fp64") == "fp64" else cep.PREC_MIXED
 
     def batch(ncell, conc, ic50):
-        b = cep.CellBatch(cep.MODEL_ORD2011, ncell, prec)
-        b.set_cells(celltype=np.full(ncell, 2.0), conc=conc, herg=ic50, bcl=np.full(ncell, 2000.0))
-        b.set_run_config(dt=1.0, nbeats_total=nbeats, nwin=min(250, nbeats))
-        b.init_consts()
-        b.set_states(np.repeat(s0[None, :], ncell, axis=0))
-        return b
+        return ord11_batch(ncell, conc, ic50, prec, nbeats, s0)
 
     log(f"bench: ORd 2011 quinidine run, {n} cells x {nbeats} beats")
     b = batch(n, np.array([c for _, c in keys]), np.vstack([q[i] for i, _ in keys]))
@@ -1035,10 +1046,40 @@ def main():
 
 
 def compact_legs(args):
-    """Short runs of the two other configurations, timed by the same driver run: one beat of the
-    200,000-cell TT06 population (mixed) and 100 Strang steps of the 512 x 512 tissue (FP32 rates,
-    FP64 state and CG; its Vm against an FP64 run of the same steps)."""
+    """Short runs of the other configurations, timed by the same driver run: one beat of the
+    200,000-cell TT06 population (mixed), 100 Strang steps of the 512 x 512 tissue (FP32 rates,
+    FP64 state and CG; its Vm against an FP64 run of the same steps), and the first 100 of the
+    1000 beats of the reference's ORd 2011 quinidine run (FP64) with two beats of a
+    100,000-cell population of the same model."""
     legs = {}
+    q, s0 = ord11_inputs()
+    keys = [(i, c) for c in (3237.0, 6474.0, 9711.0) for i in range(len(q))]
+    nb = 100
+    log(f"bench: ORd 2011 leg, {len(keys)} cells x {nb} beats + 100000 cells x 2 beats")
+    b = ord11_batch(len(keys), np.array([c for _, c in keys]), np.vstack([q[i] for i, _ in keys]),
+                    cep.PREC_FP64, 1000, s0)
+    t0 = time.perf_counter()
+    b.run_beats(nb)
+    t1 = time.perf_counter()
+    bad = int(np.sum(b.status() != 0))
+    b.close()
+    rng = np.random.default_rng(20251208)
+    npop = 100000
+    bp = ord11_batch(npop, rng.uniform(0.0, 4 * 9711.0, npop), q[rng.integers(0, len(q), npop)],
+                     cep.PREC_FP64, 2, s0)
+    bp.kernel_time()
+    t2 = time.perf_counter()
+    bp.run_beats(2)
+    t3 = time.perf_counter()
+    _, pavg, _ = bp.kernel_time()
+    bad += int(np.sum(bp.status() != 0))
+    bp.close()
+    legs["ord2011"] = {"workload": f"ORd 2011 M cell, quinidine IC50_samples10 x {{3237, 6474, 9711}} nM, BCL 2000, "
+                                   f"FP64, the first {nb} of the reference run's 1000 beats",
+                       "cell_beats_per_s": len(keys) * nb / (t1 - t0),
+                       "reference_cell_beats_per_s": ORD11_REF_CELL_BEATS_PER_S,
+                       "population_cell_beats_per_s": npop * 2 / (t3 - t2), "population_kernel_avg_ms": pavg,
+                       "failed_cells": bad}
     pop = cep.workload.tt_population(20251205, args.tt_cells)
     herg = np.concatenate([pop["ic50"], pop["hill"]], axis=1)
     n = args.tt_cells
