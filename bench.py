@@ -639,7 +639,7 @@ def tune_main(args):
     pop = make_population(10000)
     lay = []
     ref = None
-    for spec in ("", "2000:12", "2000:8", "2000:6", "2000:8,1000:12"):
+    for spec in ("", "2000:12", "2000:8", "2000:7", "2000:6", "2000:8,1000:12"):
         pg = parse_group_cells(spec)
         b, pos = make_slot_batch(pop, cep.PREC_MIXED, 200, pg)
         b.run_beats(1)
@@ -887,7 +887,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--tissue-nx", type=int, default=TISSUE_NX)
     ap.add_argument("--ncell", type=int, default=10000)
-    ap.add_argument("--group-cells", default="2000:8",
+    ap.add_argument("--group-cells", default="2000:7",
                     help="population: cells per 16-slot BDF wavefront by BCL ('bcl:cells,...'; '' = 16 everywhere)")
     ap.add_argument("--precision", choices=("mixed", "fp64"), default=None,
                     help="default mixed (FP32 rates; FP64 state, BDF/LU, CG)")
